@@ -1,0 +1,11 @@
+/*
+ * levmar.h -- Levenberg-Marquardt driver, same signature as PSBA/levmar.h:6-11.
+ * Uses and advances the caller-defined global `int itno` (PSBA/main.cpp:27).
+ */
+#pragma once
+
+#include "cl_psba.h"
+#include "psba.h"
+
+int levmar(PSBA_structPtr psbaPtr, int cnp, int pnp, int mnp, int n3Dpts, int nCams, int n2Dprojs,
+           dtype* finalErr);

@@ -1,0 +1,162 @@
+/*
+ * psba_capi.h -- C-ABI of the MI355X bundle adjuster (libpsba.so).
+ *
+ * Plain pointers and sizes, int status codes (0 = OK, negative = error), no
+ * exit().  One context owns the device state of one problem on one HIP device
+ * and stream; contexts are independent.  The caller owns all host memory; the
+ * context copies what it needs.  The C++ drop-in layer (cl_psba.h, levmar.h,
+ * trust_region.h, misc.h) is implemented on top of these entry points.
+ *
+ * Layout conventions (identical to PSBA):
+ *   observations point-major, camera-ascending within a point
+ *     (PSBA/misc.cpp:189-216);
+ *   cams: cnp doubles per camera, pose last [... v1 v2 v3 t1 t2 t3]
+ *     cnp=6  [v t]                         (PSBA/CL_files/PSBA.cl:5 cnp=6)
+ *     cnp=9  [f k1 k2 v t]
+ *     cnp=11 [fx cx cy ar skew v t]
+ *     cnp=16 [fx cx cy ar skew kc1..kc5 v t]
+ *   K: 5 doubles per camera [fx cx cy ar skew] (PSBA/psba.h:3 K_DIM);
+ *   initrot: unit quaternion (s,x,y,z) per camera (PSBA/cl_psba.h:20);
+ *   impts: 2 doubles per observation (PSBA/cl_psba.h:19).
+ */
+#ifndef PSBA_CAPI_H
+#define PSBA_CAPI_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* status codes */
+#define PSBA_OK 0
+#define PSBA_ERR_ARG (-1)
+#define PSBA_ERR_HIP (-2)
+#define PSBA_ERR_NOMEM (-3)
+#define PSBA_ERR_STATE (-4)
+#define PSBA_ERR_UNSUPPORTED (-5)
+#define PSBA_ERR_NODEVICE (-6)
+
+/* driver return codes, PSBA/psba.h:12-18 */
+#define PSBA_ITER_TURN_TO_LM 1
+#define PSBA_ITER_TURN_TO_TR 2
+#define PSBA_ITER_CONTINUE 3
+#define PSBA_ITER_ERR 4
+#define PSBA_ITER_DP_NO_CHANGE 5
+#define PSBA_ITER_ERR_SMALL_ENOUGH 6
+#define PSBA_ITER_PASS 7
+
+/* arithmetic of the Jacobian / normal-equation / Schur stages */
+#define PSBA_PREC_FP64 0  /* plain path: everything in FP64 */
+#define PSBA_PREC_MIXED 1 /* FP32 Jacobian/W/V/Schur, FP64 factor, cost and refinement */
+
+typedef struct psba_ctx psba_ctx;
+
+typedef struct {
+  int cnp, pnp, mnp;        /* camera params (6|9|11|16), 3, 2 */
+  int n_cams, n_pts, n_obs; /* nCams, n3Dpts, n2Dprojs */
+  int precision;            /* PSBA_PREC_* */
+  int device;               /* HIP device ordinal */
+} psba_desc;
+
+typedef struct {
+  int max_iter;      /* shared iteration cap (reference: 50) */
+  int max_pb_tries;  /* guard on the dogleg's factor-retry loop (0 = 64) */
+  int refine_steps;  /* mixed precision: FP64 refinement sweeps (0 = default) */
+  int reserved;
+} psba_opts;
+
+/* one record per LM trial or dogleg step (same fields as the oracle's) */
+typedef struct {
+  int itno, kind, factor_ok, accepted;
+  double cost, damp, rho, lambda, dp_sq;
+} psba_trace_rec;
+
+/* ---- lifecycle ------------------------------------------------------------ */
+/* replaces setup_cl, PSBA/cl_psba.h:93 (PSBA/cl_psba.cpp:16-133) */
+int psba_create(const psba_desc* desc, psba_ctx** out);
+/* replaces release_buffer, PSBA/cl_psba.h:94 */
+int psba_destroy(psba_ctx* ctx);
+/* replaces fill_initBuffer2, PSBA/cl_psba.h:114-121 */
+int psba_upload(psba_ctx* ctx, const double* K, const double* impts, const double* initrot,
+                const double* cams, const double* pts);
+/* replaces fill_idxBuffer, PSBA/cl_psba.h:123-126: the device index (point CSR,
+ * camera CSR, camera-pair lists) is built from the observation order */
+int psba_set_index(psba_ctx* ctx, const int* iidx, const int* jidx);
+/* addition (no reference counterpart): hold parameters fixed; mask has
+ * cnp*n_cams bytes, 1 = fixed.  NULL clears the mask. */
+int psba_set_fixed(psba_ctx* ctx, const unsigned char* mask);
+/* accessor for the optimised parameters (the reference leaves them in
+ * cams_buffer / pts3D_buffer, PSBA/cl_psba.h:21-22) */
+int psba_read_params(psba_ctx* ctx, double* cams, double* pts);
+int psba_write_params(psba_ctx* ctx, const double* cams, const double* pts);
+const char* psba_strerror(int code);
+const char* psba_last_error(void);
+int psba_device_count(void);
+
+/* ---- host integer indexing ------------------------------------------------- */
+/* replaces generate_idxs, PSBA/misc.h:22-23 (PSBA/misc.cpp:178-218); bit-exact
+ * restatement.  comm3DIdx may be NULL (it needs nCams*nCams*n3Dpts ints). */
+int psba_generate_idxs(int nCams, int n3Dpts, int n2Dprojs, const char* vmask, int* comm3DIdx,
+                       int* comm3DIdxCnt, int* iidx, int* jidx, int* blk_idx);
+
+/* ---- drivers ---------------------------------------------------------------- */
+/* replaces levmar, PSBA/levmar.h:6-11.  *itno is the shared iteration counter
+ * (the reference's global itno, PSBA/main.cpp:27); *finalErr receives the
+ * final sum of squared residuals.  Returns a PSBA_ITER_* code (> 0) or an error
+ * (< 0).  trace may be NULL. */
+int psba_levmar(psba_ctx* ctx, const psba_opts* opts, int* itno, double* finalErr,
+                psba_trace_rec* trace, int trace_cap, int* trace_len);
+/* replaces trust_region, PSBA/trust_region.h:5-11 (its blk_idx argument is
+ * unused by the reference and absent here) */
+int psba_trust_region(psba_ctx* ctx, const psba_opts* opts, int* itno, double* finalErr,
+                      psba_trace_rec* trace, int trace_cap, int* trace_len);
+/* the LM <-> dogleg hand-off loop of PSBA/main.cpp:193-208 */
+int psba_hybrid(psba_ctx* ctx, const psba_opts* opts, int* itno, double* initErr,
+                double* finalErr, psba_trace_rec* trace, int trace_cap, int* trace_len);
+
+/* ---- stage entry points (PSBA/sba_func.h:10-138) ---------------------------- */
+/* residual and its squared norm at the current parameters (use_trial = 0) or
+ * at p + dp (use_trial = 1); PSBA/sba_func.cpp:81-145 */
+int psba_stage_residual(psba_ctx* ctx, int use_trial, double* cost);
+/* Jacobian + U/V/W/g with the reference's coefficients (LM: 1,1; dogleg: 2,-2);
+ * PSBA/sba_func.cpp:153-617 */
+int psba_stage_jacobian_normal(psba_ctx* ctx, double coeff_uvw, double coeff_g);
+/* damped V^-1, S = U + mu I - sum W V^-1 W^T and ea; PSBA/sba_func.cpp:624-995 */
+int psba_stage_schur(psba_ctx* ctx, double mu);
+/* Cholesky of S and the two triangular solves for dpa; *ok = 1 on success;
+ * replaces SPDinv + matVec_mul, PSBA/cl_spdinv.h + PSBA/cl_linearalg.h */
+int psba_stage_factor_solve(psba_ctx* ctx, int* ok);
+/* eb = gb - W^T dpa and dpb = V^-1 eb; PSBA/sba_func.cpp:1001-1117 */
+int psba_stage_backsub(psba_ctx* ctx);
+/* modified Cholesky of the current S, returns lambda = |sum E| / n;
+ * PSBA/cl_cholmod.h (cholmod_blk, compute_cholmod_E) */
+int psba_stage_modchol(psba_ctx* ctx, double* lambda);
+
+/* device buffer read-back for stage parity tests */
+#define PSBA_BUF_EX 0      /* 2*n_obs residual */
+#define PSBA_BUF_JA 1      /* 2*cnp*n_obs */
+#define PSBA_BUF_JB 2      /* 6*n_obs */
+#define PSBA_BUF_U 3       /* cnp*cnp*n_cams */
+#define PSBA_BUF_V 4       /* 9*n_pts */
+#define PSBA_BUF_W 5       /* 3*cnp*n_obs */
+#define PSBA_BUF_G 6       /* n_tot */
+#define PSBA_BUF_VINV 7    /* 9*n_pts */
+#define PSBA_BUF_S 9       /* n*n, lower triangle valid (n = cnp*n_cams) */
+#define PSBA_BUF_EA 10     /* n */
+#define PSBA_BUF_DP 11     /* n_tot */
+#define PSBA_BUF_EB 12     /* 3*n_pts */
+#define PSBA_BUF_L 13      /* n*n lower factor */
+long psba_buffer_size(const psba_ctx* ctx, int which);
+int psba_read_buffer(psba_ctx* ctx, int which, double* out, long n);
+
+/* per-stage device timings of the last driver call (milliseconds) */
+typedef struct {
+  double jacobian_normal, schur, factor, solve, backsub, residual, other;
+  int trials, jacobians;
+} psba_timing;
+int psba_get_timing(psba_ctx* ctx, psba_timing* out);
+int psba_enable_timing(psba_ctx* ctx, int on);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

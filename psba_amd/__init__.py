@@ -1,0 +1,330 @@
+"""MI355X bundle adjuster: Python view of the C-ABI in include/psba_capi.h.
+
+The compute path is libpsba.so (hand-written HIP for gfx950, built by
+``python -m psba_amd.build``).  This module only marshals numpy arrays through
+ctypes; it has no compute fallback: if the library is missing, importing the
+solver classes raises.
+
+Mirrors the reference's entry points (PSBA/levmar.h, PSBA/trust_region.h,
+PSBA/cl_psba.h): ``Context.levmar`` / ``Context.trust_region`` /
+``Context.hybrid`` with the shared iteration counter ``itno`` and the ITER_*
+return codes of PSBA/psba.h.
+"""
+import ctypes as ct
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libpsba.so")
+
+# PSBA/psba.h:12-18
+ITER_TURN_TO_LM = 1
+ITER_TURN_TO_TR = 2
+ITER_CONTINUE = 3
+ITER_ERR = 4
+ITER_DP_NO_CHANGE = 5
+ITER_ERR_SMALL_ENOUGH = 6
+ITER_PASS = 7
+
+PREC_FP64 = 0
+PREC_MIXED = 1
+
+BUF = dict(EX=0, JA=1, JB=2, U=3, V=4, W=5, G=6, VINV=7, S=9, EA=10, DP=11, EB=12, L=13)
+
+
+class PsbaError(RuntimeError):
+    pass
+
+
+class Desc(ct.Structure):
+    _fields_ = [("cnp", ct.c_int), ("pnp", ct.c_int), ("mnp", ct.c_int), ("n_cams", ct.c_int),
+                ("n_pts", ct.c_int), ("n_obs", ct.c_int), ("precision", ct.c_int), ("device", ct.c_int)]
+
+
+class Opts(ct.Structure):
+    _fields_ = [("max_iter", ct.c_int), ("max_pb_tries", ct.c_int), ("refine_steps", ct.c_int),
+                ("reserved", ct.c_int)]
+
+
+class TraceRec(ct.Structure):
+    _fields_ = [("itno", ct.c_int), ("kind", ct.c_int), ("factor_ok", ct.c_int), ("accepted", ct.c_int),
+                ("cost", ct.c_double), ("damp", ct.c_double), ("rho", ct.c_double), ("lambda_", ct.c_double),
+                ("dp_sq", ct.c_double)]
+
+
+class Timing(ct.Structure):
+    _fields_ = [("jacobian_normal", ct.c_double), ("schur", ct.c_double), ("factor", ct.c_double),
+                ("solve", ct.c_double), ("backsub", ct.c_double), ("residual", ct.c_double),
+                ("other", ct.c_double), ("trials", ct.c_int), ("jacobians", ct.c_int)]
+
+
+class _SceneC(ct.Structure):
+    _fields_ = [("cnp", ct.c_int), ("n_cams", ct.c_int), ("n_pts", ct.c_int), ("n_obs", ct.c_int),
+                ("K", ct.POINTER(ct.c_double)), ("q0", ct.POINTER(ct.c_double)),
+                ("cams", ct.POINTER(ct.c_double)), ("pts", ct.POINTER(ct.c_double)),
+                ("meas", ct.POINTER(ct.c_double)), ("iidx", ct.POINTER(ct.c_int)),
+                ("jidx", ct.POINTER(ct.c_int)), ("fixmask", ct.POINTER(ct.c_ubyte)),
+                ("gt_q0", ct.POINTER(ct.c_double)), ("gt_cams", ct.POINTER(ct.c_double)),
+                ("gt_pts", ct.POINTER(ct.c_double)), ("sum_t2", ct.c_double), ("n_outliers", ct.c_int),
+                ("zipf_s", ct.c_double)]
+
+
+class SceneOpts(ct.Structure):
+    _fields_ = [("cnp", ct.c_int), ("n_cams", ct.c_int), ("n_pts", ct.c_int), ("zipf_s", ct.c_double),
+                ("noise", ct.c_double), ("outlier_frac", ct.c_double), ("perturb", ct.c_double),
+                ("size_class", ct.c_int), ("problem", ct.c_ulonglong)]
+
+
+_lib = None
+
+
+def lib():
+    """Load libpsba.so (raises if it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise PsbaError("libpsba.so not built; run `python -m psba_amd.build`")
+    L = ct.CDLL(LIB_PATH)
+    vp, dp, ip = ct.c_void_p, ct.POINTER(ct.c_double), ct.POINTER(ct.c_int)
+    sig = {
+        "psba_create": (ct.c_int, [ct.POINTER(Desc), ct.POINTER(vp)]),
+        "psba_destroy": (ct.c_int, [vp]),
+        "psba_upload": (ct.c_int, [vp, dp, dp, dp, dp, dp]),
+        "psba_set_index": (ct.c_int, [vp, ip, ip]),
+        "psba_set_fixed": (ct.c_int, [vp, ct.POINTER(ct.c_ubyte)]),
+        "psba_read_params": (ct.c_int, [vp, dp, dp]),
+        "psba_write_params": (ct.c_int, [vp, dp, dp]),
+        "psba_strerror": (ct.c_char_p, [ct.c_int]),
+        "psba_last_error": (ct.c_char_p, []),
+        "psba_device_count": (ct.c_int, []),
+        "psba_generate_idxs": (ct.c_int, [ct.c_int, ct.c_int, ct.c_int, ct.c_char_p, ip, ip, ip, ip, ip]),
+        "psba_levmar": (ct.c_int, [vp, ct.POINTER(Opts), ip, dp, ct.POINTER(TraceRec), ct.c_int, ip]),
+        "psba_trust_region": (ct.c_int, [vp, ct.POINTER(Opts), ip, dp, ct.POINTER(TraceRec), ct.c_int, ip]),
+        "psba_hybrid": (ct.c_int, [vp, ct.POINTER(Opts), ip, dp, dp, ct.POINTER(TraceRec), ct.c_int, ip]),
+        "psba_stage_residual": (ct.c_int, [vp, ct.c_int, dp]),
+        "psba_stage_jacobian_normal": (ct.c_int, [vp, ct.c_double, ct.c_double]),
+        "psba_stage_schur": (ct.c_int, [vp, ct.c_double]),
+        "psba_stage_factor_solve": (ct.c_int, [vp, ip]),
+        "psba_stage_backsub": (ct.c_int, [vp]),
+        "psba_stage_modchol": (ct.c_int, [vp, dp]),
+        "psba_buffer_size": (ct.c_long, [vp, ct.c_int]),
+        "psba_read_buffer": (ct.c_int, [vp, ct.c_int, dp, ct.c_long]),
+        "psba_get_timing": (ct.c_int, [vp, ct.POINTER(Timing)]),
+        "psba_enable_timing": (ct.c_int, [vp, ct.c_int]),
+        "psba_scene_generate": (ct.POINTER(_SceneC), [ct.c_int, ct.c_ulonglong, ct.POINTER(SceneOpts)]),
+        "psba_scene_free": (None, [ct.POINTER(_SceneC)]),
+        "psba_problem_seed": (ct.c_ulonglong, [ct.c_ulonglong, ct.c_ulonglong]),
+    }
+    for name, (res, args) in sig.items():
+        f = getattr(L, name)
+        f.restype = res
+        f.argtypes = args
+    _lib = L
+    return L
+
+
+def _dptr(a):
+    return a.ctypes.data_as(ct.POINTER(ct.c_double))
+
+
+def _iptr(a):
+    return a.ctypes.data_as(ct.POINTER(ct.c_int))
+
+
+def _check(rc, what):
+    if rc < 0:
+        L = lib()
+        raise PsbaError("%s: %s (%s)" % (what, L.psba_strerror(rc).decode(), L.psba_last_error().decode()))
+    return rc
+
+
+def device_count():
+    return lib().psba_device_count()
+
+
+class Scene:
+    """A seeded synthetic scene (include/psba_scene.h); all arrays are numpy copies."""
+
+    def __init__(self, config, seed, **kw):
+        o = SceneOpts(noise=-1.0, outlier_frac=-1.0, perturb=-1.0)
+        for k, v in kw.items():
+            setattr(o, k, v)
+        p = lib().psba_scene_generate(int(config), ct.c_ulonglong(seed), ct.byref(o))
+        if not p:
+            raise PsbaError("scene generation failed")
+        s = p.contents
+        C, nc, npt, no = s.cnp, s.n_cams, s.n_pts, s.n_obs
+
+        def arr(ptr, n, dt):
+            return np.ctypeslib.as_array(ptr, shape=(n,)).astype(dt, copy=True)
+
+        self.config, self.seed = config, seed
+        self.cnp, self.n_cams, self.n_pts, self.n_obs = C, nc, npt, no
+        self.K = arr(s.K, 5 * nc, np.float64)
+        self.q0 = arr(s.q0, 4 * nc, np.float64)
+        self.cams = arr(s.cams, C * nc, np.float64)
+        self.pts = arr(s.pts, 3 * npt, np.float64)
+        self.meas = arr(s.meas, 2 * no, np.float64)
+        self.iidx = arr(s.iidx, no, np.int32)
+        self.jidx = arr(s.jidx, no, np.int32)
+        self.fixmask = arr(s.fixmask, C * nc, np.uint8)
+        self.gt_q0 = arr(s.gt_q0, 4 * nc, np.float64)
+        self.gt_cams = arr(s.gt_cams, C * nc, np.float64)
+        self.gt_pts = arr(s.gt_pts, 3 * npt, np.float64)
+        self.sum_t2, self.n_outliers, self.zipf_s = s.sum_t2, s.n_outliers, s.zipf_s
+        lib().psba_scene_free(p)
+
+
+class Context:
+    """Device state of one problem (the HIP-backed PSBA_struct)."""
+
+    def __init__(self, cnp, n_cams, n_pts, n_obs, precision=PREC_FP64, device=0):
+        L = lib()
+        self._h = ct.c_void_p()
+        d = Desc(cnp, 3, 2, n_cams, n_pts, n_obs, precision, device)
+        _check(L.psba_create(ct.byref(d), ct.byref(self._h)), "psba_create")
+        self.cnp, self.n_cams, self.n_pts, self.n_obs = cnp, n_cams, n_pts, n_obs
+        self.itno = 0
+
+    @classmethod
+    def from_scene(cls, sc, fix=True, **kw):
+        c = cls(sc.cnp, sc.n_cams, sc.n_pts, sc.n_obs, **kw)
+        c.upload(sc.K, sc.meas, sc.q0, sc.cams, sc.pts)
+        c.set_index(sc.iidx, sc.jidx)
+        if fix:
+            c.set_fixed(sc.fixmask)
+        return c
+
+    def close(self):
+        if self._h:
+            lib().psba_destroy(self._h)
+            self._h = ct.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def upload(self, K, impts, initrot, cams, pts):
+        self._keep = [np.ascontiguousarray(x, dtype=np.float64) for x in (K, impts, initrot, cams, pts)]
+        _check(lib().psba_upload(self._h, *[_dptr(x) for x in self._keep]), "psba_upload")
+
+    def set_index(self, iidx, jidx):
+        i = np.ascontiguousarray(iidx, dtype=np.int32)
+        j = np.ascontiguousarray(jidx, dtype=np.int32)
+        _check(lib().psba_set_index(self._h, _iptr(i), _iptr(j)), "psba_set_index")
+
+    def set_fixed(self, mask):
+        if mask is None:
+            _check(lib().psba_set_fixed(self._h, None), "psba_set_fixed")
+            return
+        m = np.ascontiguousarray(mask, dtype=np.uint8)
+        _check(lib().psba_set_fixed(self._h, m.ctypes.data_as(ct.POINTER(ct.c_ubyte))), "psba_set_fixed")
+
+    def read_params(self):
+        cams = np.empty(self.cnp * self.n_cams)
+        pts = np.empty(3 * self.n_pts)
+        _check(lib().psba_read_params(self._h, _dptr(cams), _dptr(pts)), "psba_read_params")
+        return cams, pts
+
+    def write_params(self, cams, pts):
+        c = np.ascontiguousarray(cams, dtype=np.float64)
+        p = np.ascontiguousarray(pts, dtype=np.float64)
+        _check(lib().psba_write_params(self._h, _dptr(c), _dptr(p)), "psba_write_params")
+
+    def _opts(self, max_iter=50, max_pb_tries=0):
+        return Opts(max_iter, max_pb_tries, 0, 0)
+
+    def _run(self, fn, trace_cap, opts, extra_init=False):
+        tr = (TraceRec * max(trace_cap, 1))()
+        n = ct.c_int(0)
+        it = ct.c_int(self.itno)
+        fe = ct.c_double(0.0)
+        if extra_init:
+            ie = ct.c_double(0.0)
+            rc = fn(self._h, ct.byref(opts), ct.byref(it), ct.byref(ie), ct.byref(fe), tr, trace_cap, ct.byref(n))
+        else:
+            rc = fn(self._h, ct.byref(opts), ct.byref(it), ct.byref(fe), tr, trace_cap, ct.byref(n))
+        _check(rc, fn.__name__)
+        self.itno = it.value
+        recs = [tr[k] for k in range(min(n.value, trace_cap))]
+        out = dict(code=rc, itno=it.value, final_err=fe.value, trace=recs)
+        if extra_init:
+            out["init_err"] = ie.value
+        return out
+
+    def levmar(self, max_iter=50, trace_cap=4096):
+        """PSBA/levmar.h:6-11 on the device; returns code, itno, final_err, trace."""
+        return self._run(lib().psba_levmar, trace_cap, self._opts(max_iter))
+
+    def trust_region(self, max_iter=50, trace_cap=4096, max_pb_tries=0):
+        """PSBA/trust_region.h:5-11 on the device."""
+        return self._run(lib().psba_trust_region, trace_cap, self._opts(max_iter, max_pb_tries))
+
+    def hybrid(self, max_iter=50, trace_cap=4096):
+        """The LM <-> dogleg loop of PSBA/main.cpp:193-208."""
+        return self._run(lib().psba_hybrid, trace_cap, self._opts(max_iter), extra_init=True)
+
+    # ---- stages ----
+    def residual(self, trial=False):
+        c = ct.c_double()
+        _check(lib().psba_stage_residual(self._h, int(trial), ct.byref(c)), "residual")
+        return c.value
+
+    def jacobian_normal(self, coeff=1.0, coeff_g=1.0):
+        _check(lib().psba_stage_jacobian_normal(self._h, coeff, coeff_g), "jacobian_normal")
+
+    def schur(self, mu):
+        _check(lib().psba_stage_schur(self._h, mu), "schur")
+
+    def factor_solve(self):
+        ok = ct.c_int()
+        _check(lib().psba_stage_factor_solve(self._h, ct.byref(ok)), "factor_solve")
+        return bool(ok.value)
+
+    def backsub(self):
+        _check(lib().psba_stage_backsub(self._h), "backsub")
+
+    def modchol(self):
+        lam = ct.c_double()
+        _check(lib().psba_stage_modchol(self._h, ct.byref(lam)), "modchol")
+        return lam.value
+
+    def buffer(self, name):
+        w = BUF[name]
+        n = lib().psba_buffer_size(self._h, w)
+        out = np.empty(n)
+        _check(lib().psba_read_buffer(self._h, w, _dptr(out), n), "read_buffer")
+        return out
+
+    def enable_timing(self, on=True):
+        _check(lib().psba_enable_timing(self._h, int(on)), "enable_timing")
+
+    def timing(self):
+        t = Timing()
+        _check(lib().psba_get_timing(self._h, ct.byref(t)), "get_timing")
+        return {k: getattr(t, k) for k, _ in Timing._fields_}
+
+
+def generate_idxs(n_cams, n_pts, n_obs, vmask, with_comm=False):
+    """PSBA/misc.cpp:178-218 through the C-ABI; returns (iidx, jidx, blk_idx, cnt, comm|None)."""
+    vm = np.ascontiguousarray(vmask, dtype=np.int8).reshape(-1)
+    iidx = np.empty(n_obs, np.int32)
+    jidx = np.empty(n_obs, np.int32)
+    blk = np.empty(n_pts * n_cams, np.int32)
+    cnt = np.empty(n_cams * n_cams, np.int32)
+    comm = np.full(n_cams * n_cams * n_pts, -7, np.int32) if with_comm else None
+    rc = lib().psba_generate_idxs(n_cams, n_pts, n_obs, vm.ctypes.data_as(ct.c_char_p),
+                                  _iptr(comm) if comm is not None else None, _iptr(cnt), _iptr(iidx),
+                                  _iptr(jidx), _iptr(blk))
+    _check(rc, "generate_idxs")
+    return iidx, jidx, blk, cnt, comm

@@ -1,0 +1,90 @@
+"""Build the HIP library (libpsba.so, gfx950) and the CPU oracle.
+
+    python -m psba_amd.build            # everything
+    python -m psba_amd.build --lib      # product library only
+
+Objects go to psba_amd/_build/; the shared libraries are written in-tree
+(psba_amd/libpsba.so, oracle/liboracle.so) so they travel with the tree.
+"""
+import argparse
+import concurrent.futures as cf
+import os
+import shutil
+import subprocess
+import sys
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+CSRC = os.path.join(ROOT, "psba_amd", "csrc")
+BUILD = os.path.join(ROOT, "psba_amd", "_build")
+LIB = os.path.join(ROOT, "psba_amd", "libpsba.so")
+ORACLE_DIR = os.path.join(ROOT, "oracle")
+ORACLE_LIB = os.path.join(ORACLE_DIR, "liboracle.so")
+
+ARCH = os.environ.get("PSBA_OFFLOAD_ARCH", "gfx950")
+HIPCC = os.environ.get("HIPCC", shutil.which("hipcc") or "/opt/rocm/bin/hipcc")
+
+SOURCES = ["ctx.cpp", "drivers.cpp", "dropin.cpp", "scene.cpp",
+           "k_obs.hip", "k_schur.hip", "k_dense.hip", "k_tr.hip"]
+
+HIP_FLAGS = [
+    "-x", "hip", f"--offload-arch={ARCH}", "-O3", "-fPIC", "-std=c++17",
+    "-Wall", "-Wno-unused-function",
+    # packed FP32 instructions stay off (they are produced only by the compiler)
+    "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    "-I" + os.path.join(ROOT, "include"), "-I" + CSRC,
+]
+
+
+def _run(cmd):
+    r = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
+    out = "\n".join(l for l in r.stdout.splitlines()
+                    if "packed-fp32-ops" not in l and "not a recognized feature" not in l)
+    if r.returncode != 0:
+        raise RuntimeError("command failed: %s\n%s" % (" ".join(cmd), out))
+    return out
+
+
+def build_lib(jobs=8, verbose=False):
+    os.makedirs(BUILD, exist_ok=True)
+    objs = []
+    cmds = []
+    for s in SOURCES:
+        src = os.path.join(CSRC, s)
+        obj = os.path.join(BUILD, os.path.splitext(s)[0] + ".o")
+        objs.append(obj)
+        cmds.append([HIPCC] + HIP_FLAGS + ["-c", src, "-o", obj])
+    with cf.ThreadPoolExecutor(max_workers=jobs) as ex:
+        for out in ex.map(_run, cmds):
+            if verbose and out.strip():
+                print(out)
+    tmp = LIB + ".tmp"
+    _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp] + objs)
+    os.replace(tmp, LIB)
+    return LIB
+
+
+def build_oracle():
+    src = os.path.join(ORACLE_DIR, "oracle.c")
+    tmp = ORACLE_LIB + ".tmp"
+    _run(["gcc", "-std=c11", "-O2", "-fopenmp", "-fPIC", "-shared", "-Wall", "-Wextra",
+          "-o", tmp, src, "-lm"])
+    os.replace(tmp, ORACLE_LIB)
+    return ORACLE_LIB
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--lib", action="store_true")
+    ap.add_argument("--oracle", action="store_true")
+    ap.add_argument("-j", type=int, default=8)
+    ap.add_argument("-v", action="store_true")
+    a = ap.parse_args(argv)
+    both = not (a.lib or a.oracle)
+    if a.oracle or both:
+        print("built", build_oracle())
+    if a.lib or both:
+        print("built", build_lib(a.j, a.v))
+
+
+if __name__ == "__main__":
+    main()

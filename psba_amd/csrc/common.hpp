@@ -1,0 +1,115 @@
+// Shared helpers for the HIP kernels: error codes, launch checks and
+// deterministic (fixed-tree) block reductions for 64-wide wavefronts.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstdio>
+
+#include "psba_capi.h"
+
+namespace psba {
+
+constexpr int kWave = 64;
+
+#define PSBA_HIP_TRY(expr)                                                            \
+  do {                                                                                \
+    hipError_t _e = (expr);                                                           \
+    if (_e != hipSuccess) {                                                           \
+      psba::set_last_error(__FILE__, __LINE__, hipGetErrorString(_e));                \
+      return PSBA_ERR_HIP;                                                            \
+    }                                                                                 \
+  } while (0)
+
+#define PSBA_LAUNCH_TRY()                                                             \
+  do {                                                                                \
+    hipError_t _e = hipGetLastError();                                                \
+    if (_e != hipSuccess) {                                                           \
+      psba::set_last_error(__FILE__, __LINE__, hipGetErrorString(_e));                \
+      return PSBA_ERR_HIP;                                                            \
+    }                                                                                 \
+  } while (0)
+
+#define PSBA_TRY(expr)        \
+  do {                        \
+    int _r = (expr);          \
+    if (_r != PSBA_OK) return _r; \
+  } while (0)
+
+void set_last_error(const char* file, int line, const char* msg);
+
+// Sum over the 64 lanes of a wavefront with a fixed butterfly order; every lane
+// receives the result.
+__device__ inline double wave_sum(double v) {
+#pragma unroll
+  for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off, kWave);
+  return v;
+}
+
+__device__ inline float wave_sum(float v) {
+#pragma unroll
+  for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off, kWave);
+  return v;
+}
+
+// Deterministic block reduction of K values per thread.  `sh` must hold
+// K * (blockDim.x / 64) doubles.  Result valid in thread 0.
+template <int K>
+__device__ inline void block_sum(double (&v)[K], double* sh) {
+  const int lane = threadIdx.x & (kWave - 1), wid = threadIdx.x / kWave;
+  const int nw = blockDim.x / kWave;
+#pragma unroll
+  for (int k = 0; k < K; ++k) {
+    double s = wave_sum(v[k]);
+    if (lane == 0) sh[k * nw + wid] = s;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+      double s = 0;
+      for (int w = 0; w < nw; ++w) s += sh[k * nw + w];
+      v[k] = s;
+    }
+  }
+  __syncthreads();
+}
+
+// Two-level deterministic reduction: each block writes its K partials; the
+// last block to finish (ticket counter) sums all partials in block order and
+// writes `out[0..K)`, then resets the counter so the kernel can be relaunched
+// (also under graph replay).
+template <int K>
+__device__ inline void grid_sum_finish(double (&v)[K], double* partials, unsigned* counter,
+                                       double* out, double* sh) {
+  block_sum<K>(v, sh);
+  __shared__ bool last;
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int k = 0; k < K; ++k) partials[(size_t)blockIdx.x * K + k] = v[k];
+    __threadfence();
+    unsigned ticket = atomicAdd(counter, 1u);
+    last = (ticket == gridDim.x - 1);
+  }
+  __syncthreads();
+  if (!last) return;
+  __threadfence();
+  double acc[K];
+#pragma unroll
+  for (int k = 0; k < K; ++k) acc[k] = 0;
+  for (unsigned b = threadIdx.x; b < gridDim.x; b += blockDim.x) {
+#pragma unroll
+    for (int k = 0; k < K; ++k) acc[k] += partials[(size_t)b * K + k];
+  }
+  block_sum<K>(acc, sh);
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int k = 0; k < K; ++k) out[k] = acc[k];
+    *counter = 0u;
+  }
+}
+
+inline unsigned ceil_div(long a, long b) { return (unsigned)((a + b - 1) / b); }
+
+}  // namespace psba

@@ -1,0 +1,135 @@
+// Device state of one bundle-adjustment problem (the HIP-backed PSBA_struct).
+//
+// HBM layout (all FP64 in the plain path):
+//   parameters   cams[cnp*nc], pts[3*np]                 (AoS per camera / point)
+//   observations meas[2*no], obs_cam[no], obs_pt[no]     (point-major order)
+//   per-obs      e[2*no], JA[2*cnp*no], JB[6*no], W[3*cnp*no]  (AoS per obs)
+//   per-camera   U[cnp*cnp*nc] (full, symmetric), g[0..n)
+//   per-point    V[6*np] (xx xy xz yy yz zz), Vinv[6*np], z[3*np] = V^-1 g_b
+//   reduced      S[n_pad*n_pad] row-major, lower triangle; padded rows/cols
+//                beyond n = cnp*nc hold the identity so every tile is full.
+//   index        point CSR (pt_ptr), camera CSR (cam_ptr, cam_obs ascending
+//                point), camera-pair item lists (obs of camera k, obs of camera
+//                l) for k >= l in ascending point order, split into fixed
+//                chunks for the deterministic two-stage reductions.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <vector>
+
+#include "common.hpp"
+
+namespace psba {
+
+constexpr int kCamChunk = 256;   // observations per camera chunk (U/g_a/e_a)
+constexpr int kPairChunk = 64;   // shared points per camera-pair chunk (S)
+constexpr int kNB = 32;          // dense tile size of the Cholesky / solves
+constexpr int kMaxSolveN = 8192; // largest reduced system the single-block solve holds in LDS
+
+inline int cam_chunk_size(int cnp) { return cnp > 11 ? 128 : kCamChunk; }
+
+// scalar slots in ctx->d_scal (device) / ctx->h_scal (pinned host)
+enum Scalar {
+  SC_COST = 0,     // sum of squared residuals at the evaluated parameters
+  SC_DP2,          // |dp|^2
+  SC_RHODEN,       // sum dp (mu dp + g)
+  SC_P2,           // |p|^2 after commit
+  SC_MAXDIAG,      // max(UVdiag)
+  SC_GG,           // g.g
+  SC_D0, SC_D1, SC_D2, SC_D3, SC_D4, SC_D5,  // generic dot products
+  SC_LAMBDA,       // modified-Cholesky lambda
+  SC_FACTOR_OK,    // 1.0 when the Cholesky succeeded
+  SC_COUNT = 32
+};
+
+enum Counter { CT_RESID = 0, CT_STEP, CT_COMMIT, CT_DOT, CT_JMUL, CT_MAXD, CT_COUNT = 16 };
+
+struct Timing {
+  double jac = 0, schur = 0, factor = 0, solve = 0, backsub = 0, resid = 0, other = 0;
+  int trials = 0, jacobians = 0;
+};
+
+}  // namespace psba
+
+struct psba_ctx {
+  int cnp = 0, nc = 0, np = 0, no = 0;
+  int n = 0, ntot = 0, n_pad = 0;
+  int precision = 0, device = 0;
+  bool uploaded = false, indexed = false;
+  hipStream_t stream = nullptr;
+
+  // problem data
+  double *K = nullptr, *q0 = nullptr, *meas = nullptr, *cams = nullptr, *pts = nullptr;
+  int *obs_cam = nullptr, *obs_pt = nullptr;
+  unsigned char* fix = nullptr;
+  std::vector<unsigned char> h_fix;
+
+  // index
+  int* pt_ptr = nullptr;
+  int *cam_ptr = nullptr, *cam_obs = nullptr;
+  int n_cchunk = 0;
+  int *cchunk_begin = nullptr, *cchunk_end = nullptr, *cam_cchunk_ptr = nullptr;
+  int npairs = 0;
+  long n_items = 0;
+  int2* pair_items = nullptr;  // (obs in camera k, obs in camera l)
+  int *pair_k = nullptr, *pair_l = nullptr;
+  int n_schunk = 0;
+  int *schunk_begin = nullptr, *schunk_end = nullptr, *pair_schunk_ptr = nullptr;
+  int n_lblocks = 0;            // nc*(nc+1)/2 lower block positions
+  int* lblock_pair = nullptr;   // pair index of each lower block position or -1
+  double sum_t2 = 0;            // sum over points of track length squared
+
+  // work buffers
+  double *e = nullptr, *JA = nullptr, *JB = nullptr, *W = nullptr;
+  double *U = nullptr, *V = nullptr, *g = nullptr, *uvdiag = nullptr;
+  double *Vinv = nullptr, *z = nullptr;
+  double *ucpart = nullptr, *schpart = nullptr, *eapart = nullptr;
+  double *S = nullptr, *ea = nullptr, *xsol = nullptr, *dp = nullptr, *eb = nullptr;
+  double *vecA = nullptr, *vecB = nullptr, *vecC = nullptr, *vecD = nullptr;  // n_tot scratch
+  double* mc_work = nullptr;   // modified Cholesky scratch (3*n_pad)
+  double* Saux = nullptr;      // copy of S for the dogleg's recovery path (allocated on first use)
+  double* red_part = nullptr;  // reduction partials
+  long red_part_cap = 0;
+  unsigned* counters = nullptr;
+  double* d_scal = nullptr;
+  double* h_scal = nullptr;  // pinned
+
+  // timing
+  bool timing_on = false;
+  psba::Timing timing;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+};
+
+namespace psba {
+
+// ---- launchers (k_obs.hip) ----
+int launch_residual(psba_ctx* c, bool trial);            // -> d_scal[SC_COST], e (current only)
+int launch_jacobian_normal(psba_ctx* c, double coeff, double coeff_g);  // e, JA, JB, W, U, V, g, uvdiag
+int launch_maxdiag(psba_ctx* c);                          // -> d_scal[SC_MAXDIAG]
+int launch_vinv(psba_ctx* c, double mu);                  // Vinv, z
+int launch_backsub(psba_ctx* c);                          // eb, dp[n..)
+int launch_step_stats(psba_ctx* c, double mu);            // SC_DP2, SC_RHODEN
+int launch_commit(psba_ctx* c);                           // p += dp, SC_P2
+// out[SC_D0..] = dots of (J x1).(J x1), (J x1).(J x2), (J x2).(J x2); x2 may be null
+int launch_jmul_dots(psba_ctx* c, const double* x1, const double* x2);
+// generic n_tot dot products: SC_D0 = a.b (+ more)
+int launch_dot(psba_ctx* c, const double* a, const double* b, int slot);
+int launch_axpby(psba_ctx* c, double* out, double alpha, const double* a, double beta,
+                 const double* b, long n);
+
+// ---- launchers (k_schur.hip) ----
+int launch_schur(psba_ctx* c, double mu);                 // S (lower), ea
+
+// ---- launchers (k_dense.hip) ----
+int launch_factor(psba_ctx* c);                           // S -> L in place, SC_FACTOR_OK
+int launch_solve(psba_ctx* c);                            // ea -> dp[0..n)
+int launch_modchol(psba_ctx* c, double* S_src);           // SC_LAMBDA (factor in S)
+int launch_copy_S(psba_ctx* c, double* dst, const double* src);
+
+// ---- launchers (k_tr.hip) ----
+int launch_cauchy(psba_ctx* c, double* out, const double* g, double gtg, double gtBg);
+int launch_tr_dots(psba_ctx* c, const double* PU, const double* PB, const double* g, int mode);  // SC_D0..D4
+int launch_comb(psba_ctx* c, double* out, const double* a, const double* b, double s1, double s2, int op);
+
+}  // namespace psba

@@ -1,0 +1,430 @@
+// Levenberg-Marquardt and dogleg drivers of the C-ABI.
+//
+// Every trial runs as one batch of kernels on the context's stream; the host
+// reads back one block of scalars per trial (factor status, |dp|^2, gain-ratio
+// denominator, trial cost) and applies the reference's control logic:
+//   LM      PSBA/levmar.cpp:45-256 (+ compute_rho :271-280)
+//   dogleg  PSBA/trust_region.cpp:49-288, compute_PB :292-405,
+//           compute_p_2 :520-595
+//   hybrid  PSBA/main.cpp:193-208
+// The damping is applied on the fly to the undamped U/V blocks, so the
+// reference's update_UV / restore_UVdiag pair has no counterpart.
+#include <cmath>
+#include <cstring>
+
+#include "context.hpp"
+
+using namespace psba;
+
+namespace {
+
+constexpr double kInitMu = 1e-3;       // PSBA/psba.h:6
+constexpr double kStopThresh = 1e-12;  // PSBA/psba.h:7
+constexpr double kEps = 1e-12;         // PSBA/psba.h:8
+constexpr double kEps2 = 1e-12;        // PSBA/psba.h:9
+constexpr double kMaxDelta = 10000.0;  // PSBA/trust_region.cpp:18
+
+int fetch(psba_ctx* c) {
+  PSBA_HIP_TRY(hipMemcpyAsync(c->h_scal, c->d_scal, sizeof(double) * SC_COUNT, hipMemcpyDeviceToHost, c->stream));
+  PSBA_HIP_TRY(hipStreamSynchronize(c->stream));
+  return PSBA_OK;
+}
+
+void push(psba_trace_rec* tr, int cap, int* len, const psba_trace_rec& r) {
+  if (!tr || !len) return;
+  if (*len < cap) tr[*len] = r;
+  ++*len;
+}
+
+int opt_max_iter(const psba_opts* o) { return (o && o->max_iter > 0) ? o->max_iter : 50; }
+int opt_max_pb(const psba_opts* o) { return (o && o->max_pb_tries > 0) ? o->max_pb_tries : 64; }
+
+int check_ready(psba_ctx* c) {
+  if (!c) return PSBA_ERR_ARG;
+  if (!c->uploaded || !c->indexed) return PSBA_ERR_STATE;
+  PSBA_HIP_TRY(hipSetDevice(c->device));
+  return PSBA_OK;
+}
+
+// optional stage timing with events on the context's stream
+struct StageTimer {
+  psba_ctx* c;
+  double* slot;
+  StageTimer(psba_ctx* cc, double* s) : c(cc), slot(s) {
+    if (c->timing_on) (void)hipEventRecord(c->ev0, c->stream);
+  }
+  ~StageTimer() {
+    if (!c->timing_on) return;
+    (void)hipEventRecord(c->ev1, c->stream);
+    (void)hipEventSynchronize(c->ev1);
+    float ms = 0;
+    (void)hipEventElapsedTime(&ms, c->ev0, c->ev1);
+    *slot += ms;
+  }
+};
+
+// damped reduced system + factor + solve + back-substitution: dp
+int trial_solve(psba_ctx* c, double mu) {
+  {
+    StageTimer t(c, &c->timing.schur);
+    PSBA_TRY(launch_vinv(c, mu));
+    PSBA_TRY(launch_schur(c, mu));
+  }
+  {
+    StageTimer t(c, &c->timing.factor);
+    PSBA_TRY(launch_factor(c));
+  }
+  {
+    StageTimer t(c, &c->timing.solve);
+    PSBA_TRY(launch_solve(c));
+  }
+  {
+    StageTimer t(c, &c->timing.backsub);
+    PSBA_TRY(launch_backsub(c));
+  }
+  c->timing.trials++;
+  return PSBA_OK;
+}
+
+int jacobian(psba_ctx* c, double coeff, double coeff_g) {
+  StageTimer t(c, &c->timing.jac);
+  c->timing.jacobians++;
+  return launch_jacobian_normal(c, coeff, coeff_g);
+}
+
+}  // namespace
+
+extern "C" {
+
+int psba_levmar(psba_ctx* c, const psba_opts* o, int* itno, double* finalErr, psba_trace_rec* trace, int cap,
+                int* len) {
+  PSBA_TRY(check_ready(c));
+  if (!itno || !finalErr) return PSBA_ERR_ARG;
+  const int max_iter = opt_max_iter(o);
+  double mu = 0.0, p_L2 = 0.0;
+  int gooditer_cnt = 0, nu = 2;
+  bool first = true, p_pending = false;
+  PSBA_TRY(launch_residual(c, false));
+  PSBA_TRY(fetch(c));
+  double ex_L2 = c->h_scal[SC_COST];
+  int flag = PSBA_ITER_CONTINUE;
+  for (; *itno < max_iter && flag == PSBA_ITER_CONTINUE; (*itno)++) {
+    PSBA_TRY(jacobian(c, 1.0, 1.0));
+    if (first) {
+      PSBA_TRY(launch_maxdiag(c));
+      PSBA_TRY(fetch(c));
+      mu = kInitMu * c->h_scal[SC_MAXDIAG];
+      first = false;
+      p_L2 = 1e+3;
+      nu = 2;
+    }
+    while (true) {
+      psba_trace_rec rec{*itno, 0, 0, 0, NAN, mu, NAN, 0.0, NAN};
+      PSBA_TRY(trial_solve(c, mu));
+      {
+        StageTimer t(c, &c->timing.resid);
+        PSBA_TRY(launch_step_stats(c, mu));
+        PSBA_TRY(launch_residual(c, true));
+      }
+      PSBA_TRY(fetch(c));
+      if (p_pending) {
+        p_L2 = c->h_scal[SC_P2];
+        p_pending = false;
+      }
+      const bool ok = c->h_scal[SC_FACTOR_OK] != 0.0;
+      rec.factor_ok = ok;
+      if (ok) {
+        const double dp_L2 = c->h_scal[SC_DP2];
+        rec.dp_sq = dp_L2;
+        if (dp_L2 < p_L2 * kStopThresh * kStopThresh) {
+          flag = PSBA_ITER_DP_NO_CHANGE;
+          push(trace, cap, len, rec);
+          break;
+        }
+        if (dp_L2 >= (p_L2 + kStopThresh) / (kEps * kEps)) {
+          flag = PSBA_ITER_ERR;
+          push(trace, cap, len, rec);
+          break;
+        }
+        const double new_ex_L2 = c->h_scal[SC_COST];
+        const double rho = (ex_L2 - new_ex_L2) / c->h_scal[SC_RHODEN];
+        rec.cost = new_ex_L2;
+        rec.rho = rho;
+        if (rho > 0) {
+          double tmp = 2 * rho - 1;
+          tmp = 1.0 - tmp * tmp * tmp;
+          mu = mu * ((tmp >= (1.0 / 3.0)) ? tmp : (1.0 / 3.0));
+          nu = 2;
+          PSBA_TRY(launch_commit(c));
+          p_pending = true;
+          ex_L2 = new_ex_L2;
+          rec.accepted = 1;
+          push(trace, cap, len, rec);
+          if (std::fabs(rho - 1) < (1.0 / 5.0)) {
+            gooditer_cnt++;
+            if (gooditer_cnt >= 5) {
+              flag = PSBA_ITER_TURN_TO_TR;
+              break;
+            }
+          } else {
+            gooditer_cnt = 0;
+          }
+          break;
+        }
+      } else {
+        gooditer_cnt = 0;
+      }
+      push(trace, cap, len, rec);
+      mu *= nu;
+      if (nu >= (1 << 30)) {  // levmar.cpp:238-243, int nu doubles until 2*nu wraps
+        flag = PSBA_ITER_ERR;
+        break;
+      }
+      nu = 2 * nu;
+    }
+    if (ex_L2 <= kStopThresh) flag = PSBA_ITER_ERR_SMALL_ENOUGH;
+  }
+  PSBA_HIP_TRY(hipStreamSynchronize(c->stream));
+  *finalErr = ex_L2;
+  return flag;
+}
+
+}  // extern "C"
+
+namespace {
+
+// PSBA/trust_region.cpp:292-405.  Returns 1 when P_B (in vecB) was produced.
+int compute_PB(psba_ctx* c, double* lambda, int* solved) {
+  *solved = 0;
+  {
+    StageTimer t(c, &c->timing.schur);
+    PSBA_TRY(launch_vinv(c, *lambda));
+    PSBA_TRY(launch_schur(c, *lambda));
+  }
+  if (!c->Saux) PSBA_HIP_TRY(hipMalloc((void**)&c->Saux, sizeof(double) * (size_t)c->n_pad * c->n_pad));
+  PSBA_TRY(launch_copy_S(c, c->Saux, c->S));
+  {
+    StageTimer t(c, &c->timing.factor);
+    PSBA_TRY(launch_factor(c));
+  }
+  PSBA_TRY(fetch(c));
+  c->timing.trials++;
+  if (c->h_scal[SC_FACTOR_OK] == 0.0) {
+    if (*lambda == 0.0) {
+      StageTimer t(c, &c->timing.other);
+      PSBA_TRY(launch_modchol(c, c->Saux));
+      PSBA_TRY(fetch(c));
+      *lambda = c->h_scal[SC_LAMBDA];
+      return PSBA_OK;
+    }
+    *lambda = 2 * (*lambda);
+    return PSBA_OK;
+  }
+  {
+    StageTimer t(c, &c->timing.solve);
+    PSBA_TRY(launch_solve(c));
+  }
+  {
+    StageTimer t(c, &c->timing.backsub);
+    PSBA_TRY(launch_backsub(c));
+  }
+  PSBA_TRY(launch_axpby(c, c->vecB, -1.0, c->dp, 0.0, nullptr, c->ntot));
+  *solved = 1;
+  return PSBA_OK;
+}
+
+// PSBA/trust_region.cpp:520-595: the step P (in dp); returns its "norm" as the
+// reference reports it (including the bookkeeping of :564-569).
+int compute_p_2(psba_ctx* c, double pUtBpU, double pUtBpB, double pBtBpB, double delta, double* p_norm_out) {
+  const double* PU = c->vecA;
+  const double* PB = c->vecB;
+  PSBA_TRY(launch_tr_dots(c, PU, PB, c->g, 0));
+  PSBA_TRY(fetch(c));
+  const double pUg = c->h_scal[SC_D0], pBg = c->h_scal[SC_D1], pUpU = c->h_scal[SC_D2], pBpB = c->h_scal[SC_D4];
+  const double den = -pUtBpB * pUtBpB + pBtBpB * pUtBpU;
+  const double eta1 = (pBg * pUtBpB) / den - (pBtBpB * pUg) / den;
+  const double eta2 = (pUg * pUtBpB) / den - (pBg * pUtBpU) / den;
+  PSBA_TRY(launch_comb(c, c->dp, PU, PB, eta1, eta2, 0));
+  PSBA_TRY(launch_dot(c, c->dp, c->dp, SC_D5));
+  PSBA_TRY(fetch(c));
+  double p_norm = std::sqrt(c->h_scal[SC_D5]);
+  if (p_norm > delta) {
+    const double pU_norm = std::sqrt(pUpU), pB_norm = std::sqrt(pBpB);
+    if (pU_norm > delta) {
+      PSBA_TRY(launch_comb(c, c->dp, PU, PU, delta, pU_norm, 1));
+      *p_norm_out = delta;
+      return PSBA_OK;
+    } else if (pB_norm <= delta) {
+      PSBA_TRY(launch_axpby(c, c->dp, 1.0, PB, 0.0, nullptr, c->ntot));
+      *p_norm_out = std::sqrt(p_norm + pBpB);
+      return PSBA_OK;
+    } else {
+      PSBA_TRY(launch_tr_dots(c, PU, PB, c->g, 1));
+      PSBA_TRY(fetch(c));
+      double a = c->h_scal[SC_D0], b = c->h_scal[SC_D1], cc = c->h_scal[SC_D2];
+      b = 2 * b;
+      cc = cc - delta * delta;
+      double disc = b * b - 4 * a * cc;
+      if (std::fabs(disc) < 1e-12) disc = 0;
+      const double tau = (-b + std::sqrt(disc)) / (2 * a);
+      PSBA_TRY(launch_comb(c, c->dp, PU, PB, tau, 0.0, 2));
+      *p_norm_out = delta;
+      return PSBA_OK;
+    }
+  }
+  *p_norm_out = p_norm;
+  return PSBA_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int psba_trust_region(psba_ctx* c, const psba_opts* o, int* itno, double* finalErr, psba_trace_rec* trace, int cap,
+                      int* len) {
+  PSBA_TRY(check_ready(c));
+  if (!itno || !finalErr) return PSBA_ERR_ARG;
+  const int max_iter = opt_max_iter(o), max_pb = opt_max_pb(o);
+  int notgood_cnt = 0, good_iters = 0, nu = 2;
+  double dk = 1, lambda = 0, origin_lambda = 0.0;
+  PSBA_TRY(launch_residual(c, false));
+  PSBA_TRY(fetch(c));
+  double ex_L2 = c->h_scal[SC_COST];
+  int flag = PSBA_ITER_CONTINUE;
+  double* P_U = c->vecA;
+  double* P_B = c->vecB;
+  for (; *itno < max_iter; (*itno)++) {
+    PSBA_TRY(jacobian(c, 2.0, -2.0));
+    PSBA_TRY(launch_jmul_dots(c, c->g, nullptr));
+    PSBA_TRY(launch_dot(c, c->g, c->g, SC_GG));
+    PSBA_TRY(fetch(c));
+    const double gtBg = 2 * c->h_scal[SC_D0];
+    const double gtg = c->h_scal[SC_GG];
+    PSBA_TRY(launch_cauchy(c, P_U, c->g, gtg, gtBg));
+    int solved = 0, tries = 0;
+    bool ret_early = false;
+    while (!solved) {
+      PSBA_TRY(compute_PB(c, &lambda, &solved));
+      if (!solved) {
+        if (origin_lambda != 0.0) {
+          if (nu > 4) {
+            flag = PSBA_ITER_TURN_TO_LM;
+            *finalErr = ex_L2;
+            ret_early = true;
+            break;
+          }
+          lambda = lambda * nu;
+          nu = nu * 2;
+        }
+        if (++tries >= max_pb) {
+          flag = PSBA_ITER_TURN_TO_LM;
+          *finalErr = ex_L2;
+          ret_early = true;
+          break;
+        }
+      } else {
+        nu = 2;
+        origin_lambda = lambda;
+      }
+    }
+    if (ret_early) break;
+    PSBA_TRY(launch_jmul_dots(c, P_U, P_B));
+    PSBA_TRY(fetch(c));
+    const double pUtBpU = 2 * c->h_scal[SC_D0];
+    const double pUtBpB = 2 * c->h_scal[SC_D1];
+    const double pBtBpB = 2 * c->h_scal[SC_D2];
+    flag = PSBA_ITER_CONTINUE;
+    while (flag == PSBA_ITER_CONTINUE) {
+      psba_trace_rec rec{*itno, 1, 1, 0, NAN, dk, NAN, lambda, NAN};
+      double p_norm = 0;
+      PSBA_TRY(compute_p_2(c, pUtBpU, pUtBpB, pBtBpB, dk, &p_norm));
+      rec.dp_sq = p_norm * p_norm;
+      PSBA_TRY(launch_residual(c, true));
+      PSBA_TRY(fetch(c));
+      const double act_ex_L2 = c->h_scal[SC_COST];
+      rec.cost = act_ex_L2;
+      if (std::fabs((ex_L2 - act_ex_L2) / ex_L2) < kEps2) {
+        flag = PSBA_ITER_DP_NO_CHANGE;
+        push(trace, cap, len, rec);
+        break;
+      }
+      PSBA_TRY(launch_jmul_dots(c, c->dp, nullptr));
+      PSBA_TRY(launch_dot(c, c->g, c->dp, SC_D5));
+      PSBA_TRY(fetch(c));
+      const double Jx_norm = 2 * c->h_scal[SC_D0];
+      double pred_ex_L2 = c->h_scal[SC_D5];
+      pred_ex_L2 += ex_L2 + Jx_norm / 2;
+      const double rho = (ex_L2 - act_ex_L2) / (ex_L2 - pred_ex_L2);
+      rec.rho = rho;
+      if (rho < (1.0 / 4.0) || act_ex_L2 > ex_L2) {
+        dk = dk / 4;
+      } else if (rho >= (3.0 / 4.0) && act_ex_L2 < ex_L2) {
+        flag = PSBA_ITER_PASS;
+        PSBA_TRY(launch_commit(c));
+        *finalErr = act_ex_L2;
+        dk = std::fmin(2 * dk, kMaxDelta);
+        rec.accepted = 1;
+      } else if (rho >= (1.0 / 4.0) && rho < (3.0 / 4.0) && act_ex_L2 < ex_L2) {
+        flag = PSBA_ITER_PASS;
+        PSBA_TRY(launch_commit(c));
+        *finalErr = act_ex_L2;
+        rec.accepted = 1;
+      } else if (std::isnan(rho)) {
+        flag = PSBA_ITER_TURN_TO_LM;
+        *finalErr = ex_L2;
+        push(trace, cap, len, rec);
+        ret_early = true;
+        break;
+      }
+      push(trace, cap, len, rec);
+      if (std::fabs((act_ex_L2 - ex_L2) / ex_L2) <= kEps2) {
+        flag = PSBA_ITER_ERR_SMALL_ENOUGH;
+        break;
+      }
+      if (rho < 1.0 / 4) {
+        notgood_cnt++;
+        if (notgood_cnt >= 5) {
+          flag = PSBA_ITER_TURN_TO_LM;
+          break;
+        }
+      } else {
+        notgood_cnt = 0;
+      }
+      if (rho > 3.0 / 4 && act_ex_L2 < ex_L2) {
+        good_iters++;
+        if (good_iters >= 10) {
+          lambda = 0.0;
+          origin_lambda = 0.0;
+          good_iters = 0;
+        }
+      } else {
+        good_iters = 0;
+      }
+      if (rho > (1.0 / 4) && act_ex_L2 < ex_L2) ex_L2 = act_ex_L2;
+    }
+    if (ret_early) break;
+    if (flag != PSBA_ITER_PASS) break;
+  }
+  PSBA_HIP_TRY(hipStreamSynchronize(c->stream));
+  return flag;
+}
+
+int psba_hybrid(psba_ctx* c, const psba_opts* o, int* itno, double* initErr, double* finalErr,
+                psba_trace_rec* trace, int cap, int* len) {
+  PSBA_TRY(check_ready(c));
+  if (initErr) {
+    PSBA_TRY(launch_residual(c, false));
+    PSBA_TRY(fetch(c));
+    *initErr = c->h_scal[SC_COST];
+  }
+  int flag;
+  while (true) {
+    flag = psba_levmar(c, o, itno, finalErr, trace, cap, len);
+    if (flag != PSBA_ITER_TURN_TO_TR) break;
+    flag = psba_trust_region(c, o, itno, finalErr, trace, cap, len);
+    if (flag != PSBA_ITER_TURN_TO_LM) break;
+  }
+  return flag;
+}
+
+}  // extern "C"

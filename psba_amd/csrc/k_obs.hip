@@ -1,0 +1,490 @@
+// Observation-, point- and camera-level kernels of the plain FP64 path:
+// residual + cost, Jacobian + normal-equation blocks (U, V, W, g), damped
+// point-block inverses, back-substitution, step statistics and commit.
+//
+// Reference stages: PSBA/CL_files/compute_exQT.cl (residual),
+// compute_jacobiQT.cl (Jacobian), compute_U.cl, compute_V.cl,
+// compute_Wblks.cl, compute_g.cl (normal equations), update_UV.cl +
+// compute_Vinv.cl (damped V^-1), compute_eb.cl + compute_dpb.cl
+// (back-substitution), compute_newp.cl + update_p.cl (commit),
+// compute_Jmultiply.cl (J x for the dogleg's model decrease).
+#include "context.hpp"
+#include "model.hpp"
+
+namespace psba {
+
+#define PSBA_DISPATCH_CNP(cnp, FN, ...)        \
+  switch (cnp) {                               \
+    case 6: FN<6>(__VA_ARGS__); break;         \
+    case 9: FN<9>(__VA_ARGS__); break;         \
+    case 11: FN<11>(__VA_ARGS__); break;       \
+    case 16: FN<16>(__VA_ARGS__); break;       \
+    default: return PSBA_ERR_UNSUPPORTED;      \
+  }
+
+// ---------------------------------------------------------------------------
+// residual and cost
+// ---------------------------------------------------------------------------
+// One thread per observation.  With dp != nullptr the parameters are p + dp
+// (the trial point), computed with the same additions the commit performs.
+template <int CNP>
+__global__ __launch_bounds__(256) void k_residual(const double* __restrict__ K, const double* __restrict__ q0,
+                                                  const double* __restrict__ cams, const double* __restrict__ pts,
+                                                  const double* __restrict__ dp, int n,
+                                                  const double* __restrict__ meas, const int* __restrict__ obs_cam,
+                                                  const int* __restrict__ obs_pt, int no, double* __restrict__ e_out,
+                                                  double* partials, unsigned* counter, double* out) {
+  __shared__ double sh[4];
+  double acc[1] = {0.0};
+  int o = blockIdx.x * blockDim.x + threadIdx.x;
+  if (o < no) {
+    int j = obs_cam[o], i = obs_pt[o];
+    double cam[CNP], M[3], uv[2];
+#pragma unroll
+    for (int k = 0; k < CNP; ++k) cam[k] = dp ? cams[j * CNP + k] + dp[j * CNP + k] : cams[j * CNP + k];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) M[k] = dp ? pts[3 * i + k] + dp[n + 3 * i + k] : pts[3 * i + k];
+    project<double, CNP, false>(K + 5 * j, q0 + 4 * j, cam, M, uv, nullptr, nullptr);
+    double e0 = meas[2 * o] - uv[0], e1 = meas[2 * o + 1] - uv[1];
+    if (e_out) {
+      e_out[2 * o] = e0;
+      e_out[2 * o + 1] = e1;
+    }
+    acc[0] = e0 * e0 + e1 * e1;
+  }
+  grid_sum_finish<1>(acc, partials, counter, out, sh);
+}
+
+template <int CNP>
+static void run_residual(psba_ctx* c, bool trial) {
+  unsigned nb = ceil_div(c->no, 256);
+  hipLaunchKernelGGL(k_residual<CNP>, dim3(nb), dim3(256), 0, c->stream, c->K, c->q0, c->cams, c->pts,
+                     trial ? c->dp : nullptr, c->n, c->meas, c->obs_cam, c->obs_pt, c->no,
+                     trial ? nullptr : c->e, c->red_part, c->counters + CT_RESID, c->d_scal + SC_COST);
+}
+
+int launch_residual(psba_ctx* c, bool trial) {
+  PSBA_DISPATCH_CNP(c->cnp, run_residual, c, trial);
+  PSBA_LAUNCH_TRY();
+  return PSBA_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Jacobian, W and residual per observation
+// ---------------------------------------------------------------------------
+template <int CNP>
+__global__ __launch_bounds__(256) void k_jacobian(const double* __restrict__ K, const double* __restrict__ q0,
+                                                  const double* __restrict__ cams, const double* __restrict__ pts,
+                                                  const double* __restrict__ meas, const int* __restrict__ obs_cam,
+                                                  const int* __restrict__ obs_pt, const unsigned char* __restrict__ fix,
+                                                  int no, double coeff, double* __restrict__ e,
+                                                  double* __restrict__ JA, double* __restrict__ JB,
+                                                  double* __restrict__ W) {
+  int o = blockIdx.x * blockDim.x + threadIdx.x;
+  if (o >= no) return;
+  int j = obs_cam[o], i = obs_pt[o];
+  double cam[CNP], M[3], uv[2], ja[2 * CNP], jb[6];
+#pragma unroll
+  for (int k = 0; k < CNP; ++k) cam[k] = cams[j * CNP + k];
+#pragma unroll
+  for (int k = 0; k < 3; ++k) M[k] = pts[3 * i + k];
+  project<double, CNP, true>(K + 5 * j, q0 + 4 * j, cam, M, uv, ja, jb);
+#pragma unroll
+  for (int k = 0; k < CNP; ++k)
+    if (fix[j * CNP + k]) ja[k] = ja[CNP + k] = 0.0;
+  double e0 = meas[2 * o] - uv[0], e1 = meas[2 * o + 1] - uv[1];
+  e[2 * o] = e0;
+  e[2 * o + 1] = e1;
+  double* pa = JA + (size_t)o * 2 * CNP;
+#pragma unroll
+  for (int k = 0; k < 2 * CNP; ++k) pa[k] = ja[k];
+#pragma unroll
+  for (int k = 0; k < 6; ++k) JB[(size_t)o * 6 + k] = jb[k];
+  double* pw = W + (size_t)o * 3 * CNP;
+#pragma unroll
+  for (int r = 0; r < CNP; ++r)
+#pragma unroll
+    for (int cc = 0; cc < 3; ++cc) pw[3 * r + cc] = coeff * (ja[r] * jb[cc] + ja[CNP + r] * jb[3 + cc]);
+}
+
+// V_i (6 unique entries), g_b,i and the V part of UVdiag, one thread per point
+// over its observations in ascending camera order.
+__global__ __launch_bounds__(256) void k_point_normal(const int* __restrict__ pt_ptr, const double* __restrict__ JB,
+                                                      const double* __restrict__ e, int np, double coeff,
+                                                      double coeff_g, double* __restrict__ V,
+                                                      double* __restrict__ gb, double* __restrict__ uvd) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= np) return;
+  double v[6] = {0, 0, 0, 0, 0, 0}, g[3] = {0, 0, 0};
+  for (int o = pt_ptr[i]; o < pt_ptr[i + 1]; ++o) {
+    const double* b = JB + (size_t)o * 6;
+    double b0 = b[0], b1 = b[1], b2 = b[2], b3 = b[3], b4 = b[4], b5 = b[5];
+    double e0 = e[2 * o], e1 = e[2 * o + 1];
+    v[0] += b0 * b0 + b3 * b3;
+    v[1] += b0 * b1 + b3 * b4;
+    v[2] += b0 * b2 + b3 * b5;
+    v[3] += b1 * b1 + b4 * b4;
+    v[4] += b1 * b2 + b4 * b5;
+    v[5] += b2 * b2 + b5 * b5;
+    g[0] += b0 * e0 + b3 * e1;
+    g[1] += b1 * e0 + b4 * e1;
+    g[2] += b2 * e0 + b5 * e1;
+  }
+#pragma unroll
+  for (int k = 0; k < 6; ++k) V[(size_t)i * 6 + k] = coeff * v[k];
+#pragma unroll
+  for (int k = 0; k < 3; ++k) gb[(size_t)i * 3 + k] = coeff_g * g[k];
+  uvd[(size_t)i * 3 + 0] = coeff * v[0];
+  uvd[(size_t)i * 3 + 1] = coeff * v[3];
+  uvd[(size_t)i * 3 + 2] = coeff * v[5];
+}
+
+// Camera-segmented sums of the U_j upper triangle and g_a,j: stage 1.  Each
+// block takes one chunk (<= kCamChunk observations of one camera, ascending
+// point), stages JA and e in LDS, and thread t sums entry t over the chunk.
+template <int CNP>
+__global__ __launch_bounds__(256) void k_cam_chunks(const int* __restrict__ cam_obs, const int* __restrict__ cb,
+                                                    const int* __restrict__ ce, const double* __restrict__ JA,
+                                                    const double* __restrict__ e, double* __restrict__ part) {
+  constexpr int NU = CNP * (CNP + 1) / 2, NE = NU + CNP, REC = 2 * CNP + 2;
+  constexpr int CH = (CNP > 11) ? 128 : kCamChunk;
+  __shared__ double sJ[CH * REC];
+  const int b = blockIdx.x;
+  const int beg = cb[b], cnt = ce[b] - beg;
+  for (int f = threadIdx.x; f < cnt * REC; f += blockDim.x) {
+    int s = f / REC, el = f - s * REC;
+    int o = cam_obs[beg + s];
+    sJ[f] = el < 2 * CNP ? JA[(size_t)o * 2 * CNP + el] : e[2 * o + (el - 2 * CNP)];
+  }
+  __syncthreads();
+  for (int t = threadIdx.x; t < NE; t += blockDim.x) {
+    double acc = 0.0;
+    if (t < NU) {
+      int r = 0, rem = t;
+      while (rem >= CNP - r) {
+        rem -= CNP - r;
+        ++r;
+      }
+      int cc = r + rem;
+      for (int s = 0; s < cnt; ++s) {
+        const double* a = sJ + s * REC;
+        acc += a[r] * a[cc] + a[CNP + r] * a[CNP + cc];
+      }
+    } else {
+      int r = t - NU;
+      for (int s = 0; s < cnt; ++s) {
+        const double* a = sJ + s * REC;
+        acc += a[r] * a[2 * CNP] + a[CNP + r] * a[2 * CNP + 1];
+      }
+    }
+    part[(size_t)b * NE + t] = acc;
+  }
+}
+
+// stage 2: per camera, sum its chunk partials in chunk order.
+template <int CNP>
+__global__ __launch_bounds__(256) void k_cam_finish(const int* __restrict__ ccp, const double* __restrict__ part,
+                                                    int nc, double coeff, double coeff_g, double* __restrict__ U,
+                                                    double* __restrict__ ga, double* __restrict__ uvd) {
+  constexpr int NU = CNP * (CNP + 1) / 2, NE = NU + CNP;
+  const int j = blockIdx.x;
+  for (int t = threadIdx.x; t < NE; t += blockDim.x) {
+    double acc = 0.0;
+    for (int q = ccp[j]; q < ccp[j + 1]; ++q) acc += part[(size_t)q * NE + t];
+    if (t < NU) {
+      int r = 0, rem = t;
+      while (rem >= CNP - r) {
+        rem -= CNP - r;
+        ++r;
+      }
+      int cc = r + rem;
+      double val = coeff * acc;
+      U[(size_t)j * CNP * CNP + r * CNP + cc] = val;
+      U[(size_t)j * CNP * CNP + cc * CNP + r] = val;
+      if (r == cc) uvd[j * CNP + r] = val;
+    } else {
+      ga[j * CNP + (t - NU)] = coeff_g * acc;
+    }
+  }
+}
+
+template <int CNP>
+static void run_jac_normal(psba_ctx* c, double coeff, double coeff_g) {
+  hipLaunchKernelGGL(k_jacobian<CNP>, dim3(ceil_div(c->no, 256)), dim3(256), 0, c->stream, c->K, c->q0, c->cams,
+                     c->pts, c->meas, c->obs_cam, c->obs_pt, c->fix, c->no, coeff, c->e, c->JA, c->JB, c->W);
+  hipLaunchKernelGGL(k_point_normal, dim3(ceil_div(c->np, 256)), dim3(256), 0, c->stream, c->pt_ptr, c->JB, c->e,
+                     c->np, coeff, coeff_g, c->V, c->g + c->n, c->uvdiag + c->n);
+  if (c->n_cchunk > 0)
+    hipLaunchKernelGGL(k_cam_chunks<CNP>, dim3(c->n_cchunk), dim3(256), 0, c->stream, c->cam_obs, c->cchunk_begin,
+                       c->cchunk_end, c->JA, c->e, c->ucpart);
+  hipLaunchKernelGGL(k_cam_finish<CNP>, dim3(c->nc), dim3(256), 0, c->stream, c->cam_cchunk_ptr, c->ucpart, c->nc,
+                     coeff, coeff_g, c->U, c->g, c->uvdiag);
+}
+
+int launch_jacobian_normal(psba_ctx* c, double coeff, double coeff_g) {
+  PSBA_DISPATCH_CNP(c->cnp, run_jac_normal, c, coeff, coeff_g);
+  PSBA_LAUNCH_TRY();
+  return PSBA_OK;
+}
+
+// max over UVdiag (single block; once per LM entry)
+__global__ __launch_bounds__(1024) void k_maxdiag(const double* __restrict__ d, int n, double* out) {
+  __shared__ double sh[16];
+  double m = 0.0;
+  for (int q = threadIdx.x; q < n; q += blockDim.x) m = fmax(m, d[q]);
+#pragma unroll
+  for (int off = 32; off >= 1; off >>= 1) m = fmax(m, __shfl_xor(m, off, kWave));
+  if ((threadIdx.x & 63) == 0) sh[threadIdx.x / 64] = m;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double r = 0.0;
+    for (int w = 0; w < (int)(blockDim.x / 64); ++w) r = fmax(r, sh[w]);
+    *out = r;
+  }
+}
+
+int launch_maxdiag(psba_ctx* c) {
+  hipLaunchKernelGGL(k_maxdiag, dim3(1), dim3(1024), 0, c->stream, c->uvdiag, c->ntot, c->d_scal + SC_MAXDIAG);
+  PSBA_LAUNCH_TRY();
+  return PSBA_OK;
+}
+
+// ---------------------------------------------------------------------------
+// damped point-block inverse and z = V^-1 g_b
+// ---------------------------------------------------------------------------
+// Regular branch: adjugate over T = -det (PSBA/CL_files/compute_Vinv.cl:29,76-86).
+// |T| < 1e-16: the reference's pivoted-elimination divisor (compute_Vinv.cl:31-73).
+__global__ __launch_bounds__(256) void k_vinv(const double* __restrict__ V, const double* __restrict__ gb, int np,
+                                              double mu, double* __restrict__ Vinv, double* __restrict__ z) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= np) return;
+  const double* v = V + (size_t)i * 6;
+  double a11 = v[0] + mu, a12 = v[1], a13 = v[2], a22 = v[3] + mu, a23 = v[4], a33 = v[5] + mu;
+  double T = a33 * a12 * a12 - 2 * a12 * a13 * a23 + a22 * a13 * a13 + a11 * a23 * a23 - a11 * a22 * a33;
+  double c00 = a22 * a33 - a23 * a23, c01 = a13 * a23 - a12 * a33, c02 = a12 * a23 - a13 * a22;
+  double c11 = a11 * a33 - a13 * a13, c12 = a12 * a13 - a11 * a23, c22 = a11 * a22 - a12 * a12;
+  double d;
+  if (fabs(T) < 1e-16) {
+    double m[3][3] = {{a11, a12, a13}, {a12, a22, a23}, {a13, a23, a33}};
+    int p = 0;
+    if (m[0][0] < m[1][0]) p = 1;
+    if (m[p][0] < m[2][0]) p = 2;
+    if (p != 0)
+      for (int q = 0; q < 3; ++q) {
+        double t = m[0][q]; m[0][q] = m[p][q]; m[p][q] = t;
+      }
+    m[1][0] /= m[0][0];
+    m[2][0] /= m[0][0];
+    m[1][1] -= m[1][0] * m[0][1];
+    m[1][2] -= m[1][0] * m[0][2];
+    m[2][1] -= m[2][0] * m[0][1];
+    m[2][2] -= m[2][0] * m[0][2];
+    if (m[1][1] < m[2][1])
+      for (int q = 0; q < 3; ++q) {
+        double t = m[1][q]; m[1][q] = m[2][q]; m[2][q] = t;
+      }
+    if (m[1][1] != 0.0) {
+      m[2][1] /= m[1][1];
+      m[2][2] -= m[2][1] * m[1][2];
+    }
+    d = m[0][0] * m[1][1] * m[2][2];
+  } else {
+    d = -T;
+  }
+  double w[6] = {c00 / d, c01 / d, c02 / d, c11 / d, c12 / d, c22 / d};
+  double* out = Vinv + (size_t)i * 6;
+#pragma unroll
+  for (int k = 0; k < 6; ++k) out[k] = w[k];
+  double g0 = gb[3 * i], g1 = gb[3 * i + 1], g2 = gb[3 * i + 2];
+  z[3 * i + 0] = w[0] * g0 + w[1] * g1 + w[2] * g2;
+  z[3 * i + 1] = w[1] * g0 + w[3] * g1 + w[4] * g2;
+  z[3 * i + 2] = w[2] * g0 + w[4] * g1 + w[5] * g2;
+}
+
+int launch_vinv(psba_ctx* c, double mu) {
+  hipLaunchKernelGGL(k_vinv, dim3(ceil_div(c->np, 256)), dim3(256), 0, c->stream, c->V, c->g + c->n, c->np, mu,
+                     c->Vinv, c->z);
+  PSBA_LAUNCH_TRY();
+  return PSBA_OK;
+}
+
+// ---------------------------------------------------------------------------
+// back-substitution: eb_i = g_b,i - sum_j W_ij^T dpa_j ; dpb_i = V_i^-1 eb_i
+// ---------------------------------------------------------------------------
+template <int CNP>
+__global__ __launch_bounds__(256) void k_backsub(const int* __restrict__ pt_ptr, const int* __restrict__ obs_cam,
+                                                 const double* __restrict__ W, const double* __restrict__ Vinv,
+                                                 const double* __restrict__ gb, int np, int n,
+                                                 double* __restrict__ dp, double* __restrict__ eb) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= np) return;
+  double s[3] = {0, 0, 0};
+  for (int o = pt_ptr[i]; o < pt_ptr[i + 1]; ++o) {
+    const double* w = W + (size_t)o * 3 * CNP;
+    const double* da = dp + obs_cam[o] * CNP;
+#pragma unroll
+    for (int k = 0; k < CNP; ++k) {
+      double d = da[k];
+      s[0] += w[3 * k] * d;
+      s[1] += w[3 * k + 1] * d;
+      s[2] += w[3 * k + 2] * d;
+    }
+  }
+  double e0 = gb[3 * i] - s[0], e1 = gb[3 * i + 1] - s[1], e2 = gb[3 * i + 2] - s[2];
+  eb[3 * i] = e0;
+  eb[3 * i + 1] = e1;
+  eb[3 * i + 2] = e2;
+  const double* w = Vinv + (size_t)i * 6;
+  dp[n + 3 * i + 0] = w[0] * e0 + w[1] * e1 + w[2] * e2;
+  dp[n + 3 * i + 1] = w[1] * e0 + w[3] * e1 + w[4] * e2;
+  dp[n + 3 * i + 2] = w[2] * e0 + w[4] * e1 + w[5] * e2;
+}
+
+template <int CNP>
+static void run_backsub(psba_ctx* c) {
+  hipLaunchKernelGGL(k_backsub<CNP>, dim3(ceil_div(c->np, 256)), dim3(256), 0, c->stream, c->pt_ptr, c->obs_cam,
+                     c->W, c->Vinv, c->g + c->n, c->np, c->n, c->dp, c->eb);
+}
+
+int launch_backsub(psba_ctx* c) {
+  PSBA_DISPATCH_CNP(c->cnp, run_backsub, c);
+  PSBA_LAUNCH_TRY();
+  return PSBA_OK;
+}
+
+// ---------------------------------------------------------------------------
+// step statistics, commit, dot products
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_step_stats(const double* __restrict__ dp, const double* __restrict__ g,
+                                                    long ntot, double mu, double* partials, unsigned* counter,
+                                                    double* out) {
+  __shared__ double sh[8];
+  double acc[2] = {0.0, 0.0};
+  long q = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q < ntot) {
+    double d = dp[q];
+    acc[0] = d * d;
+    acc[1] = d * (mu * d + g[q]);
+  }
+  grid_sum_finish<2>(acc, partials, counter, out, sh);
+}
+
+int launch_step_stats(psba_ctx* c, double mu) {
+  hipLaunchKernelGGL(k_step_stats, dim3(ceil_div(c->ntot, 256)), dim3(256), 0, c->stream, c->dp, c->g,
+                     (long)c->ntot, mu, c->red_part, c->counters + CT_STEP, c->d_scal + SC_DP2);
+  PSBA_LAUNCH_TRY();
+  return PSBA_OK;
+}
+
+__global__ __launch_bounds__(256) void k_commit(double* __restrict__ cams, double* __restrict__ pts,
+                                                const double* __restrict__ dp, int n, long ntot, double* partials,
+                                                unsigned* counter, double* out) {
+  __shared__ double sh[4];
+  double acc[1] = {0.0};
+  long q = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q < ntot) {
+    double v;
+    if (q < n) {
+      v = cams[q] + dp[q];
+      cams[q] = v;
+    } else {
+      v = pts[q - n] + dp[q];
+      pts[q - n] = v;
+    }
+    acc[0] = v * v;
+  }
+  grid_sum_finish<1>(acc, partials, counter, out, sh);
+}
+
+int launch_commit(psba_ctx* c) {
+  hipLaunchKernelGGL(k_commit, dim3(ceil_div(c->ntot, 256)), dim3(256), 0, c->stream, c->cams, c->pts, c->dp, c->n,
+                     (long)c->ntot, c->red_part, c->counters + CT_COMMIT, c->d_scal + SC_P2);
+  PSBA_LAUNCH_TRY();
+  return PSBA_OK;
+}
+
+// (J x1).(J x1), (J x1).(J x2), (J x2).(J x2) over the observations, J the
+// projection Jacobian (coefficient 1), PSBA/CL_files/compute_Jmultiply.cl:6-52
+// followed by the host dot products of PSBA/trust_region.cpp:126,174-176,210.
+template <int CNP>
+__global__ __launch_bounds__(256) void k_jmul_dots(const double* __restrict__ JA, const double* __restrict__ JB,
+                                                   const int* __restrict__ obs_cam, const int* __restrict__ obs_pt,
+                                                   int no, int n, const double* __restrict__ x1,
+                                                   const double* __restrict__ x2, double* partials,
+                                                   unsigned* counter, double* out) {
+  __shared__ double sh[12];
+  double acc[3] = {0.0, 0.0, 0.0};
+  int o = blockIdx.x * blockDim.x + threadIdx.x;
+  if (o < no) {
+    int j = obs_cam[o], i = obs_pt[o];
+    const double* ja = JA + (size_t)o * 2 * CNP;
+    const double* jb = JB + (size_t)o * 6;
+    double y1[2], y2[2] = {0, 0};
+#pragma unroll
+    for (int r = 0; r < 2; ++r) {
+      double s = 0, t = 0;
+#pragma unroll
+      for (int k = 0; k < CNP; ++k) {
+        s += ja[r * CNP + k] * x1[j * CNP + k];
+        if (x2) t += ja[r * CNP + k] * x2[j * CNP + k];
+      }
+#pragma unroll
+      for (int k = 0; k < 3; ++k) {
+        s += jb[3 * r + k] * x1[n + 3 * i + k];
+        if (x2) t += jb[3 * r + k] * x2[n + 3 * i + k];
+      }
+      y1[r] = s;
+      y2[r] = t;
+    }
+    acc[0] = y1[0] * y1[0] + y1[1] * y1[1];
+    acc[1] = y1[0] * y2[0] + y1[1] * y2[1];
+    acc[2] = y2[0] * y2[0] + y2[1] * y2[1];
+  }
+  grid_sum_finish<3>(acc, partials, counter, out, sh);
+}
+
+template <int CNP>
+static void run_jmul(psba_ctx* c, const double* x1, const double* x2) {
+  hipLaunchKernelGGL(k_jmul_dots<CNP>, dim3(ceil_div(c->no, 256)), dim3(256), 0, c->stream, c->JA, c->JB,
+                     c->obs_cam, c->obs_pt, c->no, c->n, x1, x2, c->red_part, c->counters + CT_JMUL,
+                     c->d_scal + SC_D0);
+}
+
+int launch_jmul_dots(psba_ctx* c, const double* x1, const double* x2) {
+  PSBA_DISPATCH_CNP(c->cnp, run_jmul, c, x1, x2);
+  PSBA_LAUNCH_TRY();
+  return PSBA_OK;
+}
+
+__global__ __launch_bounds__(256) void k_dot(const double* __restrict__ a, const double* __restrict__ b, long n,
+                                             double* partials, unsigned* counter, double* out) {
+  __shared__ double sh[4];
+  double acc[1] = {0.0};
+  long q = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q < n) acc[0] = a[q] * b[q];
+  grid_sum_finish<1>(acc, partials, counter, out, sh);
+}
+
+int launch_dot(psba_ctx* c, const double* a, const double* b, int slot) {
+  hipLaunchKernelGGL(k_dot, dim3(ceil_div(c->ntot, 256)), dim3(256), 0, c->stream, a, b, (long)c->ntot,
+                     c->red_part, c->counters + CT_DOT, c->d_scal + slot);
+  PSBA_LAUNCH_TRY();
+  return PSBA_OK;
+}
+
+__global__ __launch_bounds__(256) void k_axpby(double* __restrict__ out, double alpha, const double* __restrict__ a,
+                                               double beta, const double* __restrict__ b, long n) {
+  long q = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= n) return;
+  double v = alpha * a[q];
+  if (b) v += beta * b[q];
+  out[q] = v;
+}
+
+int launch_axpby(psba_ctx* c, double* out, double alpha, const double* a, double beta, const double* b, long n) {
+  hipLaunchKernelGGL(k_axpby, dim3(ceil_div(n, 256)), dim3(256), 0, c->stream, out, alpha, a, beta, b, n);
+  PSBA_LAUNCH_TRY();
+  return PSBA_OK;
+}
+
+}  // namespace psba

@@ -1,0 +1,155 @@
+// Reduced camera system of the plain FP64 path:
+//   S_kl = delta_kl (U_k + mu I) - sum_{i in P(k,l)} W_ik V_i^-1 W_il^T   (k >= l)
+//   e_a,j = g_a,j - sum_i W_ij (V_i^-1 g_b,i)
+// Reference: PSBA/CL_files/compute_Yblks.cl, compute_S.cl, compute_ea.cl.
+// Y = W V^-1 is formed on the fly in LDS (never written to HBM); e_a uses
+// z_i = V_i^-1 g_b,i from the inverse kernel.  Both sums are deterministic
+// two-stage reductions over fixed chunks (item order = ascending point).
+#include "context.hpp"
+
+namespace psba {
+
+template <int CNP>
+__global__ __launch_bounds__(256) void k_schur_chunks(const int2* __restrict__ items, const int* __restrict__ sb,
+                                                      const int* __restrict__ se, const int* __restrict__ obs_pt,
+                                                      const double* __restrict__ W, const double* __restrict__ Vinv,
+                                                      double* __restrict__ part) {
+  constexpr int C2 = CNP * CNP, REC = 3 * CNP;
+  constexpr int G = (256 / C2) > 0 ? (256 / C2) : 1;
+  __shared__ double sY[kPairChunk * REC];
+  __shared__ double sW[kPairChunk * REC];
+  __shared__ double sG[G * C2];
+  const int b = blockIdx.x;
+  const int beg = sb[b], cnt = se[b] - beg;
+  // Y rows: task = (item, row r) -> Y[it][r][0..2] = W_a[r][:] V^-1
+  for (int task = threadIdx.x; task < cnt * CNP; task += blockDim.x) {
+    int it = task / CNP, r = task - it * CNP;
+    int2 pr = items[beg + it];
+    const double* w = W + (size_t)pr.x * REC + 3 * r;
+    const double* vi = Vinv + (size_t)obs_pt[pr.x] * 6;
+    double w0 = w[0], w1 = w[1], w2 = w[2];
+    double v00 = vi[0], v01 = vi[1], v02 = vi[2], v11 = vi[3], v12 = vi[4], v22 = vi[5];
+    double* y = sY + it * REC + 3 * r;
+    y[0] = w0 * v00 + w1 * v01 + w2 * v02;
+    y[1] = w0 * v01 + w1 * v11 + w2 * v12;
+    y[2] = w0 * v02 + w1 * v12 + w2 * v22;
+  }
+  for (int f = threadIdx.x; f < cnt * REC; f += blockDim.x) {
+    int it = f / REC, el = f - it * REC;
+    sW[f] = W[(size_t)items[beg + it].y * REC + el];
+  }
+  __syncthreads();
+  const int grp = threadIdx.x / C2, ent = threadIdx.x - grp * C2;
+  if (grp < G) {
+    const int r = ent / CNP, cc = ent - r * CNP;
+    double acc = 0.0;
+    for (int it = grp; it < cnt; it += G) {
+      const double* y = sY + it * REC + 3 * r;
+      const double* w = sW + it * REC + 3 * cc;
+      acc += y[0] * w[0] + y[1] * w[1] + y[2] * w[2];
+    }
+    sG[grp * C2 + ent] = acc;
+  }
+  __syncthreads();
+  for (int t = threadIdx.x; t < C2; t += blockDim.x) {
+    double s = 0.0;
+    for (int q = 0; q < G; ++q) s += sG[q * C2 + t];
+    part[(size_t)b * C2 + t] = s;
+  }
+}
+
+// one block per lower block position (k >= l) of S, including empty ones
+template <int CNP>
+__global__ __launch_bounds__(256) void k_schur_finish(const int* __restrict__ lblock_pair,
+                                                      const int* __restrict__ psp, const double* __restrict__ part,
+                                                      const double* __restrict__ U,
+                                                      const unsigned char* __restrict__ fix, double mu, int ld,
+                                                      double* __restrict__ S) {
+  constexpr int C2 = CNP * CNP;
+  const int b = blockIdx.x;
+  int k = (int)((sqrt(8.0 * b + 1.0) - 1.0) * 0.5);
+  while ((k + 1) * (k + 2) / 2 <= b) ++k;
+  while (k * (k + 1) / 2 > b) --k;
+  const int l = b - k * (k + 1) / 2;
+  const int q = lblock_pair[b];
+  for (int t = threadIdx.x; t < C2; t += blockDim.x) {
+    const int r = t / CNP, cc = t - r * CNP;
+    double acc = 0.0;
+    if (q >= 0)
+      for (int h = psp[q]; h < psp[q + 1]; ++h) acc += part[(size_t)h * C2 + t];
+    double val;
+    if (k == l) {
+      double u = U[(size_t)k * C2 + t];
+      if (r == cc) u = u + mu;
+      val = u - acc;
+      if (r == cc && fix[k * CNP + r]) val = 1.0;
+    } else {
+      val = -acc;
+    }
+    S[(size_t)(k * CNP + r) * ld + l * CNP + cc] = val;
+  }
+}
+
+// e_a chunks over the camera chunks: p_o = W_o z_i, column sums
+template <int CNP>
+__global__ __launch_bounds__(256) void k_ea_chunks(const int* __restrict__ cam_obs, const int* __restrict__ cb,
+                                                   const int* __restrict__ ce, const int* __restrict__ obs_pt,
+                                                   const double* __restrict__ W, const double* __restrict__ z,
+                                                   double* __restrict__ part) {
+  constexpr int CH = (CNP > 11) ? 128 : kCamChunk;
+  __shared__ double sP[CH * CNP];
+  const int b = blockIdx.x;
+  const int beg = cb[b], cnt = ce[b] - beg;
+  for (int task = threadIdx.x; task < cnt * CNP; task += blockDim.x) {
+    int s = task / CNP, r = task - s * CNP;
+    int o = cam_obs[beg + s];
+    const double* w = W + (size_t)o * 3 * CNP + 3 * r;
+    const double* zi = z + (size_t)obs_pt[o] * 3;
+    sP[task] = w[0] * zi[0] + w[1] * zi[1] + w[2] * zi[2];
+  }
+  __syncthreads();
+  if (threadIdx.x < CNP) {
+    double acc = 0.0;
+    for (int s = 0; s < cnt; ++s) acc += sP[s * CNP + threadIdx.x];
+    part[(size_t)b * CNP + threadIdx.x] = acc;
+  }
+}
+
+template <int CNP>
+__global__ __launch_bounds__(64) void k_ea_finish(const int* __restrict__ ccp, const double* __restrict__ part,
+                                                  const double* __restrict__ ga, int nc, double* __restrict__ ea) {
+  int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= nc * CNP) return;
+  int j = t / CNP, r = t - j * CNP;
+  double acc = 0.0;
+  for (int q = ccp[j]; q < ccp[j + 1]; ++q) acc += part[(size_t)q * CNP + r];
+  ea[t] = ga[t] - acc;
+}
+
+template <int CNP>
+static void run_schur(psba_ctx* c, double mu) {
+  if (c->n_schunk > 0)
+    hipLaunchKernelGGL(k_schur_chunks<CNP>, dim3(c->n_schunk), dim3(256), 0, c->stream, c->pair_items,
+                       c->schunk_begin, c->schunk_end, c->obs_pt, c->W, c->Vinv, c->schpart);
+  hipLaunchKernelGGL(k_schur_finish<CNP>, dim3(c->n_lblocks), dim3(CNP * CNP > 128 ? 256 : 128), 0, c->stream,
+                     c->lblock_pair, c->pair_schunk_ptr, c->schpart, c->U, c->fix, mu, c->n_pad, c->S);
+  if (c->n_cchunk > 0)
+    hipLaunchKernelGGL(k_ea_chunks<CNP>, dim3(c->n_cchunk), dim3(256), 0, c->stream, c->cam_obs, c->cchunk_begin,
+                       c->cchunk_end, c->obs_pt, c->W, c->z, c->eapart);
+  hipLaunchKernelGGL(k_ea_finish<CNP>, dim3(ceil_div((long)c->nc * CNP, 64)), dim3(64), 0, c->stream,
+                     c->cam_cchunk_ptr, c->eapart, c->g, c->nc, c->ea);
+}
+
+int launch_schur(psba_ctx* c, double mu) {
+  switch (c->cnp) {
+    case 6: run_schur<6>(c, mu); break;
+    case 9: run_schur<9>(c, mu); break;
+    case 11: run_schur<11>(c, mu); break;
+    case 16: run_schur<16>(c, mu); break;
+    default: return PSBA_ERR_UNSUPPORTED;
+  }
+  PSBA_LAUNCH_TRY();
+  return PSBA_OK;
+}
+
+}  // namespace psba
