@@ -1,0 +1,128 @@
+"""Benchmark: time-to-converge of the 88-camera / 64,298-point synthetic scene.
+
+One step = one complete bundle-adjustment solve of scene config 2 (88 cameras,
+cnp 9 = f, k1, k2 + pose; 64,298 points; Zipf track lengths fitted to ~384k
+observations; 1 px noise + 1% outliers) on one MI355X: parameters are reset on
+the device to the same perturbed start, then Levenberg-Marquardt runs until an
+accepted step lowers the cost by less than 1e-8 relative (or 100 iterations).
+Inputs are resident in HBM before the timed region.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W]
+
+Prints one JSON line (see the "metric" and "config" fields).
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "time-to-converge vs plain FP64 at equal RMS; solves/s + p99 latency (stream)"
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def cpu_baseline(scene, gpu_iters, gpu_trials, budget_s=20.0):
+    """The oracle (CPU FP64 restatement of the reference algorithm) on this
+    host: a bounded sample of LM trials of the same scene, scaled to one full
+    solve with the GPU run's trial/iteration counts."""
+    from oracle import binding as O
+    threads = os.cpu_count() or 1
+    env_t = os.environ.get("OMP_NUM_THREADS")
+    if env_t:
+        threads = int(env_t)
+    O.set_threads(threads)
+    orc = O.Oracle.from_scene(scene)
+    t0 = time.perf_counter()
+    orc.residual()
+    orc.jacobian_normal(1.0, 1.0)
+    t_jac = time.perf_counter() - t0
+    mu = 1e-3 * orc.max_uvdiag()
+    trials, t_trial = 0, 0.0
+    while trials < 3 and (t_jac + t_trial) < budget_s:
+        t0 = time.perf_counter()
+        orc.schur(mu)
+        orc.factor_solve()
+        orc.backsub()
+        orc.residual(trial=False)
+        t_trial += time.perf_counter() - t0
+        trials += 1
+    per_trial = t_trial / trials
+    est = gpu_iters * t_jac + gpu_trials * per_trial
+    return {"value": 1.0 / est, "unit": "solves/s", "cores": threads, "kind": "port",
+            "sample": "1 Jacobian/normal-equation pass + %d LM trials of config 2 (%.1f s), scaled to the GPU "
+                      "solve's %d iterations / %d trials" % (trials, t_jac + t_trial, gpu_iters, gpu_trials)}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--seed", type=int, default=1002)
+    ap.add_argument("--no-cpu", action="store_true")
+    a = ap.parse_args()
+    if a.gpus != 1:
+        print("bench.py measures one MI355X only: multi-GPU runs are not part of this build "
+              "(run with --gpus 1)", file=sys.stderr)
+        sys.exit(2)
+    import psba_amd as P
+
+    sc = P.Scene(2, a.seed)
+    log("scene: cams=%d cnp=%d pts=%d obs=%d sum_t2=%.0f zipf_s=%.3f outliers=%d" % (
+        sc.n_cams, sc.cnp, sc.n_pts, sc.n_obs, sc.sum_t2, sc.zipf_s, sc.n_outliers))
+    ctx = P.Context.from_scene(sc)
+    ctx.snapshot()
+    kw = dict(max_iter=100, flags=P.FLAG_NO_HANDOFF, rel_tol=1e-8)
+
+    def solve():
+        ctx.restore()
+        ctx.itno = 0
+        return ctx.levmar(**kw)
+
+    for _ in range(a.warmup):
+        solve()
+    res = []
+    t0 = time.perf_counter()
+    for _ in range(a.steps):
+        res.append(solve())
+    dt = time.perf_counter() - t0
+    r = res[-1]
+    trials = len(r["trace"])
+    final = r["final_err"]
+    rms = float(np.sqrt(final / (2 * sc.n_obs)))
+    log("solve: code=%d iters=%d trials=%d final_sse=%.6e rms=%.6f px, %.3f ms/solve" % (
+        r["code"], r["itno"], trials, final, rms, 1e3 * dt / a.steps))
+    ms = 1e3 * dt / a.steps
+    out = {
+        "metric": METRIC,
+        "value": a.steps / dt,
+        "unit": "solves/s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": ms,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (seeded scene generator, config 2)",
+        "config": {"workload": "config 2: 88 cams (f,k1,k2+pose), %d pts, %d obs, LM to rel. decrease < 1e-8, "
+                               "plain FP64" % (sc.n_pts, sc.n_obs),
+                   "n_obs": sc.n_obs, "sum_t2": sc.sum_t2, "lm_iterations": r["itno"], "lm_trials": trials,
+                   "final_rms_px": rms},
+    }
+    if not a.no_cpu:
+        out["cpu_baseline"] = cpu_baseline(sc, r["itno"], trials)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

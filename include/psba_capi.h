@@ -57,11 +57,16 @@ typedef struct {
   int device;               /* HIP device ordinal */
 } psba_desc;
 
+/* psba_opts.flags */
+#define PSBA_FLAG_NO_HANDOFF 1 /* LM never returns ITER_TURN_TO_TR (pure LM solve) */
+
 typedef struct {
   int max_iter;      /* shared iteration cap (reference: 50) */
   int max_pb_tries;  /* guard on the dogleg's factor-retry loop (0 = 64) */
   int refine_steps;  /* mixed precision: FP64 refinement sweeps (0 = default) */
-  int reserved;
+  int flags;         /* PSBA_FLAG_* (0 = reference behaviour) */
+  double rel_tol;    /* > 0: LM also stops (ITER_DP_NO_CHANGE) once an accepted
+                        step lowers the cost by less than rel_tol relative */
 } psba_opts;
 
 /* one record per LM trial or dogleg step (same fields as the oracle's) */
@@ -88,6 +93,9 @@ int psba_set_fixed(psba_ctx* ctx, const unsigned char* mask);
  * cams_buffer / pts3D_buffer, PSBA/cl_psba.h:21-22) */
 int psba_read_params(psba_ctx* ctx, double* cams, double* pts);
 int psba_write_params(psba_ctx* ctx, const double* cams, const double* pts);
+/* device-side copy of the current parameters and its restore (no host traffic) */
+int psba_snapshot_params(psba_ctx* ctx);
+int psba_restore_params(psba_ctx* ctx);
 const char* psba_strerror(int code);
 const char* psba_last_error(void);
 int psba_device_count(void);
@@ -130,6 +138,10 @@ int psba_stage_backsub(psba_ctx* ctx);
 /* modified Cholesky of the current S, returns lambda = |sum E| / n;
  * PSBA/cl_cholmod.h (cholmod_blk, compute_cholmod_E) */
 int psba_stage_modchol(psba_ctx* ctx, double* lambda);
+/* test hook: load an n x n matrix (n = cnp*n_cams, row-major, lower triangle
+ * used) as S and, if b != NULL, b as e_a, so the dense stages can be checked
+ * on arbitrary matrices */
+int psba_stage_load_S(psba_ctx* ctx, const double* S, const double* b);
 
 /* device buffer read-back for stage parity tests */
 #define PSBA_BUF_EX 0      /* 2*n_obs residual */
@@ -147,6 +159,28 @@ int psba_stage_modchol(psba_ctx* ctx, double* lambda);
 #define PSBA_BUF_L 13      /* n*n lower factor */
 long psba_buffer_size(const psba_ctx* ctx, int which);
 int psba_read_buffer(psba_ctx* ctx, int which, double* out, long n);
+
+/* problem/index statistics */
+typedef struct {
+  int cnp, n_cams, n_pts, n_obs;
+  int n;              /* cnp * n_cams, order of S */
+  int n_pad;          /* S storage order (multiple of the tile size) */
+  int n_pairs;        /* camera pairs k >= l sharing a point */
+  long n_pair_items;  /* sum over points of t(t+1)/2 */
+  double sum_t2;      /* sum over points of t^2 */
+} psba_info;
+int psba_get_info(psba_ctx* ctx, psba_info* out);
+
+/* Kernel probe: HIP events recorded on the context's stream around every
+ * launch of one kernel (or kernel group); psba_probe_read returns the summed
+ * device time and the number of launches since the last read. */
+#define PSBA_PROBE_OFF 0
+#define PSBA_PROBE_SCHUR 1    /* Schur-complement assembly (k_schur_chunks) */
+#define PSBA_PROBE_FACTOR 2   /* whole blocked Cholesky + forward substitution */
+#define PSBA_PROBE_JACOBIAN 3 /* Jacobian / W per observation (k_jacobian) */
+#define PSBA_PROBE_BSOLVE 4   /* back substitution */
+int psba_probe(psba_ctx* ctx, int which);
+int psba_probe_read(psba_ctx* ctx, double* total_ms, int* count);
 
 /* per-stage device timings of the last driver call (milliseconds) */
 typedef struct {

@@ -27,8 +27,12 @@ class Problem(ct.Structure):
                 ("cams", ct.POINTER(ct.c_double)), ("pts", ct.POINTER(ct.c_double))]
 
 
+FLAG_NO_HANDOFF = 1
+
+
 class Opts(ct.Structure):
-    _fields_ = [("solve_mode", ct.c_int), ("max_iter", ct.c_int), ("max_pb_tries", ct.c_int)]
+    _fields_ = [("solve_mode", ct.c_int), ("max_iter", ct.c_int), ("max_pb_tries", ct.c_int), ("flags", ct.c_int),
+                ("rel_tol", ct.c_double)]
 
 
 class TraceRec(ct.Structure):
@@ -76,6 +80,8 @@ def lib():
         "orc_levmar": (ct.c_int, [vp, ct.POINTER(Opts), ip, dp, ct.POINTER(TraceRec), ct.c_int, ip]),
         "orc_trust_region": (ct.c_int, [vp, ct.POINTER(Opts), ip, dp, ct.POINTER(TraceRec), ct.c_int, ip]),
         "orc_hybrid": (ct.c_int, [vp, ct.POINTER(Opts), ip, dp, dp, ct.POINTER(TraceRec), ct.c_int, ip]),
+        "orc_dense_factor_solve": (ct.c_int, [ct.c_int, ct.c_int, dp, dp, dp, dp]),
+        "orc_dense_modchol": (ct.c_double, [ct.c_int, dp, dp, dp]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -107,6 +113,27 @@ def project(cnp, K5, q0, cam, M, jac=False):
     if rc != 0:
         raise ValueError("unsupported cnp")
     return (uv, JA.reshape(2, cnp), JB.reshape(2, 3)) if jac else uv
+
+
+def dense_factor_solve(S, b, solve_mode=SOLVE_CHOLESKY):
+    """(ok, x, L) for S x = b (lower triangle of S used)."""
+    Sm = np.ascontiguousarray(S, np.float64)
+    n = Sm.shape[0]
+    bb = np.ascontiguousarray(b, np.float64)
+    x = np.zeros(n)
+    L = np.zeros((n, n))
+    ok = lib().orc_dense_factor_solve(n, solve_mode, _d(Sm), _d(bb), _d(x), _d(L))
+    return bool(ok), x, L
+
+
+def dense_modchol(S):
+    """(lambda, L, E) of the dogleg's modified Cholesky."""
+    Sm = np.ascontiguousarray(S, np.float64)
+    n = Sm.shape[0]
+    L = np.zeros((n, n))
+    E = np.zeros(n)
+    lam = lib().orc_dense_modchol(n, _d(Sm), _d(L), _d(E))
+    return lam, L, E
 
 
 def generate_idxs(n_cams, n_pts, vmask, with_comm=True):
@@ -168,8 +195,8 @@ class Oracle:
         p = np.ascontiguousarray(pts, np.float64)
         lib().orc_set_params(self._h, _d(c), _d(p))
 
-    def _run(self, fn, trace_cap, solve_mode, max_iter, max_pb_tries=0, init=False):
-        o = Opts(solve_mode, max_iter, max_pb_tries)
+    def _run(self, fn, trace_cap, solve_mode, max_iter, max_pb_tries=0, init=False, flags=0, rel_tol=0.0):
+        o = Opts(solve_mode, max_iter, max_pb_tries, flags, rel_tol)
         tr = (TraceRec * max(trace_cap, 1))()
         n = ct.c_int(0)
         it = ct.c_int(self.itno)
@@ -185,8 +212,8 @@ class Oracle:
             out["init_err"] = ie.value
         return out
 
-    def levmar(self, max_iter=50, trace_cap=4096, solve_mode=SOLVE_CHOLESKY):
-        return self._run(lib().orc_levmar, trace_cap, solve_mode, max_iter)
+    def levmar(self, max_iter=50, trace_cap=4096, solve_mode=SOLVE_CHOLESKY, flags=0, rel_tol=0.0):
+        return self._run(lib().orc_levmar, trace_cap, solve_mode, max_iter, flags=flags, rel_tol=rel_tol)
 
     def trust_region(self, max_iter=50, trace_cap=4096, solve_mode=SOLVE_CHOLESKY, max_pb_tries=0):
         return self._run(lib().orc_trust_region, trace_cap, solve_mode, max_iter, max_pb_tries)

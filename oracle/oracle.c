@@ -780,36 +780,51 @@ static int spd_inverse_ref(int n, const double *S, double *L, double *Sinv) {
 /* factor the reduced camera system and solve S dpa = ea into dp[0..n);
  * PSBA/PSBA/levmar.cpp:134-140 (SPDinv + matVec_mul) in the inverse mode,
  * Cholesky + two triangular solves otherwise.  Returns 1 on success. */
-int orc_stage_factor_solve(orc_ctx *c, int solve_mode) {
-  int n = c->n;
-  c->L_valid = 0;
-  if (solve_mode == ORC_SOLVE_INVERSE) {
-    double *Sinv = c->Sinv;
-    int ok = spd_inverse_ref(n, c->S, c->L, Sinv);
+static int factor_solve(int n, int solve_mode, const double *S, const double *b, double *x, double *L,
+                        double *Sinv, double *y) {
+  /* the reference's explicit inverse works on 3x3 blocks (cnp = 6); other
+   * orders use the Cholesky solve */
+  if (solve_mode == ORC_SOLVE_INVERSE && n % 3 == 0) {
+    int ok = spd_inverse_ref(n, S, L, Sinv);
     if (!ok) return 0;
-    c->L_valid = 1;
     for (int r = 0; r < n; r++) {
       double s = 0;
-      for (int k = 0; k < n; k++) s = s + Sinv[(long)r * n + k] * c->ea[k];
-      c->dp[r] = s;
+      for (int k = 0; k < n; k++) s = s + Sinv[(long)r * n + k] * b[k];
+      x[r] = s;
     }
     return 1;
   }
-  if (!chol_plain(n, c->S, c->L)) return 0;
-  c->L_valid = 1;
-  double *y = c->xtmp;
+  if (!chol_plain(n, S, L)) return 0;
   for (int i = 0; i < n; i++) {
-    double s = c->ea[i];
-    const double *Li = c->L + (long)i * n;
+    double s = b[i];
+    const double *Li = L + (long)i * n;
     for (int k = 0; k < i; k++) s -= Li[k] * y[k];
     y[i] = s / Li[i];
   }
   for (int i = n - 1; i >= 0; i--) {
     double s = y[i];
-    for (int k = i + 1; k < n; k++) s -= c->L[(long)k * n + i] * c->dp[k];
-    c->dp[i] = s / c->L[(long)i * n + i];
+    for (int k = i + 1; k < n; k++) s -= L[(long)k * n + i] * x[k];
+    x[i] = s / L[(long)i * n + i];
   }
   return 1;
+}
+
+int orc_stage_factor_solve(orc_ctx *c, int solve_mode) {
+  c->L_valid = 0;
+  int ok = factor_solve(c->n, solve_mode, c->S, c->ea, c->dp, c->L, c->Sinv, c->xtmp);
+  c->L_valid = ok;
+  return ok;
+}
+
+int orc_dense_factor_solve(int n, int solve_mode, const double *S, const double *b, double *x, double *L) {
+  double *Lw = L ? L : (double *)xcalloc((size_t)n * n, sizeof(double));
+  double *Sinv = (double *)xcalloc((size_t)n * n, sizeof(double));
+  double *y = (double *)xcalloc(n, sizeof(double));
+  int ok = factor_solve(n, solve_mode, S, b, x, Lw, Sinv, y);
+  if (!L) free(Lw);
+  free(Sinv);
+  free(y);
+  return ok;
 }
 
 /* eb_i = g_b,i - sum_j W_ij^T dpa_j (PSBA/CL_files/compute_eb.cl:6-41) and
@@ -875,10 +890,32 @@ void orc_jmul(const orc_ctx *c, const double *x, double *out) {
  * theta_j / beta, cholmod_blk.cl:446-697).  E_i = (L L^T)_ii - S_ii.
  * S (lower triangle) is read from c->S; the factor goes to c->L.
  */
-double orc_modchol_lambda(orc_ctx *c, double *Eout) {
-  int n = c->n, nb = n / 3;
-  const double *A = c->S;
-  double *L = c->L;
+/* one column of the column-wise modified Cholesky, cholmod_blk.cl:446-697 */
+static void modchol_column(int n, const double *A, double *L, double *Cc, int xj, double delta, double beta) {
+  double sum = A[(long)xj * n + xj];
+  for (int k = 0; k < xj; k++) sum -= L[(long)xj * n + k] * L[(long)xj * n + k];
+  double dj = fmax(fabs(sum), delta);
+  double sd = sqrt(dj);
+  L[(long)xj * n + xj] = sd;
+  int over = 0;
+  for (int i = xj + 1; i < n; i++) {
+    double s = A[(long)i * n + xj];
+    for (int k = 0; k < xj; k++) s = s - L[(long)i * n + k] * L[(long)xj * n + k];
+    Cc[i] = s;
+    L[(long)i * n + xj] = s / sd;
+    if (L[(long)i * n + xj] > beta) over = 1;
+  }
+  if (over) {
+    double theta = 0;
+    for (int i = xj + 1; i < n; i++) theta = fmax(theta, fabs(Cc[i]));
+    double nd = theta / beta;
+    L[(long)xj * n + xj] = nd;
+    for (int i = xj + 1; i < n; i++) L[(long)i * n + xj] = Cc[i] / nd;
+  }
+}
+
+static double modchol_dense(int n, const double *A, double *L, double *Eout, double *Cc) {
+  int nb = n / 3;
   double xi = 0, gamma = 0;
   for (int r = 0; r < n; r++) {
     for (int k = 0; k < n; k++) {
@@ -895,7 +932,6 @@ double orc_modchol_lambda(orc_ctx *c, double *Eout) {
   beta = sqrt(beta);
 
   memset(L, 0, sizeof(double) * (size_t)n * n);
-  double *Cc = c->xtmp; /* column of C_ij values, length n */
   for (int j = 0; j < nb; j++) {
     int blk_ok = 1;
     double T[9];
@@ -928,15 +964,18 @@ double orc_modchol_lambda(orc_ctx *c, double *Eout) {
       long b = (long)(3 * j) * n + 3 * j;
       L[b] = l0; L[b + n] = l1; L[b + n + 1] = l2; L[b + 2 * n] = l3; L[b + 2 * n + 1] = l4; L[b + 2 * n + 2] = l5;
       int over = 0;
-      for (int i = j + 1; i < nb; i++) {
+      /* row triplets below the diagonal block; the last one is partial when
+       * n is not a multiple of 3 */
+      for (int i = j + 1; 3 * i < n; i++) {
         double Tij[9];
-        for (int u = 0; u < 3; u++)
+        int nu = n - 3 * i < 3 ? n - 3 * i : 3;
+        for (int u = 0; u < nu; u++)
           for (int v = 0; v < 3; v++) {
             double s = A[(long)(3 * i + u) * n + 3 * j + v];
             for (int k = 0; k < 3 * j; k++) s -= L[(long)(3 * i + u) * n + k] * L[(long)(3 * j + v) * n + k];
             Tij[3 * u + v] = s;
           }
-        for (int u = 0; u < 3; u++)
+        for (int u = 0; u < nu; u++)
           for (int v = 0; v < 3; v++) {
             double s = Tij[3 * u] * di[3 * v] + Tij[3 * u + 1] * di[3 * v + 1] + Tij[3 * u + 2] * di[3 * v + 2];
             L[(long)(3 * i + u) * n + 3 * j + v] = s;
@@ -951,30 +990,11 @@ double orc_modchol_lambda(orc_ctx *c, double *Eout) {
       for (int i = 3 * j; i < 3 * j + 3; i++)
         for (int v = 0; v < 3; v++) L[(long)i * n + 3 * j + v] = 0.0;
     }
-    for (int col = 0; col < 3; col++) {
-      int xj = 3 * j + col;
-      double sum = A[(long)xj * n + xj];
-      for (int k = 0; k < xj; k++) sum -= L[(long)xj * n + k] * L[(long)xj * n + k];
-      double dj = fmax(fabs(sum), delta);
-      double sd = sqrt(dj);
-      L[(long)xj * n + xj] = sd;
-      int over = 0;
-      for (int i = xj + 1; i < n; i++) {
-        double s = A[(long)i * n + xj];
-        for (int k = 0; k < xj; k++) s = s - L[(long)i * n + k] * L[(long)xj * n + k];
-        Cc[i] = s;
-        L[(long)i * n + xj] = s / sd;
-        if (L[(long)i * n + xj] > beta) over = 1;
-      }
-      if (over) {
-        double theta = 0;
-        for (int i = xj + 1; i < n; i++) theta = fmax(theta, fabs(Cc[i]));
-        double nd = theta / beta;
-        L[(long)xj * n + xj] = nd;
-        for (int i = xj + 1; i < n; i++) L[(long)i * n + xj] = Cc[i] / nd;
-      }
-    }
+    for (int col = 0; col < 3; col++) modchol_column(n, A, L, Cc, 3 * j + col, delta, beta);
   }
+  /* columns past the last full 3-block (n not a multiple of 3, possible with
+   * cnp 11 or 16; the reference's cnp = 6 never has them): column-wise path */
+  for (int xj = 3 * nb; xj < n; xj++) modchol_column(n, A, L, Cc, xj, delta, beta);
   double sumE = 0;
   for (int i = 0; i < n; i++) {
     double s = 0;
@@ -983,8 +1003,21 @@ double orc_modchol_lambda(orc_ctx *c, double *Eout) {
     if (Eout) Eout[i] = e;
     sumE += e;
   }
-  c->L_valid = 1;
   return fabs(sumE) / n;
+}
+
+double orc_modchol_lambda(orc_ctx *c, double *Eout) {
+  c->L_valid = 1;
+  return modchol_dense(c->n, c->S, c->L, Eout, c->xtmp);
+}
+
+double orc_dense_modchol(int n, const double *S, double *L, double *E) {
+  double *Lw = L ? L : (double *)xcalloc((size_t)n * n, sizeof(double));
+  double *Cc = (double *)xcalloc(n, sizeof(double));
+  double lam = modchol_dense(n, S, Lw, E, Cc);
+  if (!L) free(Lw);
+  free(Cc);
+  return lam;
 }
 
 long orc_buffer_size(const orc_ctx *c, int which) {
@@ -1049,10 +1082,14 @@ static void opts_default(const orc_opts *o, orc_opts *d) {
   d->solve_mode = ORC_SOLVE_CHOLESKY;
   d->max_iter = 50;
   d->max_pb_tries = 64;
+  d->flags = 0;
+  d->rel_tol = 0.0;
   if (o) {
     d->solve_mode = o->solve_mode;
     if (o->max_iter > 0) d->max_iter = o->max_iter;
     if (o->max_pb_tries > 0) d->max_pb_tries = o->max_pb_tries;
+    d->flags = o->flags;
+    d->rel_tol = o->rel_tol;
   }
 }
 
@@ -1122,10 +1159,17 @@ int orc_levmar(orc_ctx *c, const orc_opts *oin, int *itno, double *finalErr,
           nu = 2;
           orc_stage_commit(c);
           p_L2 = params_l2_full(c);
+          double rel_dec = (ex_L2 - new_ex_L2) / ex_L2;
           ex_L2 = new_ex_L2;
           rec.accepted = 1;
           trace_push(trace, trace_cap, trace_len, &rec);
-          if (fabs(rho - 1) < (1.0 / 5.0)) {
+          if (o.rel_tol > 0 && rel_dec < o.rel_tol) {
+            iter_flag = ORC_ITER_DP_NO_CHANGE;
+            break;
+          }
+          if (o.flags & ORC_FLAG_NO_HANDOFF) {
+            gooditer_cnt = 0;
+          } else if (fabs(rho - 1) < (1.0 / 5.0)) {
             gooditer_cnt++;
             if (gooditer_cnt >= 5) {
               iter_flag = ORC_ITER_TURN_TO_TR;

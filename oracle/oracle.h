@@ -52,10 +52,14 @@ typedef struct {
   const double *pts;            /* 3*n_pts initial parameters */
 } orc_problem;
 
+#define ORC_FLAG_NO_HANDOFF 1 /* LM never returns ORC_ITER_TURN_TO_TR */
+
 typedef struct {
   int solve_mode;   /* ORC_SOLVE_* */
   int max_iter;     /* shared iteration cap, 50 in the reference */
   int max_pb_tries; /* guard on the dogleg's factor-retry loop (reference: unbounded) */
+  int flags;        /* ORC_FLAG_* */
+  double rel_tol;   /* > 0: stop after an accepted step with relative decrease < rel_tol */
 } orc_opts;
 
 /* one record per LM trial or dogleg step */
@@ -79,6 +83,12 @@ int orc_cnp_supported(int cnp);
  * JA (2 x cnp row-major) and JB (2 x 3 row-major) of the projection. */
 int orc_project(int cnp, const double *K5, const double *q0, const double *cam,
                 const double *M, double *uv, double *JA, double *JB);
+
+/* ---- dense kernels on caller matrices (n x n row-major, lower used) ------- */
+/* factor + solve S x = b; returns 1 on success.  L (n*n) may be NULL. */
+int orc_dense_factor_solve(int n, int solve_mode, const double *S, const double *b, double *x, double *L);
+/* modified Cholesky: returns lambda = |sum E| / n; L and E may be NULL */
+double orc_dense_modchol(int n, const double *S, double *L, double *E);
 
 /* ---- integer indexing, PSBA/misc.cpp:178-218 ------------------------------ */
 /* comm3DIdx may be NULL (it is n_cams*n_cams*n_pts ints). */

@@ -42,9 +42,12 @@ class Desc(ct.Structure):
                 ("n_pts", ct.c_int), ("n_obs", ct.c_int), ("precision", ct.c_int), ("device", ct.c_int)]
 
 
+FLAG_NO_HANDOFF = 1
+
+
 class Opts(ct.Structure):
     _fields_ = [("max_iter", ct.c_int), ("max_pb_tries", ct.c_int), ("refine_steps", ct.c_int),
-                ("reserved", ct.c_int)]
+                ("flags", ct.c_int), ("rel_tol", ct.c_double)]
 
 
 class TraceRec(ct.Structure):
@@ -57,6 +60,14 @@ class Timing(ct.Structure):
     _fields_ = [("jacobian_normal", ct.c_double), ("schur", ct.c_double), ("factor", ct.c_double),
                 ("solve", ct.c_double), ("backsub", ct.c_double), ("residual", ct.c_double),
                 ("other", ct.c_double), ("trials", ct.c_int), ("jacobians", ct.c_int)]
+
+
+class Info(ct.Structure):
+    _fields_ = [("cnp", ct.c_int), ("n_cams", ct.c_int), ("n_pts", ct.c_int), ("n_obs", ct.c_int), ("n", ct.c_int),
+                ("n_pad", ct.c_int), ("n_pairs", ct.c_int), ("n_pair_items", ct.c_long), ("sum_t2", ct.c_double)]
+
+
+PROBE_OFF, PROBE_SCHUR, PROBE_FACTOR, PROBE_JACOBIAN, PROBE_BSOLVE = 0, 1, 2, 3, 4
 
 
 class _SceneC(ct.Structure):
@@ -96,6 +107,8 @@ def lib():
         "psba_set_fixed": (ct.c_int, [vp, ct.POINTER(ct.c_ubyte)]),
         "psba_read_params": (ct.c_int, [vp, dp, dp]),
         "psba_write_params": (ct.c_int, [vp, dp, dp]),
+        "psba_snapshot_params": (ct.c_int, [vp]),
+        "psba_restore_params": (ct.c_int, [vp]),
         "psba_strerror": (ct.c_char_p, [ct.c_int]),
         "psba_last_error": (ct.c_char_p, []),
         "psba_device_count": (ct.c_int, []),
@@ -109,10 +122,14 @@ def lib():
         "psba_stage_factor_solve": (ct.c_int, [vp, ip]),
         "psba_stage_backsub": (ct.c_int, [vp]),
         "psba_stage_modchol": (ct.c_int, [vp, dp]),
+        "psba_stage_load_S": (ct.c_int, [vp, dp, dp]),
         "psba_buffer_size": (ct.c_long, [vp, ct.c_int]),
         "psba_read_buffer": (ct.c_int, [vp, ct.c_int, dp, ct.c_long]),
         "psba_get_timing": (ct.c_int, [vp, ct.POINTER(Timing)]),
         "psba_enable_timing": (ct.c_int, [vp, ct.c_int]),
+        "psba_get_info": (ct.c_int, [vp, ct.POINTER(Info)]),
+        "psba_probe": (ct.c_int, [vp, ct.c_int]),
+        "psba_probe_read": (ct.c_int, [vp, dp, ip]),
         "psba_scene_generate": (ct.POINTER(_SceneC), [ct.c_int, ct.c_ulonglong, ct.POINTER(SceneOpts)]),
         "psba_scene_free": (None, [ct.POINTER(_SceneC)]),
         "psba_problem_seed": (ct.c_ulonglong, [ct.c_ulonglong, ct.c_ulonglong]),
@@ -241,8 +258,14 @@ class Context:
         p = np.ascontiguousarray(pts, dtype=np.float64)
         _check(lib().psba_write_params(self._h, _dptr(c), _dptr(p)), "psba_write_params")
 
-    def _opts(self, max_iter=50, max_pb_tries=0):
-        return Opts(max_iter, max_pb_tries, 0, 0)
+    def snapshot(self):
+        _check(lib().psba_snapshot_params(self._h), "psba_snapshot_params")
+
+    def restore(self):
+        _check(lib().psba_restore_params(self._h), "psba_restore_params")
+
+    def _opts(self, max_iter=50, max_pb_tries=0, flags=0, rel_tol=0.0):
+        return Opts(max_iter, max_pb_tries, 0, flags, rel_tol)
 
     def _run(self, fn, trace_cap, opts, extra_init=False):
         tr = (TraceRec * max(trace_cap, 1))()
@@ -262,9 +285,9 @@ class Context:
             out["init_err"] = ie.value
         return out
 
-    def levmar(self, max_iter=50, trace_cap=4096):
+    def levmar(self, max_iter=50, trace_cap=4096, flags=0, rel_tol=0.0):
         """PSBA/levmar.h:6-11 on the device; returns code, itno, final_err, trace."""
-        return self._run(lib().psba_levmar, trace_cap, self._opts(max_iter))
+        return self._run(lib().psba_levmar, trace_cap, self._opts(max_iter, 0, flags, rel_tol))
 
     def trust_region(self, max_iter=50, trace_cap=4096, max_pb_tries=0):
         """PSBA/trust_region.h:5-11 on the device."""
@@ -299,12 +322,37 @@ class Context:
         _check(lib().psba_stage_modchol(self._h, ct.byref(lam)), "modchol")
         return lam.value
 
+    def load_S(self, S, b=None):
+        """Test hook: S (n x n) as the reduced matrix and b as its right-hand side."""
+        Sm = np.ascontiguousarray(S, dtype=np.float64)
+        bb = None if b is None else np.ascontiguousarray(b, dtype=np.float64)
+        _check(lib().psba_stage_load_S(self._h, _dptr(Sm), None if bb is None else _dptr(bb)), "load_S")
+
+    def factor_only(self):
+        """Factor + solve of the loaded S; returns (ok, x) with x = S^-1 b."""
+        ok = self.factor_solve()
+        return ok, self.buffer("DP")[: self.cnp * self.n_cams]
+
     def buffer(self, name):
         w = BUF[name]
         n = lib().psba_buffer_size(self._h, w)
         out = np.empty(n)
         _check(lib().psba_read_buffer(self._h, w, _dptr(out), n), "read_buffer")
         return out
+
+    def info(self):
+        i = Info()
+        _check(lib().psba_get_info(self._h, ct.byref(i)), "get_info")
+        return {k: getattr(i, k) for k, _ in Info._fields_}
+
+    def probe(self, which):
+        _check(lib().psba_probe(self._h, which), "probe")
+
+    def probe_read(self):
+        t = ct.c_double()
+        n = ct.c_int()
+        _check(lib().psba_probe_read(self._h, ct.byref(t), ct.byref(n)), "probe_read")
+        return t.value, n.value
 
     def enable_timing(self, on=True):
         _check(lib().psba_enable_timing(self._h, int(on)), "enable_timing")

@@ -24,7 +24,7 @@ namespace psba {
 
 constexpr int kCamChunk = 256;   // observations per camera chunk (U/g_a/e_a)
 constexpr int kPairChunk = 64;   // shared points per camera-pair chunk (S)
-constexpr int kNB = 32;          // dense tile size of the Cholesky / solves
+constexpr int kNB = 64;          // dense tile size of the Cholesky / solves
 constexpr int kMaxSolveN = 8192; // largest reduced system the single-block solve holds in LDS
 
 inline int cam_chunk_size(int cnp) { return cnp > 11 ? 128 : kCamChunk; }
@@ -89,11 +89,17 @@ struct psba_ctx {
   double *vecA = nullptr, *vecB = nullptr, *vecC = nullptr, *vecD = nullptr;  // n_tot scratch
   double* mc_work = nullptr;   // modified Cholesky scratch (3*n_pad)
   double* Saux = nullptr;      // copy of S for the dogleg's recovery path (allocated on first use)
+  double* snap = nullptr;      // parameter snapshot (psba_snapshot_params)
   double* red_part = nullptr;  // reduction partials
   long red_part_cap = 0;
   unsigned* counters = nullptr;
   double* d_scal = nullptr;
   double* h_scal = nullptr;  // pinned
+
+  // kernel probe (events around one kernel's launches)
+  int probe_id = 0;
+  std::vector<hipEvent_t> probe_pool;
+  std::vector<std::pair<hipEvent_t, hipEvent_t>> probe_pending;
 
   // timing
   bool timing_on = false;
@@ -102,6 +108,10 @@ struct psba_ctx {
 };
 
 namespace psba {
+
+// probe helpers: record an event before/after a launch when probing `id`
+void probe_begin(psba_ctx* c, int id);
+void probe_end(psba_ctx* c, int id);
 
 // ---- launchers (k_obs.hip) ----
 int launch_residual(psba_ctx* c, bool trial);            // -> d_scal[SC_COST], e (current only)

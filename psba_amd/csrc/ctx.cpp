@@ -75,6 +75,33 @@ static void free_index(psba_ctx* c) {
   c->indexed = false;
 }
 
+static hipEvent_t probe_event(psba_ctx* c) {
+  if (c->probe_pool.empty()) {
+    hipEvent_t e = nullptr;
+    if (hipEventCreate(&e) != hipSuccess) return nullptr;
+    return e;
+  }
+  hipEvent_t e = c->probe_pool.back();
+  c->probe_pool.pop_back();
+  return e;
+}
+
+void probe_begin(psba_ctx* c, int id) {
+  if (c->probe_id != id) return;
+  hipEvent_t e = probe_event(c);
+  if (!e) return;
+  (void)hipEventRecord(e, c->stream);
+  c->probe_pending.push_back({e, nullptr});
+}
+
+void probe_end(psba_ctx* c, int id) {
+  if (c->probe_id != id || c->probe_pending.empty() || c->probe_pending.back().second) return;
+  hipEvent_t e = probe_event(c);
+  if (!e) return;
+  (void)hipEventRecord(e, c->stream);
+  c->probe_pending.back().second = e;
+}
+
 }  // namespace psba
 
 using namespace psba;
@@ -205,12 +232,17 @@ int psba_destroy(psba_ctx* c) {
   free_index(c);
   double* bufs[] = {c->K, c->q0, c->meas, c->cams, c->pts, c->e, c->JA, c->JB, c->W, c->U, c->V, c->g,
                     c->uvdiag, c->Vinv, c->z, c->S, c->ea, c->xsol, c->dp, c->eb, c->vecA, c->vecB,
-                    c->vecC, c->vecD, c->mc_work, c->red_part, c->d_scal, c->Saux};
+                    c->vecC, c->vecD, c->mc_work, c->red_part, c->d_scal, c->Saux, c->snap};
   for (double* p : bufs)
     if (p) (void)hipFree(p);
   dfree(c->fix);
   dfree(c->counters);
   if (c->h_scal) (void)hipHostFree(c->h_scal);
+  for (auto& pr : c->probe_pending) {
+    if (pr.first) (void)hipEventDestroy(pr.first);
+    if (pr.second) (void)hipEventDestroy(pr.second);
+  }
+  for (auto e : c->probe_pool) (void)hipEventDestroy(e);
   if (c->ev0) (void)hipEventDestroy(c->ev0);
   if (c->ev1) (void)hipEventDestroy(c->ev1);
   if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -260,6 +292,24 @@ int psba_write_params(psba_ctx* c, const double* cams, const double* pts) {
   if (cams) PSBA_HIP_TRY(hipMemcpyAsync(c->cams, cams, sizeof(double) * c->n, hipMemcpyHostToDevice, c->stream));
   if (pts) PSBA_HIP_TRY(hipMemcpyAsync(c->pts, pts, sizeof(double) * 3 * c->np, hipMemcpyHostToDevice, c->stream));
   PSBA_HIP_TRY(hipStreamSynchronize(c->stream));
+  return PSBA_OK;
+}
+
+int psba_snapshot_params(psba_ctx* c) {
+  if (!c) return fail(PSBA_ERR_ARG, "null context");
+  PSBA_HIP_TRY(hipSetDevice(c->device));
+  if (!c->snap) PSBA_TRY(dalloc(&c->snap, (size_t)c->ntot));
+  PSBA_HIP_TRY(hipMemcpyAsync(c->snap, c->cams, sizeof(double) * c->n, hipMemcpyDeviceToDevice, c->stream));
+  PSBA_HIP_TRY(hipMemcpyAsync(c->snap + c->n, c->pts, sizeof(double) * 3 * c->np, hipMemcpyDeviceToDevice, c->stream));
+  PSBA_HIP_TRY(hipStreamSynchronize(c->stream));
+  return PSBA_OK;
+}
+
+int psba_restore_params(psba_ctx* c) {
+  if (!c || !c->snap) return fail(PSBA_ERR_STATE, "no parameter snapshot");
+  PSBA_HIP_TRY(hipSetDevice(c->device));
+  PSBA_HIP_TRY(hipMemcpyAsync(c->cams, c->snap, sizeof(double) * c->n, hipMemcpyDeviceToDevice, c->stream));
+  PSBA_HIP_TRY(hipMemcpyAsync(c->pts, c->snap + c->n, sizeof(double) * 3 * c->np, hipMemcpyDeviceToDevice, c->stream));
   return PSBA_OK;
 }
 
@@ -490,6 +540,26 @@ int psba_stage_modchol(psba_ctx* c, double* lambda) {
   return rc;
 }
 
+int psba_stage_load_S(psba_ctx* c, const double* S, const double* b) {
+  if (!c || !S) return fail(PSBA_ERR_ARG, "null argument");
+  PSBA_HIP_TRY(hipSetDevice(c->device));
+  PSBA_HIP_TRY(hipMemcpy2DAsync(c->S, sizeof(double) * c->n_pad, S, sizeof(double) * c->n, sizeof(double) * c->n,
+                                c->n, hipMemcpyHostToDevice, c->stream));
+  if (c->n_pad > c->n) {
+    // padding rows: identity (their first n columns are zero)
+    std::vector<double> pad((size_t)(c->n_pad - c->n) * c->n_pad, 0.0);
+    for (int r = c->n; r < c->n_pad; ++r) pad[(size_t)(r - c->n) * c->n_pad + r] = 1.0;
+    PSBA_HIP_TRY(hipMemcpyAsync(c->S + (size_t)c->n * c->n_pad, pad.data(), sizeof(double) * pad.size(),
+                                hipMemcpyHostToDevice, c->stream));
+  }
+  if (b) {
+    PSBA_HIP_TRY(hipMemsetAsync(c->ea, 0, sizeof(double) * c->n_pad, c->stream));
+    PSBA_HIP_TRY(hipMemcpyAsync(c->ea, b, sizeof(double) * c->n, hipMemcpyHostToDevice, c->stream));
+  }
+  PSBA_HIP_TRY(hipStreamSynchronize(c->stream));
+  return PSBA_OK;
+}
+
 long psba_buffer_size(const psba_ctx* c, int which) {
   if (!c) return -1;
   const long C = c->cnp;
@@ -555,6 +625,49 @@ int psba_read_buffer(psba_ctx* c, int which, double* out, long n) {
     }
   }
   return fail(PSBA_ERR_ARG, "unknown buffer %d", which);
+}
+
+int psba_get_info(psba_ctx* c, psba_info* o) {
+  if (!c || !o) return fail(PSBA_ERR_ARG, "null argument");
+  o->cnp = c->cnp;
+  o->n_cams = c->nc;
+  o->n_pts = c->np;
+  o->n_obs = c->no;
+  o->n = c->n;
+  o->n_pad = c->n_pad;
+  o->n_pairs = c->npairs;
+  o->n_pair_items = c->n_items;
+  o->sum_t2 = c->sum_t2;
+  return PSBA_OK;
+}
+
+int psba_probe(psba_ctx* c, int which) {
+  if (!c) return fail(PSBA_ERR_ARG, "null context");
+  c->probe_id = which;
+  return PSBA_OK;
+}
+
+int psba_probe_read(psba_ctx* c, double* total_ms, int* count) {
+  if (!c) return fail(PSBA_ERR_ARG, "null context");
+  PSBA_HIP_TRY(hipSetDevice(c->device));
+  PSBA_HIP_TRY(hipStreamSynchronize(c->stream));
+  double tot = 0;
+  int n = 0;
+  for (auto& pr : c->probe_pending) {
+    if (pr.first && pr.second) {
+      float ms = 0;
+      if (hipEventElapsedTime(&ms, pr.first, pr.second) == hipSuccess) {
+        tot += ms;
+        ++n;
+      }
+    }
+    if (pr.first) c->probe_pool.push_back(pr.first);
+    if (pr.second) c->probe_pool.push_back(pr.second);
+  }
+  c->probe_pending.clear();
+  if (total_ms) *total_ms = tot;
+  if (count) *count = n;
+  return PSBA_OK;
 }
 
 int psba_enable_timing(psba_ctx* c, int on) {

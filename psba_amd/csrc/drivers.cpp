@@ -101,6 +101,8 @@ int psba_levmar(psba_ctx* c, const psba_opts* o, int* itno, double* finalErr, ps
   PSBA_TRY(check_ready(c));
   if (!itno || !finalErr) return PSBA_ERR_ARG;
   const int max_iter = opt_max_iter(o);
+  const bool no_handoff = o && (o->flags & PSBA_FLAG_NO_HANDOFF);
+  const double rel_tol = o ? o->rel_tol : 0.0;
   double mu = 0.0, p_L2 = 0.0;
   int gooditer_cnt = 0, nu = 2;
   bool first = true, p_pending = false;
@@ -157,10 +159,17 @@ int psba_levmar(psba_ctx* c, const psba_opts* o, int* itno, double* finalErr, ps
           nu = 2;
           PSBA_TRY(launch_commit(c));
           p_pending = true;
+          const double rel_dec = (ex_L2 - new_ex_L2) / ex_L2;
           ex_L2 = new_ex_L2;
           rec.accepted = 1;
           push(trace, cap, len, rec);
-          if (std::fabs(rho - 1) < (1.0 / 5.0)) {
+          if (rel_tol > 0 && rel_dec < rel_tol) {
+            flag = PSBA_ITER_DP_NO_CHANGE;
+            break;
+          }
+          if (no_handoff) {
+            gooditer_cnt = 0;
+          } else if (std::fabs(rho - 1) < (1.0 / 5.0)) {
             gooditer_cnt++;
             if (gooditer_cnt >= 5) {
               flag = PSBA_ITER_TURN_TO_TR;

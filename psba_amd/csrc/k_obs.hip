@@ -210,8 +210,10 @@ __global__ __launch_bounds__(256) void k_cam_finish(const int* __restrict__ ccp,
 
 template <int CNP>
 static void run_jac_normal(psba_ctx* c, double coeff, double coeff_g) {
+  probe_begin(c, PSBA_PROBE_JACOBIAN);
   hipLaunchKernelGGL(k_jacobian<CNP>, dim3(ceil_div(c->no, 256)), dim3(256), 0, c->stream, c->K, c->q0, c->cams,
                      c->pts, c->meas, c->obs_cam, c->obs_pt, c->fix, c->no, coeff, c->e, c->JA, c->JB, c->W);
+  probe_end(c, PSBA_PROBE_JACOBIAN);
   hipLaunchKernelGGL(k_point_normal, dim3(ceil_div(c->np, 256)), dim3(256), 0, c->stream, c->pt_ptr, c->JB, c->e,
                      c->np, coeff, coeff_g, c->V, c->g + c->n, c->uvdiag + c->n);
   if (c->n_cchunk > 0)

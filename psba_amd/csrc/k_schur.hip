@@ -59,14 +59,22 @@ __global__ __launch_bounds__(256) void k_schur_chunks(const int2* __restrict__ i
 }
 
 // one block per lower block position (k >= l) of S, including empty ones
+// Blocks [0, nlb) write the lower block positions; the remaining blocks
+// rewrite the padding rows n..n_pad-1 as identity rows (a failed
+// factorisation may have left anything there).
 template <int CNP>
 __global__ __launch_bounds__(256) void k_schur_finish(const int* __restrict__ lblock_pair,
                                                       const int* __restrict__ psp, const double* __restrict__ part,
                                                       const double* __restrict__ U,
                                                       const unsigned char* __restrict__ fix, double mu, int ld,
-                                                      double* __restrict__ S) {
+                                                      int nlb, int n, double* __restrict__ S) {
   constexpr int C2 = CNP * CNP;
   const int b = blockIdx.x;
+  if (b >= nlb) {
+    const int r = n + (b - nlb);
+    for (int q = threadIdx.x; q <= r; q += blockDim.x) S[(size_t)r * ld + q] = (q == r) ? 1.0 : 0.0;
+    return;
+  }
   int k = (int)((sqrt(8.0 * b + 1.0) - 1.0) * 0.5);
   while ((k + 1) * (k + 2) / 2 <= b) ++k;
   while (k * (k + 1) / 2 > b) --k;
@@ -128,11 +136,15 @@ __global__ __launch_bounds__(64) void k_ea_finish(const int* __restrict__ ccp, c
 
 template <int CNP>
 static void run_schur(psba_ctx* c, double mu) {
-  if (c->n_schunk > 0)
+  if (c->n_schunk > 0) {
+    probe_begin(c, PSBA_PROBE_SCHUR);
     hipLaunchKernelGGL(k_schur_chunks<CNP>, dim3(c->n_schunk), dim3(256), 0, c->stream, c->pair_items,
                        c->schunk_begin, c->schunk_end, c->obs_pt, c->W, c->Vinv, c->schpart);
-  hipLaunchKernelGGL(k_schur_finish<CNP>, dim3(c->n_lblocks), dim3(CNP * CNP > 128 ? 256 : 128), 0, c->stream,
-                     c->lblock_pair, c->pair_schunk_ptr, c->schpart, c->U, c->fix, mu, c->n_pad, c->S);
+    probe_end(c, PSBA_PROBE_SCHUR);
+  }
+  hipLaunchKernelGGL(k_schur_finish<CNP>, dim3(c->n_lblocks + (c->n_pad - c->n)), dim3(CNP * CNP > 128 ? 256 : 128),
+                     0, c->stream, c->lblock_pair, c->pair_schunk_ptr, c->schpart, c->U, c->fix, mu, c->n_pad,
+                     c->n_lblocks, c->n, c->S);
   if (c->n_cchunk > 0)
     hipLaunchKernelGGL(k_ea_chunks<CNP>, dim3(c->n_cchunk), dim3(256), 0, c->stream, c->cam_obs, c->cchunk_begin,
                        c->cchunk_end, c->obs_pt, c->W, c->z, c->eapart);
