@@ -23,6 +23,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 METRIC = "time-to-converge vs plain FP64 at equal RMS; solves/s + p99 latency (stream)"
+FP64_PEAK_TFLOPS = 78.6  # MI355X dense FP64 (matrix and vector)
 
 
 def log(*a):
@@ -89,11 +90,17 @@ def main():
 
     for _ in range(a.warmup):
         solve()
+    # HIP events around every Schur-assembly launch on the context's stream
+    ctx.probe(P.PROBE_SCHUR)
+    ctx.probe_read()
     res = []
     t0 = time.perf_counter()
     for _ in range(a.steps):
-        res.append(solve())
+        res.append(solve())  # returns after its final device read-back (stream drained)
     dt = time.perf_counter() - t0
+    schur_ms, schur_n = ctx.probe_read()
+    ctx.probe(P.PROBE_OFF)
+    info = ctx.info()
     r = res[-1]
     trials = len(r["trace"])
     final = r["final_err"]
@@ -119,6 +126,19 @@ def main():
                    "n_obs": sc.n_obs, "sum_t2": sc.sum_t2, "lm_iterations": r["itno"], "lm_trials": trials,
                    "final_rms_px": rms},
     }
+    # Schur assembly S = U - sum_i W_ij V_i^-1 W_ik^T over the lower camera-pair
+    # blocks: per (point, pair j >= k of its cameras) one C x 3 by 3 x C product,
+    # 3 C^2 FMAs = 6 C^2 flops; sum over points of t(t+1)/2 such products.
+    C = sc.cnp
+    flops = 6.0 * C * C * info["n_pair_items"]
+    avg_s = schur_ms * 1e-3 / max(schur_n, 1)
+    achieved = flops / avg_s / 1e12
+    out["roofline"] = {"bound": "mfma", "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+                       "frac": achieved / FP64_PEAK_TFLOPS, "traffic": None,
+                       "kernel": "k_schur_chunks", "avg_launch_us": avg_s * 1e6, "launches": schur_n,
+                       "flops_per_launch": flops}
+    log("schur: %d launches, %.1f us avg, %.2f TFLOP/s (%.1f%% of FP64 peak)" % (
+        schur_n, avg_s * 1e6, achieved, 100 * achieved / FP64_PEAK_TFLOPS))
     if not a.no_cpu:
         out["cpu_baseline"] = cpu_baseline(sc, r["itno"], trials)
     print(json.dumps(out), flush=True)

@@ -87,6 +87,13 @@ class SceneOpts(ct.Structure):
                 ("size_class", ct.c_int), ("problem", ct.c_ulonglong)]
 
 
+class SbaData(ct.Structure):
+    _fields_ = [("n_cams", ct.c_int), ("n_pts", ct.c_int), ("n_obs", ct.c_int), ("cnp", ct.c_int),
+                ("pnp", ct.c_int), ("mnp", ct.c_int), ("motstruct", ct.POINTER(ct.c_double)),
+                ("initrot", ct.POINTER(ct.c_double)), ("imgpts", ct.POINTER(ct.c_double)),
+                ("covimgpts", ct.POINTER(ct.c_double)), ("vmask", ct.POINTER(ct.c_char))]
+
+
 _lib = None
 
 
@@ -133,6 +140,10 @@ def lib():
         "psba_scene_generate": (ct.POINTER(_SceneC), [ct.c_int, ct.c_ulonglong, ct.POINTER(SceneOpts)]),
         "psba_scene_free": (None, [ct.POINTER(_SceneC)]),
         "psba_problem_seed": (ct.c_ulonglong, [ct.c_ulonglong, ct.c_ulonglong]),
+        "psba_read_sba": (ct.c_int, [ct.c_char_p, ct.c_char_p, ct.c_int, ct.c_int, ct.c_int, ct.c_void_p, ct.c_int,
+                                     ct.POINTER(SbaData)]),
+        "psba_sba_free": (None, [ct.POINTER(SbaData)]),
+        "psba_quat2vec": (None, [dp, ct.c_int, dp, ct.c_int]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -159,6 +170,32 @@ def _check(rc, what):
 
 def device_count():
     return lib().psba_device_count()
+
+
+def read_sba(cams_path, pts_path, cnp=11, pnp=3, mnp=2, quat_filter=True, filecnp=None):
+    """PSBA/readparams.cpp:444-541 through psba_read_sba.  With quat_filter the
+    camera lines are passed through quat2vec (as PSBA/main.cpp:102-108 does),
+    each holding filecnp = cnp + 1 values.  Returns a dict of numpy copies."""
+    L = lib()
+    d = SbaData()
+    filt = ct.cast(L.psba_quat2vec, ct.c_void_p) if quat_filter else None
+    fc = (cnp + 1 if filecnp is None else filecnp) if quat_filter else 0
+    _check(L.psba_read_sba(os.fsencode(cams_path), os.fsencode(pts_path), cnp, pnp, mnp, filt, fc, ct.byref(d)),
+           "psba_read_sba")
+    try:
+        nc, npt, no = d.n_cams, d.n_pts, d.n_obs
+
+        def arr(ptr, n):
+            return np.ctypeslib.as_array(ptr, shape=(n,)).copy() if n else np.zeros(0)
+
+        ms = arr(d.motstruct, nc * cnp + npt * pnp)
+        out = dict(n_cams=nc, n_pts=npt, n_obs=no, cams=ms[:nc * cnp], pts=ms[nc * cnp:],
+                   initrot=arr(d.initrot, 4 * nc), imgpts=arr(d.imgpts, no * mnp),
+                   cov=arr(d.covimgpts, no * mnp * mnp) if d.covimgpts else None,
+                   vmask=np.frombuffer(ct.string_at(d.vmask, npt * nc), np.int8).reshape(npt, nc).copy())
+    finally:
+        L.psba_sba_free(ct.byref(d))
+    return out
 
 
 class Scene:

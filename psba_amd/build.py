@@ -23,7 +23,7 @@ ORACLE_LIB = os.path.join(ORACLE_DIR, "liboracle.so")
 ARCH = os.environ.get("PSBA_OFFLOAD_ARCH", "gfx950")
 HIPCC = os.environ.get("HIPCC", shutil.which("hipcc") or "/opt/rocm/bin/hipcc")
 
-SOURCES = ["ctx.cpp", "drivers.cpp", "dropin.cpp", "scene.cpp",
+SOURCES = ["ctx.cpp", "drivers.cpp", "dropin.cpp", "scene.cpp", "sba_io.cpp",
            "k_obs.hip", "k_schur.hip", "k_dense.hip", "k_tr.hip"]
 
 HIP_FLAGS = [
@@ -63,6 +63,19 @@ def build_lib(jobs=8, verbose=False):
     return LIB
 
 
+DROPIN_SRC = os.path.join(ROOT, "tests", "dropin", "main_dropin.cpp")
+DROPIN_BIN = os.path.join(ROOT, "tests", "dropin", "main_dropin")
+
+
+def build_dropin():
+    """The reference-style host program of tests/dropin, linked to libpsba.so."""
+    tmp = DROPIN_BIN + ".tmp"
+    _run(["g++", "-std=c++17", "-O2", "-Wall", "-I" + os.path.join(ROOT, "include"), DROPIN_SRC,
+          "-L" + os.path.dirname(LIB), "-lpsba", "-Wl,-rpath,$ORIGIN/../../psba_amd", "-o", tmp])
+    os.replace(tmp, DROPIN_BIN)
+    return DROPIN_BIN
+
+
 def build_oracle():
     src = os.path.join(ORACLE_DIR, "oracle.c")
     tmp = ORACLE_LIB + ".tmp"
@@ -84,6 +97,7 @@ def main(argv=None):
         print("built", build_oracle())
     if a.lib or both:
         print("built", build_lib(a.j, a.v))
+        print("built", build_dropin())
 
 
 if __name__ == "__main__":

@@ -148,6 +148,11 @@ int psba_create(const psba_desc* d, psba_ctx** out) {
   if (d->pnp != 3 || d->mnp != 2) return fail(PSBA_ERR_UNSUPPORTED, "pnp must be 3 and mnp 2");
   if (d->n_cams <= 0 || d->n_pts <= 0 || d->n_obs <= 0) return fail(PSBA_ERR_ARG, "empty problem");
   if (d->precision != PSBA_PREC_FP64) return fail(PSBA_ERR_UNSUPPORTED, "precision %d not available", d->precision);
+  {
+    long n = (long)d->cnp * d->n_cams, n_pad = (n + kNB - 1) / kNB * kNB;
+    if (n_pad > kMaxSolveN)
+      return fail(PSBA_ERR_UNSUPPORTED, "reduced camera system of order %ld exceeds the limit %d", n, kMaxSolveN);
+  }
   int ndev = 0;
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return fail(PSBA_ERR_NODEVICE, "no HIP device visible");
   if (d->device < 0 || d->device >= ndev) return fail(PSBA_ERR_ARG, "device %d out of range", d->device);

@@ -10,6 +10,7 @@
 #include "levmar.h"
 #include "misc.h"
 #include "psba_capi.h"
+#include "psba_io.h"
 #include "trust_region.h"
 
 // The shared iteration counter.  The application normally defines it (as
@@ -86,19 +87,7 @@ int trust_region(PSBA_structPtr p, int cnp, int pnp, int mnp, int n3Dpts, int nC
 }
 
 // PSBA/misc.cpp:21-49
-void quat2vec(dtype* inp, int nin, dtype* outp, int nout) {
-  int i = 0;
-  if (nin > 7)
-    for (; i < nin - 7; ++i) outp[i] = inp[i];
-  double mag = std::sqrt(inp[i] * inp[i] + inp[i + 1] * inp[i + 1] + inp[i + 2] * inp[i + 2] + inp[i + 3] * inp[i + 3]);
-  double sg = (inp[i] >= 0.0) ? 1.0 : -1.0;
-  mag = sg / mag;
-  outp[i] = inp[i + 1] * mag;
-  outp[i + 1] = inp[i + 2] * mag;
-  outp[i + 2] = inp[i + 3] * mag;
-  i += 3;
-  for (; i < nout; ++i) outp[i] = inp[i + 1];
-}
+void quat2vec(dtype* inp, int nin, dtype* outp, int nout) { psba_quat2vec(inp, nin, outp, nout); }
 
 // PSBA/misc.cpp:60-85
 void vec2quat(dtype* inp, int nin, dtype* outp, int nout) {

@@ -22,10 +22,12 @@ def pytest_configure(config):
 def _ensure_built():
     lib = os.path.join(ROOT, "psba_amd", "libpsba.so")
     orc = os.path.join(ROOT, "oracle", "liboracle.so")
-    if not (os.path.exists(lib) and os.path.exists(orc)):
+    dropin = os.path.join(ROOT, "tests", "dropin", "main_dropin")
+    if not (os.path.exists(lib) and os.path.exists(orc) and os.path.exists(dropin)):
         from psba_amd import build
         build.build_oracle()
         build.build_lib()
+        build.build_dropin()
 
 
 _ensure_built()

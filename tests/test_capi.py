@@ -1,0 +1,105 @@
+"""The C-ABI library as a drop-in: every function declared in include/*.h is
+exported by libpsba.so (C linkage for psba_*, C++ linkage for the reference's
+own entry points), the reference-style host program builds against the
+headers, and without a device the API fails loudly with PSBA_ERR_NODEVICE."""
+import os
+import re
+import subprocess
+
+import pytest
+
+from conftest import ROOT
+
+INCLUDE = os.path.join(ROOT, "include")
+LIB = os.path.join(ROOT, "psba_amd", "libpsba.so")
+DROPIN_NAMES = {"setup_cl", "release_buffer", "fill_initBuffer2", "fill_idxBuffer", "read_params", "levmar",
+                "trust_region", "quat2vec", "vec2quat", "compute_L2_sq", "dotProduct", "generate_idxs",
+                "readInitialSBAEstimate"}
+
+
+def declared_functions():
+    names = set()
+    for h in os.listdir(INCLUDE):
+        if not h.endswith(".h"):
+            continue
+        text = open(os.path.join(INCLUDE, h)).read()
+        text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+        text = re.sub(r"//[^\n]*", "", text)
+        for m in re.finditer(r"^[A-Za-z_][\w \*]*?[\s\*]([A-Za-z_]\w*)\s*\(", text, flags=re.M):
+            name = m.group(1)
+            if name in ("typedef", "void", "if", "return") or m.group(0).startswith("typedef"):
+                continue
+            names.add(name)
+    return names
+
+
+def exported(demangle):
+    out = subprocess.run(["nm", "-D", "--defined-only"] + (["-C"] if demangle else []) + [LIB],
+                         stdout=subprocess.PIPE, text=True, check=True).stdout
+    return out
+
+
+def test_every_declared_function_is_exported():
+    names = declared_functions()
+    assert {"psba_create", "psba_levmar", "psba_trust_region", "psba_read_sba", "levmar", "trust_region",
+            "setup_cl", "readInitialSBAEstimate"} <= names
+    plain = set(l.split()[-1] for l in exported(False).splitlines())
+    dem = exported(True)
+    missing = []
+    for n in sorted(names):
+        if n in DROPIN_NAMES:
+            if not re.search(r"\b%s\(" % re.escape(n), dem):
+                missing.append(n)
+        elif n not in plain:
+            missing.append(n)
+    assert not missing, missing
+    assert "itno" in plain  # the shared iteration counter (weak, PSBA/main.cpp:27)
+
+
+def test_reference_style_program_links(tmp_path):
+    out = str(tmp_path / "main_dropin")
+    r = subprocess.run(["g++", "-std=c++17", "-O1", "-Wall", "-Werror", "-I" + INCLUDE,
+                        os.path.join(ROOT, "tests", "dropin", "main_dropin.cpp"), "-L" + os.path.dirname(LIB),
+                        "-lpsba", "-o", out], stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
+    assert r.returncode == 0, r.stdout
+
+
+def test_c_headers_compile_as_c(tmp_path):
+    src = tmp_path / "t.c"
+    src.write_text('#include "psba_capi.h"\n#include "psba_scene.h"\n#include "psba_io.h"\n'
+                   "int main(void) { psba_desc d = {6, 3, 2, 1, 1, 1, PSBA_PREC_FP64, 0}; (void)d;"
+                   " return psba_device_count() < 0; }\n")
+    r = subprocess.run(["gcc", "-std=c99", "-Wall", "-Werror", "-I" + INCLUDE, str(src), "-L" + os.path.dirname(LIB),
+                        "-lpsba", "-o", str(tmp_path / "t")], stdout=subprocess.PIPE, stderr=subprocess.STDOUT,
+                       text=True)
+    assert r.returncode == 0, r.stdout
+
+
+def test_status_strings(P):
+    L = P.lib()
+    for code in (0, -1, -2, -3, -4, -5, -6):
+        s = L.psba_strerror(code)
+        assert s and len(s) >= 2
+    assert L.psba_strerror(-99)
+
+
+def test_no_device_fails_loudly(P):
+    if P.device_count() > 0:
+        pytest.skip("a HIP device is present")
+    with pytest.raises(P.PsbaError, match="no HIP device"):
+        P.Context(6, 3, 10, 20)
+    r = subprocess.run([os.path.join(ROOT, "tests", "dropin", "main_dropin"),
+                        os.path.join(ROOT, "tests", "golden", "7camsvarK.txt"),
+                        os.path.join(ROOT, "tests", "golden", "7pts.txt"), os.devnull],
+                       stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
+    assert r.returncode == 1 and "setup_cl failed" in r.stdout
+
+
+def test_argument_checks_precede_device_use(P):
+    """Sizes and models are validated before any device is touched."""
+    with pytest.raises(P.PsbaError, match="exceeds the limit"):
+        P.Context(9, 1000, 100, 200)  # reduced system of order 9000 > 8192
+    with pytest.raises(P.PsbaError, match="cnp=7"):
+        P.Context(7, 3, 10, 20)
+    with pytest.raises(P.PsbaError, match="empty"):
+        P.Context(6, 3, 0, 20)

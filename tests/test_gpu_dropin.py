@@ -1,0 +1,127 @@
+"""The drop-in boundary end to end on the device, and the full-size sparse
+scene.
+
+* tests/dropin/main_dropin follows PSBA/main.cpp:88-208 against the
+  reference's headers (readInitialSBAEstimate, setup_cl, fill_initBuffer2,
+  generate_idxs, fill_idxBuffer, levmar <-> trust_region, read_params) on the
+  reference's own data files; its result must equal the C-ABI hybrid driver
+  bit for bit and reach the oracle's minimum.
+* The 88-camera / 64,298-point scene: one full LM iteration's stages against
+  the oracle at full size, and size-independent properties of a whole solve
+  (determinism, monotone accepted costs).
+"""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+from conftest import GOLDEN, ROOT, rel
+
+pytestmark = pytest.mark.gpu
+
+DROPIN = os.path.join(ROOT, "tests", "dropin", "main_dropin")
+
+
+def load_reference_problem(P, cams_name, pts_name):
+    """What PSBA/main.cpp:102-150 hands to the drivers: cnp 6, K split off,
+    the vector part of the pose cleared (the rotation lives in initrot)."""
+    d = P.read_sba(os.path.join(GOLDEN, cams_name), os.path.join(GOLDEN, pts_name))
+    nc = d["n_cams"]
+    c11 = d["cams"].reshape(nc, 11)
+    K = c11[:, :5].reshape(-1).copy()
+    cams = c11[:, 5:].copy()
+    cams[:, :3] = 0.0
+    vm = d["vmask"]
+    iidx, jidx = np.nonzero(vm)  # point-major, camera-ascending (generate_idxs order)
+    return dict(K=K, cams=cams.reshape(-1), pts=d["pts"], q0=d["initrot"], meas=d["imgpts"],
+                iidx=iidx.astype(np.int32), jidx=jidx.astype(np.int32), n_cams=nc, n_pts=d["n_pts"],
+                n_obs=d["n_obs"])
+
+
+def run_dropin(cams, pts, tmp_path):
+    out = str(tmp_path / "params.bin")
+    r = subprocess.run([DROPIN, cams, pts, out], stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True,
+                       timeout=120)
+    assert r.returncode == 0, r.stdout
+    vals = dict(l.split("=", 1) for l in r.stdout.splitlines() if "=" in l)
+    return int(vals["iter_flag"]), int(vals["itno"]), float(vals["final_err"]), np.fromfile(out, np.float64)
+
+
+@pytest.mark.parametrize("cams_name,pts_name", [("7camsvarK.txt", "7pts.txt"), ("9camsvarK.txt", "9pts.txt")])
+def test_reference_program_on_reference_data(P, O, gpu, tmp_path, cams_name, pts_name):
+    flag, itno, final, params = run_dropin(os.path.join(GOLDEN, cams_name), os.path.join(GOLDEN, pts_name),
+                                           tmp_path)
+    pr = load_reference_problem(P, cams_name, pts_name)
+    with P.Context(6, pr["n_cams"], pr["n_pts"], pr["n_obs"]) as g:
+        g.upload(pr["K"], pr["meas"], pr["q0"], pr["cams"], pr["pts"])
+        g.set_index(pr["iidx"], pr["jidx"])
+        r = g.hybrid()
+        assert (r["code"], r["itno"]) == (flag, itno)
+        assert r["final_err"] == final
+        assert np.array_equal(np.concatenate(g.read_params()), params)
+    # the oracle on the same inputs (no camera fixed, as in the reference)
+    o = O.Oracle(6, pr["K"], pr["q0"], pr["meas"], pr["iidx"], pr["jidx"], pr["cams"], pr["pts"], None)
+    ro = o.hybrid()
+    assert abs(final - ro["final_err"]) <= 1e-6 * ro["final_err"]
+
+
+def test_reference_data_lm_phase_trial_by_trial(P, O, gpu):
+    pr = load_reference_problem(P, "7camsvarK.txt", "7pts.txt")
+    with P.Context(6, pr["n_cams"], pr["n_pts"], pr["n_obs"]) as g:
+        g.upload(pr["K"], pr["meas"], pr["q0"], pr["cams"], pr["pts"])
+        g.set_index(pr["iidx"], pr["jidx"])
+        rg = g.levmar()
+        o = O.Oracle(6, pr["K"], pr["q0"], pr["meas"], pr["iidx"], pr["jidx"], pr["cams"], pr["pts"], None)
+        ro = o.levmar()
+        assert (rg["code"], rg["itno"]) == (ro["code"], ro["itno"])
+        assert len(rg["trace"]) == len(ro["trace"])
+        for a, b in zip(rg["trace"], ro["trace"]):
+            assert (a.accepted, a.factor_ok) == (b.accepted, b.factor_ok)
+            assert abs(a.cost - b.cost) <= 1e-9 * b.cost
+
+
+@pytest.fixture(scope="module")
+def c2_full(P):
+    return P.Scene(2, 2024)
+
+
+def test_full_size_iteration_matches_oracle(P, O, gpu, c2_full):
+    sc = c2_full
+    assert (sc.n_cams, sc.n_pts) == (88, 64298)
+    O.set_threads(16)
+    g = P.Context.from_scene(sc)
+    o = O.Oracle.from_scene(sc)
+    assert rel(g.residual(), o.residual()) < 1e-13
+    g.jacobian_normal()
+    o.jacobian_normal()
+    for b in ("U", "V", "G"):
+        assert rel(g.buffer(b), o.buffer(b)) < 1e-12, b
+    mu = 1e-3 * o.max_uvdiag()
+    g.schur(mu)
+    o.schur(mu)
+    n = sc.cnp * sc.n_cams
+    tl = np.tril_indices(n)
+    assert rel(g.buffer("S").reshape(n, n)[tl], o.buffer("S").reshape(n, n)[tl]) < 1e-11
+    assert rel(g.buffer("EA"), o.buffer("EA")) < 1e-11
+    assert g.factor_solve() and o.factor_solve()
+    g.backsub()
+    o.backsub()
+    assert rel(g.buffer("DP"), o.buffer("DP")) < 1e-9
+
+
+def test_full_size_solve_is_deterministic_and_monotone(P, gpu, c2_full):
+    sc = c2_full
+    g = P.Context.from_scene(sc)
+    init = g.residual()
+    g.snapshot()
+    r1 = g.levmar(max_iter=100, flags=P.FLAG_NO_HANDOFF, rel_tol=1e-8)
+    p1 = np.concatenate(g.read_params())
+    g.restore()
+    g.itno = 0
+    r2 = g.levmar(max_iter=100, flags=P.FLAG_NO_HANDOFF, rel_tol=1e-8)
+    p2 = np.concatenate(g.read_params())
+    assert r1["final_err"] == r2["final_err"] and np.array_equal(p1, p2)
+    acc = [t.cost for t in r1["trace"] if t.accepted]
+    assert len(acc) > 5 and all(b < a for a, b in zip(acc, acc[1:]))
+    assert r1["final_err"] < init

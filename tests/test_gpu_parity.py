@@ -1,0 +1,241 @@
+"""HIP path against the oracle on the same seeded scenes, through the C-ABI.
+
+Stage by stage (residual, Jacobian/normal equations, V^-1, Schur complement,
+reduced right-hand side, factor/solve, back-substitution) and whole drivers
+(LM per trial, dogleg and the hybrid loop with the scale gauge fixed), for
+every camera model (cnp 6, 9, 11, 16).  The plain FP64 path must reproduce
+the oracle's per-trial cost and final parameters within 1e-9 relative.
+"""
+import numpy as np
+import pytest
+
+from conftest import rel
+
+pytestmark = pytest.mark.gpu
+
+STAGE_SCENES = [(0, {}), (1, dict(n_pts=600)), (1, dict(cnp=16, n_pts=600)), (2, dict(n_pts=3000, n_cams=20))]
+IDS = ["c0-cnp6", "c1-cnp11", "c1-cnp16", "c2-cnp9"]
+
+
+def pair(P, O, sc, fix=True):
+    return P.Context.from_scene(sc, fix=fix), O.Oracle.from_scene(sc, fix=fix)
+
+
+def fix_scale(sc):
+    """Hold the first translation coordinate of camera 1 as well: with camera 0
+    fixed this removes the scale freedom, so S is nonsingular at lambda = 0."""
+    C = sc.cnp
+    sc.fixmask[C + C - 3] = 1
+    return sc
+
+
+@pytest.mark.parametrize("config,kw", STAGE_SCENES, ids=IDS)
+def test_stages_match_oracle(P, O, gpu, config, kw):
+    sc = P.Scene(config, 1000 + config, **kw)
+    g, o = pair(P, O, sc)
+    assert rel(g.residual(), o.residual()) < 1e-13
+    for coeff, coeff_g in ((1.0, 1.0), (2.0, -2.0)):
+        g.jacobian_normal(coeff, coeff_g)
+        o.jacobian_normal(coeff, coeff_g)
+        for b in ("EX", "JA", "JB", "U", "V", "W", "G"):
+            assert rel(g.buffer(b), o.buffer(b)) < 1e-12, b
+    g.jacobian_normal(1.0, 1.0)
+    o.jacobian_normal(1.0, 1.0)
+    mu = 1e-3 * o.max_uvdiag()
+    g.schur(mu)
+    o.schur(mu)
+    assert rel(g.buffer("VINV"), o.buffer("VINV")) < 1e-12
+    assert rel(g.buffer("EA"), o.buffer("EA")) < 1e-11
+    n = sc.cnp * sc.n_cams
+    tl = np.tril_indices(n)
+    assert rel(g.buffer("S").reshape(n, n)[tl], o.buffer("S").reshape(n, n)[tl]) < 1e-11
+    assert g.factor_solve() and o.factor_solve()
+    g.backsub()
+    o.backsub()
+    assert rel(g.buffer("DP"), o.buffer("DP")) < 1e-9
+    assert rel(g.buffer("EB"), o.buffer("EB")) < 1e-11
+
+
+def test_schur_is_deterministic(P, gpu):
+    sc = P.Scene(2, 7, n_pts=4000, n_cams=24)
+    a = P.Context.from_scene(sc)
+    b = P.Context.from_scene(sc)
+    out = []
+    for c in (a, a, b):
+        c.jacobian_normal()
+        c.schur(1.0)
+        out.append((c.buffer("S"), c.buffer("EA")))
+    for S, ea in out[1:]:
+        assert np.array_equal(S, out[0][0]) and np.array_equal(ea, out[0][1])
+
+
+def compare_traces(tg, to, tol=1e-9, damp=True):
+    assert len(tg) == len(to)
+    for a, b in zip(tg, to):
+        assert (a.itno, a.kind, a.factor_ok, a.accepted) == (b.itno, b.kind, b.factor_ok, b.accepted)
+        if b.factor_ok:
+            assert abs(a.cost - b.cost) <= tol * abs(b.cost)
+        if damp:
+            assert abs(a.damp - b.damp) <= tol * abs(b.damp)
+
+
+LM_SCENES = [(0, {}), (1, dict(n_pts=1500)), (1, dict(cnp=16, n_pts=1500)), (2, dict(n_pts=6000, n_cams=30))]
+
+
+@pytest.mark.parametrize("config,kw", LM_SCENES, ids=IDS)
+def test_levmar_trial_by_trial(P, O, gpu, config, kw):
+    sc = P.Scene(config, 2000 + config, **kw)
+    g, o = pair(P, O, sc)
+    rg = g.levmar()
+    ro = o.levmar()
+    assert (rg["code"], rg["itno"]) == (ro["code"], ro["itno"])
+    compare_traces(rg["trace"], ro["trace"])
+    assert rel(rg["final_err"], ro["final_err"]) < 1e-9
+    assert rel(np.concatenate(g.read_params()), np.concatenate(o.read_params())) < 1e-9
+
+
+def test_levmar_pure_with_relative_stop(P, O, gpu):
+    """The solve measured by the benchmark: LM only, stopping on a relative
+    cost decrease below 1e-8 or 100 iterations."""
+    sc = P.Scene(2, 2100, n_pts=8000, n_cams=40)
+    g, o = pair(P, O, sc)
+    rg = g.levmar(max_iter=100, flags=P.FLAG_NO_HANDOFF, rel_tol=1e-8)
+    ro = o.levmar(max_iter=100, flags=O.FLAG_NO_HANDOFF, rel_tol=1e-8)
+    assert (rg["code"], rg["itno"]) == (ro["code"], ro["itno"])
+    compare_traces(rg["trace"], ro["trace"])
+    assert rel(np.concatenate(g.read_params()), np.concatenate(o.read_params())) < 1e-9
+
+
+@pytest.mark.parametrize("config,kw", [(0, {}), (1, dict(n_pts=800))], ids=["c0", "c1"])
+def test_dogleg_after_lm_with_scale_fixed(P, O, gpu, config, kw):
+    sc = fix_scale(P.Scene(config, 3000 + config, **kw))
+    g, o = pair(P, O, sc)
+    g.levmar()
+    o.levmar()
+    rg = g.trust_region()
+    ro = o.trust_region()
+    assert (rg["code"], rg["itno"]) == (ro["code"], ro["itno"])
+    compare_traces(rg["trace"], ro["trace"], tol=1e-9)
+    for a, b in zip(rg["trace"], ro["trace"]):
+        assert abs(a.lambda_ - b.lambda_) <= 1e-9 * max(abs(b.lambda_), 1e-300)
+    assert rel(np.concatenate(g.read_params()), np.concatenate(o.read_params())) < 1e-9
+
+
+@pytest.mark.parametrize("config,kw", [(0, {}), (1, dict(n_pts=800))], ids=["c0", "c1"])
+def test_hybrid_with_scale_fixed(P, O, gpu, config, kw):
+    sc = fix_scale(P.Scene(config, 4000 + config, **kw))
+    g, o = pair(P, O, sc)
+    rg = g.hybrid()
+    ro = o.hybrid()
+    assert (rg["code"], rg["itno"]) == (ro["code"], ro["itno"])
+    compare_traces(rg["trace"], ro["trace"])
+    assert rel(rg["final_err"], ro["final_err"]) < 1e-9
+    assert rel(np.concatenate(g.read_params()), np.concatenate(o.read_params())) < 1e-9
+
+
+def test_dogleg_from_the_start(P, O, gpu):
+    """trust_region() entered first (Delta = 1, lambda = 0), scale fixed."""
+    sc = fix_scale(P.Scene(0, 4100))
+    g, o = pair(P, O, sc)
+    rg = g.trust_region()
+    ro = o.trust_region()
+    assert (rg["code"], rg["itno"]) == (ro["code"], ro["itno"])
+    compare_traces(rg["trace"], ro["trace"])
+
+
+# ---- dense reduced-system kernels on loaded matrices ----
+
+def spd(n, rng, cond=1e4):
+    Q, _ = np.linalg.qr(rng.standard_normal((n, n)))
+    return (Q * np.logspace(0, np.log10(cond), n)) @ Q.T
+
+
+@pytest.mark.parametrize("n_cams", [1, 7, 11, 100, 300])
+def test_dense_factor_solve_matches_lapack(P, gpu, n_cams):
+    rng = np.random.default_rng(n_cams)
+    n = 6 * n_cams
+    with P.Context(6, n_cams, 4, 8) as c:
+        S = spd(n, rng)
+        b = rng.standard_normal(n)
+        c.load_S(S, b)
+        ok, x = c.factor_only()
+        assert ok
+        assert rel(x, np.linalg.solve(S, b)) < 1e-9
+        L = c.buffer("L").reshape(n, n)
+        assert rel(np.tril(L), np.linalg.cholesky(S)) < 1e-11
+
+
+def test_dense_factor_reports_indefinite(P, gpu):
+    rng = np.random.default_rng(9)
+    n = 6 * 20
+    with P.Context(6, 20, 4, 8) as c:
+        S = spd(n, rng)
+        S[77, 77] = -1.0
+        c.load_S(S, np.ones(n))
+        ok, _ = c.factor_only()
+        assert not ok
+        # the padding must survive a failed factorization: a good matrix after it factors
+        S2 = spd(n, rng)
+        c.load_S(S2, np.ones(n))
+        ok, x = c.factor_only()
+        assert ok and rel(x, np.linalg.solve(S2, np.ones(n))) < 1e-9
+
+
+@pytest.mark.parametrize("cnp,n_cams", [(6, 10), (11, 8), (16, 7)])
+def test_modchol_matches_oracle(P, O, gpu, cnp, n_cams):
+    rng = np.random.default_rng(cnp)
+    n = cnp * n_cams
+    Q, _ = np.linalg.qr(rng.standard_normal((n, n)))
+    d = np.concatenate([-np.linspace(1, 5, n // 4), np.linspace(1, 10, n - n // 4)])
+    S = (Q * d) @ Q.T
+    with P.Context(cnp, n_cams, 4, 8) as c:
+        c.load_S(S)
+        lam = c.modchol()
+        lam_o, L_o, E_o = O.dense_modchol(S)
+        assert abs(lam - lam_o) <= 1e-9 * lam_o
+        assert rel(np.tril(c.buffer("L").reshape(n, n)), np.tril(L_o)) < 1e-9
+
+
+# ---- edge cases ----
+
+def test_ragged_tracks_and_unseen_camera(P, O, gpu):
+    """Points left with a single observation (rank-2 V, damped by mu) and a
+    camera with no observation at all (zero U block)."""
+    sc = P.Scene(0, 5000, n_pts=300)
+    rng = np.random.default_rng(0)
+    keep = np.ones(sc.n_obs, bool)
+    # drop all of camera 3's observations, and all but one of some tracks
+    keep[sc.jidx == 3] = False
+    for i in rng.choice(sc.n_pts, 40, replace=False):
+        ks = np.nonzero((sc.iidx == i) & keep)[0]
+        keep[ks[1:]] = False
+    # points still need one observation each
+    seen = np.zeros(sc.n_pts, bool)
+    seen[sc.iidx[keep]] = True
+    assert seen.all()
+    sc.iidx, sc.jidx = sc.iidx[keep], sc.jidx[keep]
+    sc.meas = sc.meas.reshape(-1, 2)[keep].reshape(-1)
+    sc.n_obs = int(keep.sum())
+    g, o = pair(P, O, sc)
+    rg = g.levmar(max_iter=30, flags=P.FLAG_NO_HANDOFF)
+    ro = o.levmar(max_iter=30, flags=O.FLAG_NO_HANDOFF)
+    assert (rg["code"], rg["itno"]) == (ro["code"], ro["itno"])
+    compare_traces(rg["trace"], ro["trace"])
+
+
+def test_bad_index_is_rejected(P, gpu):
+    sc = P.Scene(0, 1)
+    c = P.Context(sc.cnp, sc.n_cams, sc.n_pts, sc.n_obs)
+    c.upload(sc.K, sc.meas, sc.q0, sc.cams, sc.pts)
+    j = sc.jidx.copy()
+    k = int(np.nonzero(sc.iidx[1:] == sc.iidx[:-1])[0][0])
+    j[k], j[k + 1] = j[k + 1], j[k]  # camera order within a point broken
+    with pytest.raises(P.PsbaError):
+        c.set_index(sc.iidx, j)
+    i = sc.iidx.copy()
+    i[0] = sc.n_pts  # point index out of range
+    with pytest.raises(P.PsbaError):
+        c.set_index(i, sc.jidx)
+    with pytest.raises(P.PsbaError):
+        c.levmar()  # no index: state error, nothing launched
+
