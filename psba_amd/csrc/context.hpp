@@ -57,6 +57,7 @@ struct psba_ctx {
   int n = 0, ntot = 0, n_pad = 0;
   int precision = 0, device = 0;
   bool uploaded = false, indexed = false;
+  bool s_loaded = false;  // a reduced system was loaded directly (psba_stage_load_S)
   hipStream_t stream = nullptr;
 
   // problem data

@@ -489,6 +489,15 @@ static int ready(psba_ctx* c) {
   return PSBA_OK;
 }
 
+// the dense stages also run on a reduced system loaded with psba_stage_load_S
+static int ready_dense(psba_ctx* c) {
+  if (c && c->s_loaded) {
+    PSBA_HIP_TRY(hipSetDevice(c->device));
+    return PSBA_OK;
+  }
+  return ready(c);
+}
+
 int psba_stage_residual(psba_ctx* c, int use_trial, double* cost) {
   PSBA_TRY(ready(c));
   PSBA_TRY(launch_residual(c, use_trial != 0));
@@ -514,7 +523,7 @@ int psba_stage_schur(psba_ctx* c, double mu) {
 }
 
 int psba_stage_factor_solve(psba_ctx* c, int* ok) {
-  PSBA_TRY(ready(c));
+  PSBA_TRY(ready_dense(c));
   PSBA_TRY(launch_factor(c));
   PSBA_TRY(launch_solve(c));
   PSBA_HIP_TRY(hipMemcpyAsync(c->h_scal, c->d_scal, sizeof(double) * SC_COUNT, hipMemcpyDeviceToHost, c->stream));
@@ -531,7 +540,7 @@ int psba_stage_backsub(psba_ctx* c) {
 }
 
 int psba_stage_modchol(psba_ctx* c, double* lambda) {
-  PSBA_TRY(ready(c));
+  PSBA_TRY(ready_dense(c));
   double* A = nullptr;
   PSBA_TRY(dalloc(&A, (size_t)c->n_pad * c->n_pad));
   int rc = launch_copy_S(c, A, c->S);
@@ -562,6 +571,7 @@ int psba_stage_load_S(psba_ctx* c, const double* S, const double* b) {
     PSBA_HIP_TRY(hipMemcpyAsync(c->ea, b, sizeof(double) * c->n, hipMemcpyHostToDevice, c->stream));
   }
   PSBA_HIP_TRY(hipStreamSynchronize(c->stream));
+  c->s_loaded = true;
   return PSBA_OK;
 }
 

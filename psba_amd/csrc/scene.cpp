@@ -355,10 +355,12 @@ psba_scene* psba_scene_generate(int config, unsigned long long seed, const psba_
       sp.track_mode = 1; sp.track_dist = 1; sp.tmin = 2; sp.tmax = std::min(20, sp.nc); sp.geo_p = 0.26;
       for (int k = 0; k < 3; ++k) { sp.box_lo[k] = -3; sp.box_hi[k] = 3; }
       for (int j = 0; j < sp.nc; ++j) {
+        // an arc at uneven distances, each camera aimed near the origin
         double ph = (-60.0 + 120.0 * j / std::max(1, sp.nc - 1)) * M_PI / 180.0;
-        double c[3] = {20 * std::cos(ph), 20 * std::sin(ph), 1.0 * std::sin(2 * ph)};
+        double r = 20.0 * (1.0 + 0.15 * rng.normal());
+        double c[3] = {r * std::cos(ph), r * std::sin(ph), 2.0 * std::sin(2 * ph) + 0.5 * rng.normal()};
         centres.insert(centres.end(), c, c + 3);
-        targets.insert(targets.end(), {0, 0, 0});
+        targets.insert(targets.end(), {0.7 * rng.normal(), 0.7 * rng.normal(), 0.7 * rng.normal()});
         K.insert(K.end(), {800, 320, 240, 1, 0});
         double fx = 800 * (1 + 0.02 * rng.normal());
         double cx = 320 + 5 * rng.normal(), cy = 240 + 5 * rng.normal(), ar = 1 + 0.005 * rng.normal();

@@ -21,11 +21,17 @@ def pair(P, O, sc, fix=True):
     return P.Context.from_scene(sc, fix=fix), O.Oracle.from_scene(sc, fix=fix)
 
 
-def fix_scale(sc):
-    """Hold the first translation coordinate of camera 1 as well: with camera 0
-    fixed this removes the scale freedom, so S is nonsingular at lambda = 0."""
+def fix_gauge(sc):
+    """Make the reduced system nonsingular at lambda = 0, which the dogleg
+    needs.  Camera 0 is already fixed; hold camera 1's first translation
+    coordinate too (the scale).  With all five intrinsics free (cnp >= 11)
+    every camera is a general projective camera, so the 15-parameter
+    projective ambiguity leaves 3 more flat directions: hold every camera's
+    skew (the zero-skew constraint) as well."""
     C = sc.cnp
     sc.fixmask[C + C - 3] = 1
+    if C >= 11:
+        sc.fixmask.reshape(-1, C)[:, 4] = 1
     return sc
 
 
@@ -69,14 +75,33 @@ def test_schur_is_deterministic(P, gpu):
         assert np.array_equal(S, out[0][0]) and np.array_equal(ea, out[0][1])
 
 
-def compare_traces(tg, to, tol=1e-9, damp=True):
+def close(a, b, tol):
+    """|a - b| <= tol |b|; two NaNs (a trial whose cost was never evaluated,
+    e.g. the one that stops on a vanishing step) agree."""
+    if np.isnan(a) or np.isnan(b):
+        return bool(np.isnan(a) and np.isnan(b))
+    return abs(a - b) <= tol * abs(b)
+
+
+def compare_traces(tg, to, tol=1e-9, damp_tol=1e-6):
+    """Per trial: the same decisions, cost within tol.  The damping is compared
+    more loosely: its update uses the gain ratio, whose numerator F_old - F_new
+    loses relative accuracy like eps * F / (F_old - F_new) as the solve
+    converges (down to ~1e-8 F here), so 1e-6 is what two correct FP64
+    evaluations can promise."""
     assert len(tg) == len(to)
+    current = None
     for a, b in zip(tg, to):
         assert (a.itno, a.kind, a.factor_ok, a.accepted) == (b.itno, b.kind, b.factor_ok, b.accepted)
-        if b.factor_ok:
-            assert abs(a.cost - b.cost) <= tol * abs(b.cost)
-        if damp:
-            assert abs(a.damp - b.damp) <= tol * abs(b.damp)
+        # a rejected trial that lands 1000x above the current cost has pushed a
+        # point through a camera's focal plane, where the cost is unboundedly
+        # sensitive to the step: only the decision is compared there
+        wild = not a.accepted and current is not None and b.cost > 1e3 * current
+        if b.factor_ok and not wild:
+            assert close(a.cost, b.cost, tol), (a.itno, a.cost, b.cost)
+        assert close(a.damp, b.damp, damp_tol), (a.itno, a.damp, b.damp)
+        if a.accepted:
+            current = b.cost
 
 
 LM_SCENES = [(0, {}), (1, dict(n_pts=1500)), (1, dict(cnp=16, n_pts=1500)), (2, dict(n_pts=6000, n_cams=30))]
@@ -106,24 +131,28 @@ def test_levmar_pure_with_relative_stop(P, O, gpu):
     assert rel(np.concatenate(g.read_params()), np.concatenate(o.read_params())) < 1e-9
 
 
-@pytest.mark.parametrize("config,kw", [(0, {}), (1, dict(n_pts=800))], ids=["c0", "c1"])
-def test_dogleg_after_lm_with_scale_fixed(P, O, gpu, config, kw):
-    sc = fix_scale(P.Scene(config, 3000 + config, **kw))
+TR_SCENES = [(0, {}), (1, dict(n_pts=800, n_cams=20)), (2, dict(n_pts=3000, n_cams=20))]
+TR_IDS = ["c0-cnp6", "c1-cnp11", "c2-cnp9"]
+
+
+@pytest.mark.parametrize("config,kw", TR_SCENES, ids=TR_IDS)
+def test_dogleg_after_lm_with_gauge_fixed(P, O, gpu, config, kw):
+    sc = fix_gauge(P.Scene(config, 3000 + config, **kw))
     g, o = pair(P, O, sc)
     g.levmar()
     o.levmar()
     rg = g.trust_region()
     ro = o.trust_region()
     assert (rg["code"], rg["itno"]) == (ro["code"], ro["itno"])
-    compare_traces(rg["trace"], ro["trace"], tol=1e-9)
+    compare_traces(rg["trace"], ro["trace"])
     for a, b in zip(rg["trace"], ro["trace"]):
-        assert abs(a.lambda_ - b.lambda_) <= 1e-9 * max(abs(b.lambda_), 1e-300)
+        assert close(a.lambda_, b.lambda_, 1e-9) or a.lambda_ == b.lambda_
     assert rel(np.concatenate(g.read_params()), np.concatenate(o.read_params())) < 1e-9
 
 
-@pytest.mark.parametrize("config,kw", [(0, {}), (1, dict(n_pts=800))], ids=["c0", "c1"])
-def test_hybrid_with_scale_fixed(P, O, gpu, config, kw):
-    sc = fix_scale(P.Scene(config, 4000 + config, **kw))
+@pytest.mark.parametrize("config,kw", TR_SCENES, ids=TR_IDS)
+def test_hybrid_with_gauge_fixed(P, O, gpu, config, kw):
+    sc = fix_gauge(P.Scene(config, 4000 + config, **kw))
     g, o = pair(P, O, sc)
     rg = g.hybrid()
     ro = o.hybrid()
@@ -135,7 +164,7 @@ def test_hybrid_with_scale_fixed(P, O, gpu, config, kw):
 
 def test_dogleg_from_the_start(P, O, gpu):
     """trust_region() entered first (Delta = 1, lambda = 0), scale fixed."""
-    sc = fix_scale(P.Scene(0, 4100))
+    sc = fix_gauge(P.Scene(0, 4100))
     g, o = pair(P, O, sc)
     rg = g.trust_region()
     ro = o.trust_region()
