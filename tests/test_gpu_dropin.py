@@ -110,6 +110,23 @@ def test_full_size_iteration_matches_oracle(P, O, gpu, c2_full):
     assert rel(g.buffer("DP"), o.buffer("DP")) < 1e-9
 
 
+def test_full_size_solve_matches_oracle_without_outliers(P, O, gpu):
+    """The 88-camera / 64,298-point geometry without gross outliers: the whole
+    LM solve (stop on relative decrease < 1e-8) trial by trial against the
+    oracle, and the final parameters."""
+    from test_gpu_parity import compare_traces
+    sc = P.Scene(2, 2024, outlier_frac=0.0)
+    O.set_threads(16)
+    g = P.Context.from_scene(sc)
+    o = O.Oracle.from_scene(sc)
+    rg = g.levmar(max_iter=100, flags=P.FLAG_NO_HANDOFF, rel_tol=1e-8)
+    ro = o.levmar(max_iter=100, flags=O.FLAG_NO_HANDOFF, rel_tol=1e-8)
+    assert (rg["code"], rg["itno"]) == (ro["code"], ro["itno"])
+    compare_traces(rg["trace"], ro["trace"])
+    assert rel(rg["final_err"], ro["final_err"]) < 1e-9
+    assert rel(np.concatenate(g.read_params()), np.concatenate(o.read_params())) < 1e-9
+
+
 def test_full_size_solve_is_deterministic_and_monotone(P, gpu, c2_full):
     sc = c2_full
     g = P.Context.from_scene(sc)
