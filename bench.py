@@ -30,6 +30,19 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
+def profiled_traffic(kernel):
+    """HBM bytes per launch of `kernel` from its committed rocprofv3 PMC summary
+    (profiles/<kernel>_c2_traffic.md, FETCH_SIZE + WRITE_SIZE scaled by a copy
+    calibration), or None when no summary exists."""
+    path = os.path.join(ROOT, "profiles", "%s_c2_traffic.md" % kernel)
+    if not os.path.exists(path):
+        return None, None
+    for line in open(path):
+        if line.startswith("bytes_per_launch"):
+            return float(line.split("=")[1]), os.path.relpath(path, ROOT)
+    return None, None
+
+
 def cpu_baseline(scene, kw):
     """The oracle (the CPU FP64 restatement of the reference's LM) on this
     host's cores: one complete solve of the same scene with the same stopping
@@ -117,10 +130,11 @@ def main():
     flops = 6.0 * C * C * info["n_pair_items"]
     avg_s = schur_ms * 1e-3 / max(schur_n, 1)
     achieved = flops / avg_s / 1e12
+    traffic, traffic_src = profiled_traffic("k_schur_chunks")
     out["roofline"] = {"bound": "mfma", "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
-                       "frac": achieved / FP64_PEAK_TFLOPS, "traffic": None,
+                       "frac": achieved / FP64_PEAK_TFLOPS, "traffic": traffic,
                        "kernel": "k_schur_chunks", "avg_launch_us": avg_s * 1e6, "launches": schur_n,
-                       "flops_per_launch": flops}
+                       "flops_per_launch": flops, "traffic_source": traffic_src}
     log("schur: %d launches, %.1f us avg, %.2f TFLOP/s (%.1f%% of FP64 peak)" % (
         schur_n, avg_s * 1e6, achieved, 100 * achieved / FP64_PEAK_TFLOPS))
     if not a.no_cpu:
