@@ -89,6 +89,14 @@ struct psba_ctx {
   double *S = nullptr, *ea = nullptr, *xsol = nullptr, *dp = nullptr, *eb = nullptr;
   double *vecA = nullptr, *vecB = nullptr, *vecC = nullptr, *vecD = nullptr;  // n_tot scratch
   double* mc_work = nullptr;   // modified Cholesky scratch (3*n_pad)
+  // fused factor + solve (k_chol_solve): factor, right-hand side copy, y = L^-1 b,
+  // grid-barrier counter (only grows; bar_base is its value before a launch)
+  double *Lf = nullptr, *bw = nullptr, *yv = nullptr;
+  unsigned long long* bar = nullptr;
+  unsigned long long bar_base = 0;
+  int* chol_fail = nullptr;
+  int chol_grid = 0, chol_resident = 0;
+  const double* L_cur = nullptr;  // where the last factor lives (Lf, or S after the modified Cholesky)
   double* Saux = nullptr;      // copy of S for the dogleg's recovery path (allocated on first use)
   double* snap = nullptr;      // parameter snapshot (psba_snapshot_params)
   double* red_part = nullptr;  // reduction partials
@@ -133,8 +141,7 @@ int launch_axpby(psba_ctx* c, double* out, double alpha, const double* a, double
 int launch_schur(psba_ctx* c, double mu);                 // S (lower), ea
 
 // ---- launchers (k_dense.hip) ----
-int launch_factor(psba_ctx* c);                           // S -> L in place, SC_FACTOR_OK
-int launch_solve(psba_ctx* c);                            // ea -> dp[0..n)
+int launch_factor_solve(psba_ctx* c);                     // S -> Lf, SC_FACTOR_OK; dp[0..n) = S^-1 ea
 int launch_modchol(psba_ctx* c, double* S_src);           // SC_LAMBDA (factor in S)
 int launch_copy_S(psba_ctx* c, double* dst, const double* src);
 

@@ -199,6 +199,11 @@ int psba_create(const psba_desc* d, psba_ctx** out) {
   chk(dalloc(&c->vecC, (size_t)c->ntot));
   chk(dalloc(&c->vecD, (size_t)c->ntot));
   chk(dalloc(&c->mc_work, 3 * (size_t)c->n_pad));
+  chk(dalloc(&c->Lf, (size_t)c->n_pad * c->n_pad));
+  chk(dalloc(&c->bw, (size_t)c->n_pad));
+  chk(dalloc(&c->yv, (size_t)c->n_pad));
+  chk(dalloc(&c->bar, (size_t)1));
+  chk(dalloc(&c->chol_fail, (size_t)1));
   c->red_part_cap = 8 * (long)std::max(ceil_div(c->no, 256), ceil_div(c->ntot, 256)) + 64;
   chk(dalloc(&c->red_part, (size_t)c->red_part_cap));
   chk(dalloc(&c->counters, (size_t)CT_COUNT));
@@ -216,6 +221,8 @@ int psba_create(const psba_desc* d, psba_ctx** out) {
     z(hipMemsetAsync(c->dp, 0, sizeof(double) * (size_t)c->ntot, c->stream));
     z(hipMemsetAsync(c->counters, 0, sizeof(unsigned) * CT_COUNT, c->stream));
     z(hipMemsetAsync(c->d_scal, 0, sizeof(double) * SC_COUNT, c->stream));
+    z(hipMemsetAsync(c->bar, 0, sizeof(unsigned long long), c->stream));
+    z(hipMemsetAsync(c->Lf, 0, sizeof(double) * (size_t)c->n_pad * c->n_pad, c->stream));
     if (c->n_pad > c->n)
       hipLaunchKernelGGL(k_pad_identity, dim3(1), dim3(64), 0, c->stream, c->S, c->n_pad, c->n, c->n_pad);
     z(hipStreamSynchronize(c->stream));
@@ -237,11 +244,14 @@ int psba_destroy(psba_ctx* c) {
   free_index(c);
   double* bufs[] = {c->K, c->q0, c->meas, c->cams, c->pts, c->e, c->JA, c->JB, c->W, c->U, c->V, c->g,
                     c->uvdiag, c->Vinv, c->z, c->S, c->ea, c->xsol, c->dp, c->eb, c->vecA, c->vecB,
-                    c->vecC, c->vecD, c->mc_work, c->red_part, c->d_scal, c->Saux, c->snap};
+                    c->vecC, c->vecD, c->mc_work, c->red_part, c->d_scal, c->Saux, c->snap,
+                    c->Lf, c->bw, c->yv};
   for (double* p : bufs)
     if (p) (void)hipFree(p);
   dfree(c->fix);
   dfree(c->counters);
+  dfree(c->bar);
+  dfree(c->chol_fail);
   if (c->h_scal) (void)hipHostFree(c->h_scal);
   for (auto& pr : c->probe_pending) {
     if (pr.first) (void)hipEventDestroy(pr.first);
@@ -524,8 +534,7 @@ int psba_stage_schur(psba_ctx* c, double mu) {
 
 int psba_stage_factor_solve(psba_ctx* c, int* ok) {
   PSBA_TRY(ready_dense(c));
-  PSBA_TRY(launch_factor(c));
-  PSBA_TRY(launch_solve(c));
+  PSBA_TRY(launch_factor_solve(c));
   PSBA_HIP_TRY(hipMemcpyAsync(c->h_scal, c->d_scal, sizeof(double) * SC_COUNT, hipMemcpyDeviceToHost, c->stream));
   PSBA_HIP_TRY(hipStreamSynchronize(c->stream));
   if (ok) *ok = c->h_scal[SC_FACTOR_OK] != 0.0;
@@ -633,7 +642,7 @@ int psba_read_buffer(psba_ctx* c, int which, double* out, long n) {
     case PSBA_BUF_S:
     case PSBA_BUF_L: {
       std::vector<double> h((size_t)c->n_pad * c->n_pad);
-      PSBA_TRY(d2h(h.data(), c->S, h.size()));
+      PSBA_TRY(d2h(h.data(), (which == PSBA_BUF_L && c->L_cur) ? c->L_cur : c->S, h.size()));
       for (int r = 0; r < c->n; ++r)
         for (int q = 0; q < c->n; ++q) out[(size_t)r * c->n + q] = h[(size_t)r * c->n_pad + q];
       return PSBA_OK;

@@ -72,11 +72,7 @@ int trial_solve(psba_ctx* c, double mu) {
   }
   {
     StageTimer t(c, &c->timing.factor);
-    PSBA_TRY(launch_factor(c));
-  }
-  {
-    StageTimer t(c, &c->timing.solve);
-    PSBA_TRY(launch_solve(c));
+    PSBA_TRY(launch_factor_solve(c));
   }
   {
     StageTimer t(c, &c->timing.backsub);
@@ -214,7 +210,7 @@ int compute_PB(psba_ctx* c, double* lambda, int* solved) {
   PSBA_TRY(launch_copy_S(c, c->Saux, c->S));
   {
     StageTimer t(c, &c->timing.factor);
-    PSBA_TRY(launch_factor(c));
+    PSBA_TRY(launch_factor_solve(c));
   }
   PSBA_TRY(fetch(c));
   c->timing.trials++;
@@ -228,10 +224,6 @@ int compute_PB(psba_ctx* c, double* lambda, int* solved) {
     }
     *lambda = 2 * (*lambda);
     return PSBA_OK;
-  }
-  {
-    StageTimer t(c, &c->timing.solve);
-    PSBA_TRY(launch_solve(c));
   }
   {
     StageTimer t(c, &c->timing.backsub);

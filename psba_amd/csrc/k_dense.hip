@@ -1,15 +1,15 @@
 // Dense FP64 linear algebra on the reduced camera matrix S (n_pad x n_pad,
 // row-major, lower triangle; rows/cols >= n are the identity):
-//   - blocked right-looking Cholesky S = L L^T in place (32x32 tiles):
-//     diagonal-tile factor, panel triangular solve, trailing update;
-//     replaces the reference's explicit inverse (PSBA/CL_files/SPD_inv.cl,
-//     kern_cholesky/_s2 + kern_trigMat_inv/_mul + kern_fill_rest);
-//   - forward/back substitution for S dpa = e_a (replaces matVec_mul.cl);
+//   - Cholesky S = L L^T and the solve S dpa = e_a fused in one cooperative
+//     launch over 64x64 tiles (k_chol_solve below); replaces the reference's
+//     explicit inverse (PSBA/CL_files/SPD_inv.cl, kern_cholesky/_s2 +
+//     kern_trigMat_inv/_mul + kern_fill_rest) and matVec_mul.cl;
 //   - the dogleg's modified Cholesky and lambda = |sum E| / n
 //     (PSBA/CL_files/cholmod_blk.cl, PSBA/cl_cholmod.cpp:109-202).
 // Failure of the factorisation (a pivot that is not finite and positive) is
-// recorded in d_scal[SC_FACTOR_OK]; later kernels of the same factorisation
-// then do nothing.
+// recorded in d_scal[SC_FACTOR_OK].
+#include <algorithm>
+
 #include "context.hpp"
 
 namespace psba {
@@ -18,13 +18,7 @@ constexpr int NB = kNB;   // 64
 constexpr int TP = NB + 1; // padded LDS row length
 
 // ---------------------------------------------------------------------------
-// blocked right-looking Cholesky, 64x64 tiles.  Step kb:
-//   k_potrf64   factor the diagonal tile (one wave, rows in registers)
-//   k_trsm64    L_ib,kb = A_ib,kb L_kk^-T for ib > kb (one wave per tile) and,
-//               in one extra block, the forward substitution y_kb = L_kk^-1 b_kb
-//   k_syrk64    A_ib,jb -= L_ib,kb L_jb,kb^T for kb < jb <= ib (tile GEMM) and,
-//               in extra blocks, b_jb -= L_jb,kb y_kb
-// so that after the last step b holds y = L^-1 b.
+// Tile building blocks (4 waves, lane = row, wave W owns columns 16W..16W+15).
 // ---------------------------------------------------------------------------
 // Diagonal tile: 4 waves, lane = row, wave W owns columns 16W..16W+15 of all
 // 64 rows (registers); the pivot column goes through double-buffered LDS with
@@ -59,46 +53,6 @@ __device__ __forceinline__ void potrf_wave(double (&a)[16], double (*col)[NB], i
   }
 }
 
-// A failed pivot leaves the tile untouched and clears *ok.
-__global__ __launch_bounds__(256) void k_potrf64(double* __restrict__ S, int ld, int kb, double* ok) {
-  __shared__ double T[NB * TP];
-  __shared__ double col[2][NB];
-  __shared__ int bad;
-  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-  if (kb == 0) {
-    if (threadIdx.x == 0) *ok = 1.0;
-  } else if (*ok == 0.0) {
-    return;
-  }
-  const int base = kb * NB;
-  for (int f = threadIdx.x; f < NB * NB; f += blockDim.x) {
-    int r = f >> 6, c = f & 63;
-    T[r * TP + c] = (c <= r) ? S[(size_t)(base + r) * ld + base + c] : 0.0;
-  }
-  if (threadIdx.x == 0) bad = 0;
-  __syncthreads();
-  double a[16];
-#pragma unroll
-  for (int q = 0; q < 16; ++q) a[q] = T[lane * TP + 16 * w + q];
-  switch (w) {
-    case 0: potrf_wave<0>(a, col, lane, &bad); break;
-    case 1: potrf_wave<1>(a, col, lane, &bad); break;
-    case 2: potrf_wave<2>(a, col, lane, &bad); break;
-    default: potrf_wave<3>(a, col, lane, &bad); break;
-  }
-#pragma unroll
-  for (int q = 0; q < 16; ++q) T[lane * TP + 16 * w + q] = a[q];
-  __syncthreads();
-  if (bad) {
-    if (threadIdx.x == 0) *ok = 0.0;
-    return;
-  }
-  for (int f = threadIdx.x; f < NB * NB; f += blockDim.x) {
-    int r = f >> 6, c = f & 63;
-    if (c <= r) S[(size_t)(base + r) * ld + base + c] = T[r * TP + c];
-  }
-}
-
 template <int W>
 __device__ __forceinline__ void trsm_wave(double (&x)[16], double (*col)[NB], const double* L, const double* rd,
                                           int lane) {
@@ -115,201 +69,6 @@ __device__ __forceinline__ void trsm_wave(double (&x)[16], double (*col)[NB], co
     for (int q = 0; q < 16; ++q)
       if (16 * W + q > c) x[q] -= xc * L[(16 * W + q) * TP + c];
   }
-}
-
-// Panel: L_ib,kb = A_ib,kb L_kk^-T, one block (4 waves) per tile row ib > kb;
-// lane = row, wave w owns columns 16w..16w+15.  The last block instead does
-// the forward substitution y_kb = L_kk^-1 b_kb (wave 0).
-__global__ __launch_bounds__(256) void k_trsm64(double* __restrict__ S, int ld, int kb, int T,
-                                               double* __restrict__ b, const double* ok) {
-  __shared__ double L[NB * TP];
-  __shared__ double rd[NB];
-  __shared__ double col[2][NB];
-  if (*ok == 0.0) return;
-  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, kbase = kb * NB;
-  for (int f = threadIdx.x; f < NB * NB; f += blockDim.x) {
-    int r = f >> 6, c = f & 63;
-    L[r * TP + c] = (c <= r) ? S[(size_t)(kbase + r) * ld + kbase + c] : 0.0;
-  }
-  __syncthreads();
-  if ((int)blockIdx.x == T) {
-    if (w != 0) return;
-    double Lrow[NB];
-#pragma unroll
-    for (int m = 0; m < NB; ++m) Lrow[m] = L[lane * TP + m];
-    double br = b[kbase + lane], yv = 0.0;
-#pragma unroll
-    for (int m = 0; m < NB; ++m) {
-      const double ym = __shfl(br / Lrow[m], m, 64);
-      if (lane == m) yv = ym;
-      if (lane > m) br -= Lrow[m] * ym;
-    }
-    b[kbase + lane] = yv;
-    return;
-  }
-  if (threadIdx.x < NB) rd[threadIdx.x] = 1.0 / L[threadIdx.x * TP + threadIdx.x];
-  const size_t ibase = (size_t)(kb + 1 + blockIdx.x) * NB;
-  double x[16];
-#pragma unroll
-  for (int q = 0; q < 16; ++q) x[q] = S[(ibase + lane) * ld + kbase + 16 * w + q];
-  __syncthreads();
-  switch (w) {
-    case 0: trsm_wave<0>(x, col, L, rd, lane); break;
-    case 1: trsm_wave<1>(x, col, L, rd, lane); break;
-    case 2: trsm_wave<2>(x, col, L, rd, lane); break;
-    default: trsm_wave<3>(x, col, L, rd, lane); break;
-  }
-#pragma unroll
-  for (int q = 0; q < 16; ++q) S[(ibase + lane) * ld + kbase + 16 * w + q] = x[q];
-}
-
-// blocks [0, ntiles): trailing tile (ib, jb); blocks [ntiles, ntiles+T): b update
-__global__ __launch_bounds__(256) void k_syrk64(double* __restrict__ S, int ld, int kb, int T, int ntiles,
-                                               double* __restrict__ b, const double* ok) {
-  constexpr int AP = NB + 4;  // transposed tiles: [m][r], padded
-  __shared__ double As[NB * AP];
-  __shared__ double Bs[NB * AP];
-  if (*ok == 0.0) return;
-  const int kbase = kb * NB;
-  const int t = blockIdx.x;
-  if (t >= ntiles) {
-    // b_jb -= L_jb,kb y_kb : 4 threads per row
-    const int jbase = (kb + 1 + (t - ntiles)) * NB;
-    const int r = threadIdx.x >> 2, part = threadIdx.x & 3;
-    const double* Lr = S + (size_t)(jbase + r) * ld + kbase + part * 16;
-    const double* y = b + kbase + part * 16;
-    double s = 0.0;
-#pragma unroll
-    for (int c = 0; c < 16; ++c) s += Lr[c] * y[c];
-    s += __shfl_xor(s, 1, 64);
-    s += __shfl_xor(s, 2, 64);
-    if (part == 0) b[jbase + r] -= s;
-    return;
-  }
-  int a = (int)((sqrt(8.0 * t + 1.0) - 1.0) * 0.5);
-  while ((a + 1) * (a + 2) / 2 <= t) ++a;
-  while (a * (a + 1) / 2 > t) --a;
-  const int bb = t - a * (a + 1) / 2;
-  const int ib = kb + 1 + a, jb = kb + 1 + bb;
-  const int ibase = ib * NB, jbase = jb * NB;
-  for (int f = threadIdx.x; f < NB * NB; f += blockDim.x) {
-    int r = f >> 6, m = f & 63;
-    As[m * AP + r] = S[(size_t)(ibase + r) * ld + kbase + m];
-    Bs[m * AP + r] = S[(size_t)(jbase + r) * ld + kbase + m];
-  }
-  __syncthreads();
-  const int r0 = 4 * (threadIdx.x >> 4), c0 = 4 * (threadIdx.x & 15);
-  double acc[4][4];
-#pragma unroll
-  for (int u = 0; u < 4; ++u)
-#pragma unroll
-    for (int v = 0; v < 4; ++v) acc[u][v] = 0.0;
-#pragma unroll 8
-  for (int m = 0; m < NB; ++m) {
-    const double* ar = As + m * AP + r0;
-    const double* bc = Bs + m * AP + c0;
-    double a4[4] = {ar[0], ar[1], ar[2], ar[3]};
-    double b4[4] = {bc[0], bc[1], bc[2], bc[3]};
-#pragma unroll
-    for (int u = 0; u < 4; ++u)
-#pragma unroll
-      for (int v = 0; v < 4; ++v) acc[u][v] += a4[u] * b4[v];
-  }
-#pragma unroll
-  for (int u = 0; u < 4; ++u) {
-    double* row = S + (size_t)(ibase + r0 + u) * ld + jbase + c0;
-#pragma unroll
-    for (int v = 0; v < 4; ++v)
-      if (ib != jb || c0 + v <= r0 + u) row[v] -= acc[u][v];
-  }
-}
-
-int launch_factor(psba_ctx* c) {
-  const int nbk = c->n_pad / NB;
-  double* ok = c->d_scal + SC_FACTOR_OK;
-  probe_begin(c, PSBA_PROBE_FACTOR);
-  for (int kb = 0; kb < nbk; ++kb) {
-    hipLaunchKernelGGL(k_potrf64, dim3(1), dim3(256), 0, c->stream, c->S, c->n_pad, kb, ok);
-    const int T = nbk - kb - 1;
-    hipLaunchKernelGGL(k_trsm64, dim3(T + 1), dim3(256), 0, c->stream, c->S, c->n_pad, kb, T, c->ea, ok);
-    if (T > 0) {
-      const int ntiles = T * (T + 1) / 2;
-      hipLaunchKernelGGL(k_syrk64, dim3(ntiles + T), dim3(256), 0, c->stream, c->S, c->n_pad, kb, T, ntiles, c->ea,
-                         ok);
-    }
-  }
-  probe_end(c, PSBA_PROBE_FACTOR);
-  PSBA_LAUNCH_TRY();
-  return PSBA_OK;
-}
-
-// Back substitution L^T x = y (y = forward-substituted right-hand side).
-// One block of 1024 threads (16 waves); x lives in LDS.  The rows below each
-// diagonal tile are streamed by all waves (4 independent row loads in flight
-// per wave); the 64-step triangular solve of the tile is done by wave 0 with
-// the tile staged in LDS.  Also writes dp[0..n).
-__global__ __launch_bounds__(1024) void k_bsolve64(const double* __restrict__ L, int ld, int nbk,
-                                                   const double* __restrict__ y, double* __restrict__ xout,
-                                                   double* __restrict__ dp, int n, const double* ok) {
-  constexpr int NW = 16;
-  extern __shared__ double xs[];
-  __shared__ double part[NW][NB];
-  __shared__ double Lt[NB * TP];
-  if (*ok == 0.0) return;
-  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-  const int np_ = nbk * NB;
-  for (int q = threadIdx.x; q < np_; q += blockDim.x) xs[q] = y[q];
-  __syncthreads();
-  for (int kb = nbk - 1; kb >= 0; --kb) {
-    const int base = kb * NB;
-    for (int f = threadIdx.x; f < NB * NB; f += blockDim.x) {
-      int r = f >> 6, c = f & 63;
-      Lt[r * TP + c] = L[(size_t)(base + r) * ld + base + c];
-    }
-    double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
-    int r = base + NB + wid;
-    for (; r + 3 * NW < np_; r += 4 * NW) {
-      s0 += L[(size_t)r * ld + base + lane] * xs[r];
-      s1 += L[(size_t)(r + NW) * ld + base + lane] * xs[r + NW];
-      s2 += L[(size_t)(r + 2 * NW) * ld + base + lane] * xs[r + 2 * NW];
-      s3 += L[(size_t)(r + 3 * NW) * ld + base + lane] * xs[r + 3 * NW];
-    }
-    for (; r < np_; r += NW) s0 += L[(size_t)r * ld + base + lane] * xs[r];
-    part[wid][lane] = (s0 + s1) + (s2 + s3);
-    __syncthreads();
-    if (wid == 0) {
-      double acc = 0.0;
-#pragma unroll
-      for (int w = 0; w < NW; ++w) acc += part[w][lane];
-      double br = xs[base + lane] - acc;
-      double xv = 0.0;
-#pragma unroll 8
-      for (int m = NB - 1; m >= 0; --m) {
-        const double lm = Lt[m * TP + lane];  // L[base+m][base+lane]
-        const double xm = __shfl(br / lm, m, 64);
-        if (lane == m) xv = xm;
-        if (lane < m) br -= lm * xm;
-      }
-      xs[base + lane] = xv;
-    }
-    __syncthreads();
-  }
-  for (int q = threadIdx.x; q < np_; q += blockDim.x) {
-    xout[q] = xs[q];
-    if (q < n) dp[q] = xs[q];
-  }
-}
-
-int launch_solve(psba_ctx* c) {
-  const int nbk = c->n_pad / NB;
-  const double* ok = c->d_scal + SC_FACTOR_OK;
-  if (c->n_pad > kMaxSolveN) return PSBA_ERR_UNSUPPORTED;
-  probe_begin(c, PSBA_PROBE_BSOLVE);
-  hipLaunchKernelGGL(k_bsolve64, dim3(1), dim3(1024), sizeof(double) * c->n_pad, c->stream, c->S, c->n_pad, nbk,
-                     c->ea, c->xsol, c->dp, c->n, ok);
-  probe_end(c, PSBA_PROBE_BSOLVE);
-  PSBA_LAUNCH_TRY();
-  return PSBA_OK;
 }
 
 __global__ void k_copy_mat(double* __restrict__ dst, const double* __restrict__ src, long count) {
@@ -512,6 +271,560 @@ __global__ __launch_bounds__(256) void k_modchol(const double* __restrict__ A, d
 int launch_modchol(psba_ctx* c, double* A) {
   hipLaunchKernelGGL(k_modchol, dim3(1), dim3(256), 0, c->stream, A, c->S, c->n_pad, c->n, c->mc_work,
                      c->d_scal + SC_LAMBDA);
+  PSBA_LAUNCH_TRY();
+  c->L_cur = c->S;
+  return PSBA_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Fused Cholesky factor and solve of S dpa = e_a in ONE launch.
+//
+// S is viewed as T x T tiles of 64 (T = n_pad / 64).  Phase k (k = 0..T-2)
+// applies panel k to the trailing matrix: for every tile (i, j) with
+// k+1 <= j <= i, the owning workgroup recomputes the panel rows it needs,
+//     L_ik = A_ik L_kk^-T,   L_jk = A_jk L_kk^-T        (row-parallel solves)
+//     A_ij -= L_ik L_jk^T                               (FP64 MFMA 16x16x4)
+// the task (i, k+1) stores L_ik and updates the right-hand side
+//     b_i -= L_ik y_k,
+// and the task (k+1, k+1) factors the new diagonal tile at once and forms
+// y_{k+1} = L_{k+1,k+1}^-1 b_{k+1}.  Recomputing L_ik per task trades a few
+// redundant triangular solves (cheap) for a single grid barrier per phase.
+// The back substitution L^T x = y runs in T more phases: workgroup 0 solves
+// block k (after applying x_{k+1} to it), the others apply x_{k+1} to the
+// blocks above.
+//
+// The factor goes to Lf (S is overwritten by the trailing updates); e_a is
+// left intact.  Barriers: a 64-bit arrival counter that only grows (the host
+// passes the base), each wait bounded; a timed-out wait or a non-positive
+// pivot clears d_scal[SC_FACTOR_OK].  The grid is at most one workgroup per
+// CU, so on the context's in-order stream every workgroup is resident at once;
+// should another stream's kernels hold CUs, the waiting workgroups are
+// scheduled as those finish (the bounded waits cover the worst case).
+// ---------------------------------------------------------------------------
+typedef double dvec4 __attribute__((ext_vector_type(4)));
+
+// The large unrolled tile routines below are kept out of line: inlined
+// together into the kernel they exceed the register file and spill.
+#define PSBA_OUTLINE __device__ __attribute__((noinline))
+
+struct CholSmem {
+  double P[NB * TP];  // L_kk of the current phase (lower, zeros above)
+  double X[NB * TP];  // L_ik
+  double Y[NB * TP];  // L_jk, or the diagonal tile being factored
+  double rd[NB];      // 1 / diag(L_kk)
+  double v[NB];       // y_k of the phase / solution block
+  double w[NB];       // y_{k+1} / x_{k+1}
+  double rd16[16];
+  double part[4][NB];
+  int bad;
+};
+
+// value of v on lane l (l a compile-time constant in the unrolled loops)
+__device__ __forceinline__ double bcast(double v, int l) {
+  const long long b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_readlane((int)b, l);
+  const int hi = __builtin_amdgcn_readlane((int)(b >> 32), l);
+  return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+
+// partner values inside a quad of lanes (DPP quad_perm, no LDS round trip)
+template <int CTRL>
+__device__ __forceinline__ double quad_perm(double v) {
+  const long long b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_update_dpp(0, (int)b, CTRL, 0xF, 0xF, false);
+  const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), CTRL, 0xF, 0xF, false);
+  return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+__device__ __forceinline__ double quad_xor1(double v) { return quad_perm<0xB1>(v); }  // [1,0,3,2]
+__device__ __forceinline__ double quad_xor2(double v) { return quad_perm<0x4E>(v); }  // [2,3,0,1]
+
+__device__ __forceinline__ void grid_barrier(unsigned long long* bar, unsigned long long target, int* fail) {
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+    __hip_atomic_fetch_add(bar, 1ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    unsigned spins = 0;
+    while (__hip_atomic_load(bar, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target) {
+      __builtin_amdgcn_s_sleep(2);
+      if (++spins > (1u << 24)) {  // ~1 s: never reached when every workgroup is resident
+        __hip_atomic_fetch_or(fail, 2, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        break;
+      }
+    }
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+  }
+  __syncthreads();
+}
+
+// lower triangle of a global tile into LDS (zeros above the diagonal)
+__device__ __forceinline__ void load_lower(double* dst, const double* src, int ld) {
+  for (int f = threadIdx.x; f < NB * NB; f += blockDim.x) {
+    const int r = f >> 6, c = f & 63;
+    dst[r * TP + c] = (c <= r) ? src[(size_t)r * ld + c] : 0.0;
+  }
+}
+
+__device__ __forceinline__ void store_tile(const double* src, double* dst, int ld) {
+  for (int f = threadIdx.x; f < NB * NB; f += blockDim.x) {
+    const int r = f >> 6, c = f & 63;
+    dst[(size_t)r * ld + c] = src[r * TP + c];
+  }
+}
+
+// X = A L^-T (A a global 64x64 tile, L = sm.P): rows are independent; wave w
+// owns rows 16w..16w+15, the four lanes of a quad share a row and hold its
+// columns c = 4q + g (g = lane % 4).  Per column: partial sums over the solved
+// columns, a quad butterfly (the same total on all four lanes), and the owning
+// lane divides.  Branch-free; no workgroup barriers.
+PSBA_OUTLINE void trsm_rows(double* out, const double* A, int ld, const double* L, const double* rd) {
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int r = 16 * w + (lane >> 2), g = lane & 3;
+  double x[16], rdv[16], Lc[16], Ln[16];
+#pragma unroll
+  for (int q = 0; q < 16; ++q) {
+    x[q] = A[(size_t)r * ld + 4 * q + g];
+    rdv[q] = rd[4 * q + g];
+    Lc[q] = 0.0;
+    Ln[q] = 0.0;
+  }
+#pragma unroll
+  for (int c = 0; c < NB; ++c) {
+    const int qf = c >> 2, cg = c & 3;
+    // next row of L, loaded while this column is reduced
+    if (c + 1 < NB) {
+#pragma unroll
+      for (int q = 0; q <= ((c + 1) >> 2); ++q) Ln[q] = L[(c + 1) * TP + 4 * q + g];
+    }
+    double s = 0.0;
+#pragma unroll
+    for (int q = 0; q < qf; ++q) s = fma(x[q], Lc[q], s);
+    if (cg != 0) {
+      const double t = fma(x[qf], Lc[qf], s);
+      s = (g < cg) ? t : s;
+    }
+    s += quad_xor1(s);
+    s += quad_xor2(s);
+    const double nx = (x[qf] - s) * rdv[qf];
+    x[qf] = (g == cg) ? nx : x[qf];
+#pragma unroll
+    for (int q = 0; q < 16; ++q) Lc[q] = Ln[q];
+  }
+#pragma unroll
+  for (int q = 0; q < 16; ++q) out[r * TP + 4 * q + g] = x[q];
+}
+
+// value of v on lane N of each 16-lane row (DPP row_newbcast)
+template <int N>
+__device__ __forceinline__ double row_bcast(double v) {
+  const long long b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_update_dpp(0, (int)b, 0x150 + N, 0xF, 0xF, false);
+  const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), 0x150 + N, 0xF, 0xF, false);
+  return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+
+// 1/x for a normal, finite x: hardware reciprocal estimate refined by two
+// Newton steps (relative error ~1 ulp); keeps the division's long dependent
+// sequence off the pivot chain
+__device__ __forceinline__ double fast_rcp(double x) {
+  double r = __builtin_amdgcn_rcp(x);
+  r = fma(r, fma(-x, r, 1.0), r);
+  r = fma(r, fma(-x, r, 1.0), r);
+  return r;
+}
+
+template <int C, int Q>
+struct Potrf16Row {
+  static __device__ __forceinline__ void update(double (&a)[16], double ur) {
+    a[Q] = fma(-ur, row_bcast<Q>(a[C]), a[Q]);
+    Potrf16Row<C, Q + 1>::update(a, ur);
+  }
+};
+template <int C>
+struct Potrf16Row<C, 16> {
+  static __device__ __forceinline__ void update(double (&)[16], double) {}
+};
+
+template <int C>
+struct Potrf16Step {
+  static __device__ __forceinline__ void run(double (&a)[16], int lane, int* bad) {
+    const double piv = row_bcast<C>(a[C]);
+    if (lane == 0 && !(piv > 0.0)) *bad = 1;
+    const double ur = a[C] * fast_rcp(piv);
+    Potrf16Row<C, C + 1>::update(a, ur);
+    Potrf16Step<C + 1>::run(a, lane, bad);
+  }
+};
+template <>
+struct Potrf16Step<16> {
+  static __device__ __forceinline__ void run(double (&)[16], int, int*) {}
+};
+
+template <int Q>
+struct Potrf16Scale {
+  static __device__ __forceinline__ void run(const double (&a)[16], double* B, int r, int lane, double d, double rdl) {
+    const double rq = row_bcast<Q>(rdl);
+    const double v = (Q < r) ? a[Q] * rq : ((Q == r) ? d : 0.0);
+    if (lane < 16) B[r * TP + Q] = v;
+    Potrf16Scale<Q + 1>::run(a, B, r, lane, d, rdl);
+  }
+};
+template <>
+struct Potrf16Scale<16> {
+  static __device__ __forceinline__ void run(const double (&)[16], double*, int, int, double, double) {}
+};
+
+// 16x16 Cholesky of B (LDS, stride TP) by one wave, lane = row (the four
+// 16-lane rows of the wave carry identical copies).  Outer-product LDL^T with
+// the pivot column broadcast by DPP row_newbcast (one reciprocal per pivot on
+// the dependent chain), then L = U sqrt(D) with one sqrt per lane.
+// rd16 = 1 / diag(L).
+__device__ __forceinline__ void potrf16(double* B, double* rd16, int* bad) {
+  const int lane = threadIdx.x & 63;
+  const int r = lane & 15;
+  double a[16];
+#pragma unroll
+  for (int q = 0; q < 16; ++q) a[q] = B[r * TP + q];
+  Potrf16Step<0>::run(a, lane, bad);
+  double dd = a[0];
+#pragma unroll
+  for (int q = 1; q < 16; ++q) dd = (r == q) ? a[q] : dd;
+  const double d = sqrt(dd), rdl = 1.0 / d;
+  Potrf16Scale<0>::run(a, B, r, lane, d, rdl);
+  if (lane < 16) rd16[r] = rdl;
+}
+
+// Cholesky of the 64x64 tile in sm.Y (lower part), in place, zeros above.
+// Right-looking over 16-column blocks: the diagonal block (one wave), the
+// panel below (one lane per row), the trailing lower blocks (MFMA, K = 16).
+PSBA_OUTLINE int tile_potrf(CholSmem& sm) {
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int li = lane & 15, lk = lane >> 4;
+  if (threadIdx.x == 0) sm.bad = 0;
+  __syncthreads();
+#pragma unroll 1
+  for (int kb = 0; kb < 4; ++kb) {
+    const int b0 = 16 * kb;
+    if (w == 0) potrf16(sm.Y + b0 * TP + b0, sm.rd16, &sm.bad);
+    __syncthreads();
+    const int R = 48 - b0;
+    if ((int)threadIdx.x < R) {
+      const int row = b0 + 16 + threadIdx.x;
+      double x[16], Lc[16], Ln[16], rdv[16];
+#pragma unroll
+      for (int c = 0; c < 16; ++c) {
+        x[c] = sm.Y[row * TP + b0 + c];
+        rdv[c] = sm.rd16[c];
+        Lc[c] = 0.0;
+        Ln[c] = 0.0;
+      }
+#pragma unroll
+      for (int c = 0; c < 16; ++c) {
+        if (c + 1 < 16) {
+#pragma unroll
+          for (int m = 0; m <= c; ++m) Ln[m] = sm.Y[(b0 + c + 1) * TP + b0 + m];
+        }
+        double s0 = 0.0, s1 = 0.0;
+#pragma unroll
+        for (int m = 0; m + 1 < c; m += 2) {
+          s0 = fma(x[m], Lc[m], s0);
+          s1 = fma(x[m + 1], Lc[m + 1], s1);
+        }
+        if (c & 1) s0 = fma(x[c - 1], Lc[c - 1], s0);
+        x[c] = (x[c] - (s0 + s1)) * rdv[c];
+#pragma unroll
+        for (int m = 0; m < 16; ++m) Lc[m] = Ln[m];
+      }
+#pragma unroll
+      for (int c = 0; c < 16; ++c) sm.Y[row * TP + b0 + c] = x[c];
+    }
+    __syncthreads();
+    const int m = 3 - kb, nblk = m * (m + 1) / 2;
+    for (int bi = w; bi < nblk; bi += 4) {
+      int jb = 0, cum = 0;
+      while (cum + (m - jb) <= bi) { cum += m - jb; ++jb; }
+      const int ib = kb + 1 + jb + (bi - cum), jc = kb + 1 + jb;
+      dvec4 acc;
+#pragma unroll
+      for (int rr = 0; rr < 4; ++rr) acc[rr] = sm.Y[(16 * ib + 4 * rr + lk) * TP + 16 * jc + li];
+#pragma unroll
+      for (int k0 = 0; k0 < 16; k0 += 4) {
+        const double av = -sm.Y[(16 * ib + li) * TP + b0 + k0 + lk];
+        const double bv = sm.Y[(16 * jc + li) * TP + b0 + k0 + lk];
+        acc = __builtin_amdgcn_mfma_f64_16x16x4f64(av, bv, acc, 0, 0, 0);
+      }
+#pragma unroll
+      for (int rr = 0; rr < 4; ++rr) sm.Y[(16 * ib + 4 * rr + lk) * TP + 16 * jc + li] = acc[rr];
+    }
+    __syncthreads();
+  }
+  for (int f = threadIdx.x; f < NB * NB; f += blockDim.x) {
+    const int rr = f >> 6, c = f & 63;
+    if (c > rr) sm.Y[rr * TP + c] = 0.0;
+  }
+  __syncthreads();
+  return sm.bad;
+}
+
+// acc (MFMA layout: reg r of lane l is row 4r + l/16, column l%16 of its 16x16
+// block; wave w owns the 32x32 quadrant rows 32*(w>>1), cols 32*(w&1)) <- C
+__device__ __forceinline__ void acc_load(dvec4 (&acc)[2][2], const double* C, int ld) {
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int R0 = 32 * (w >> 1), C0 = 32 * (w & 1);
+  const int li = lane & 15, lk = lane >> 4;
+#pragma unroll
+  for (int a = 0; a < 2; ++a)
+#pragma unroll
+    for (int b = 0; b < 2; ++b)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) acc[a][b][r] = C[(size_t)(R0 + 16 * a + 4 * r + lk) * ld + C0 + 16 * b + li];
+}
+
+__device__ __forceinline__ void acc_store(const dvec4 (&acc)[2][2], double* dst, int ld) {
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int R0 = 32 * (w >> 1), C0 = 32 * (w & 1);
+  const int li = lane & 15, lk = lane >> 4;
+#pragma unroll
+  for (int a = 0; a < 2; ++a)
+#pragma unroll
+    for (int b = 0; b < 2; ++b)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) dst[(size_t)(R0 + 16 * a + 4 * r + lk) * ld + C0 + 16 * b + li] = acc[a][b][r];
+}
+
+// acc -= X Y^T (LDS tiles, K = 64); `lower_only` leaves the upper quadrant
+__device__ __forceinline__ void acc_update(dvec4 (&acc)[2][2], const double* X, const double* Y, bool lower_only) {
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int R0 = 32 * (w >> 1), C0 = 32 * (w & 1);
+  const int li = lane & 15, lk = lane >> 4;
+  if (lower_only && C0 > R0) return;
+#pragma unroll 4
+  for (int k0 = 0; k0 < NB; k0 += 4) {
+    const double a0 = -X[(R0 + li) * TP + k0 + lk];
+    const double a1 = -X[(R0 + 16 + li) * TP + k0 + lk];
+    const double b0 = Y[(C0 + li) * TP + k0 + lk];
+    const double b1 = Y[(C0 + 16 + li) * TP + k0 + lk];
+    acc[0][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b0, acc[0][0], 0, 0, 0);
+    acc[0][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b1, acc[0][1], 0, 0, 0);
+    acc[1][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b0, acc[1][0], 0, 0, 0);
+    acc[1][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b1, acc[1][1], 0, 0, 0);
+  }
+}
+
+// y = L^-1 b (L = lower tile in LDS), wave 0, lane = row; the lane's row of
+// L is loaded first, the solved value is broadcast by readlane, and every
+// lane updates unconditionally (a lane past its own row no longer matters,
+// entries above the diagonal are zero).  y -> yg (global) and ys (LDS).
+PSBA_OUTLINE void tile_forward(const double* L, const double* b, double* yg, double* ys) {
+  if (threadIdx.x >= 64) return;
+  const int lane = threadIdx.x;
+  double Lr[NB];
+#pragma unroll
+  for (int m = 0; m < NB; ++m) Lr[m] = L[lane * TP + m];
+  double br = b[lane], yv = 0.0;
+  const double rdl = 1.0 / L[lane * TP + lane];
+#pragma unroll
+  for (int m = 0; m < NB; ++m) {
+    const double ym = bcast(br * rdl, m);
+    yv = (lane == m) ? ym : yv;
+    br = fma(-Lr[m], ym, br);
+  }
+  yg[lane] = yv;
+  ys[lane] = yv;
+}
+
+// x = L^-T r (L lower tile in LDS), wave 0, same scheme with the lane's column
+PSBA_OUTLINE void tile_backward(const double* L, double br, double* xg, double* xsm) {
+  if (threadIdx.x >= 64) return;
+  const int lane = threadIdx.x;
+  double Lc[NB];
+#pragma unroll
+  for (int m = 0; m < NB; ++m) Lc[m] = L[m * TP + lane];
+  double xv = 0.0;
+  const double rdl = 1.0 / L[lane * TP + lane];
+#pragma unroll
+  for (int m = NB - 1; m >= 0; --m) {
+    const double xm = bcast(br * rdl, m);
+    xv = (lane == m) ? xm : xv;
+    br = fma(-Lc[m], xm, br);
+  }
+  xg[lane] = xv;
+  xsm[lane] = xv;
+}
+
+// part[w][c] = sum over rows rr = w, w+4, .. of Lt[rr][c] * x[rr]; returns the
+// fixed-order total for lane c (valid on all waves after the barrier inside)
+__device__ __forceinline__ double tile_tmatvec(CholSmem& sm, const double* Lt, int ld, const double* x) {
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  double s = 0.0;
+#pragma unroll
+  for (int rr = w; rr < NB; rr += 4) s += Lt[(size_t)rr * ld + lane] * x[rr];
+  sm.part[w][lane] = s;
+  __syncthreads();
+  const double t = ((sm.part[0][lane] + sm.part[1][lane]) + sm.part[2][lane]) + sm.part[3][lane];
+  __syncthreads();
+  return t;
+}
+
+__global__ __launch_bounds__(256) void k_chol_solve(double* __restrict__ A, int ld, int T, double* __restrict__ Lf,
+                                                    const double* __restrict__ ea, double* __restrict__ bw,
+                                                    double* __restrict__ yv, double* __restrict__ xs,
+                                                    double* __restrict__ dp, int n, double* __restrict__ ok,
+                                                    unsigned long long* bar, unsigned long long bar_base,
+                                                    int* __restrict__ fail) {
+  __shared__ CholSmem sm;
+  const int G = gridDim.x;
+  const int tid = threadIdx.x;
+  unsigned long long nbar = 0;
+  auto tile = [&](double* base, int i, int j) { return base + (size_t)i * NB * ld + (size_t)j * NB; };
+  // workgroup 0 carries the diagonal chain: it keeps L_kk and y_k in LDS when
+  // the diagonal task was its only task of the phase
+  bool carry = false;
+
+  // ---- phase -1: factor tile (0,0) and y_0; copy the right-hand side
+  if (blockIdx.x == 0) {
+    for (int f = tid; f < NB * NB; f += blockDim.x) {
+      const int r = f >> 6, c = f & 63;
+      sm.Y[r * TP + c] = A[(size_t)r * ld + c];
+    }
+    if (tid < NB) sm.part[0][tid] = ea[tid];
+    __syncthreads();
+    const int bad = tile_potrf(sm);
+    if (bad && tid == 0) __hip_atomic_fetch_or(fail, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    store_tile(sm.Y, Lf, ld);
+    tile_forward(sm.Y, sm.part[0], yv, sm.w);
+    carry = true;
+  }
+  for (int q = NB + (int)blockIdx.x * (int)blockDim.x + tid; q < T * NB; q += G * (int)blockDim.x) bw[q] = ea[q];
+  grid_barrier(bar, bar_base + (unsigned long long)G * (++nbar), fail);
+
+  // ---- phases 0 .. T-2: factor and forward substitution
+  for (int k = 0; k + 1 < T; ++k) {
+    const int m = T - 1 - k, ntasks = m * (m + 1) / 2;
+    if ((int)blockIdx.x < ntasks) {
+      if (carry) {
+        for (int f = tid; f < NB * NB; f += blockDim.x) {
+          const int r = f >> 6, c = f & 63;
+          sm.P[r * TP + c] = sm.Y[r * TP + c];
+        }
+        if (tid < NB) sm.v[tid] = sm.w[tid];
+      } else {
+        load_lower(sm.P, tile(Lf, k, k), ld);
+        if (tid < NB) sm.v[tid] = yv[(size_t)k * NB + tid];
+      }
+      __syncthreads();
+      if (tid < NB) sm.rd[tid] = 1.0 / sm.P[tid * TP + tid];
+      __syncthreads();
+    }
+    carry = false;
+    for (int t = blockIdx.x; t < ntasks; t += G) {
+      // task t -> tile (i, j): columns j = k+1.. in order, rows i = j.. within a column
+      int a = 0, cum = 0;
+      while (cum + (m - a) <= t) { cum += m - a; ++a; }
+      const int j = k + 1 + a, i = j + (t - cum);
+      dvec4 acc[2][2];
+      acc_load(acc, tile(A, i, j), ld);
+      trsm_rows(sm.X, tile(A, i, k), ld, sm.P, sm.rd);
+      if (i != j) trsm_rows(sm.Y, tile(A, j, k), ld, sm.P, sm.rd);
+      __syncthreads();
+      acc_update(acc, sm.X, (i != j) ? sm.Y : sm.X, i == j);
+      if (j == k + 1) {
+        // store L_ik; b_i -= L_ik y_k (4 threads per row, fixed order)
+        store_tile(sm.X, tile(Lf, i, k), ld);
+        const int r = tid >> 2, p = tid & 3;
+        double s = 0.0;
+#pragma unroll
+        for (int c = 0; c < 16; ++c) s += sm.X[r * TP + 16 * p + c] * sm.v[16 * p + c];
+        s += __shfl_xor(s, 1, 64);
+        s += __shfl_xor(s, 2, 64);
+        if (p == 0) {
+          const double bi = bw[(size_t)i * NB + r] - s;
+          bw[(size_t)i * NB + r] = bi;
+          if (i == j) sm.part[0][r] = bi;
+        }
+      }
+      if (i == j && j == k + 1) {
+        // the next diagonal tile: factor it and form y_{k+1}
+        __syncthreads();
+        acc_store(acc, sm.Y, TP);
+        __syncthreads();
+        const int bad = tile_potrf(sm);
+        if (bad && tid == 0) __hip_atomic_fetch_or(fail, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        store_tile(sm.Y, tile(Lf, i, i), ld);
+        tile_forward(sm.Y, sm.part[0], yv + (size_t)i * NB, sm.w);
+        carry = (t + G >= ntasks);
+      } else {
+        acc_store(acc, tile(A, i, j), ld);
+      }
+      __syncthreads();
+    }
+    grid_barrier(bar, bar_base + (unsigned long long)G * (++nbar), fail);
+  }
+
+  // ---- back substitution L^T x = y, right-looking over block columns.  The
+  // residual r lives in yv (y is not needed afterwards).  Step k: workgroup 0
+  // applies x_{k+1} to r_k and solves x_k; the other workgroups apply x_{k+1}
+  // to r_i, i < k.
+  for (int k = T - 1; k >= 0; --k) {
+    if (blockIdx.x == 0) {
+      load_lower(sm.P, tile(Lf, k, k), ld);
+      double br = yv[(size_t)k * NB + (tid & 63)];
+      if (k + 1 < T) br -= tile_tmatvec(sm, tile(Lf, k + 1, k), ld, sm.w);
+      __syncthreads();
+      tile_backward(sm.P, br, xs + (size_t)k * NB, sm.v);
+      __syncthreads();
+      if (tid < NB) {
+        sm.w[tid] = sm.v[tid];
+        const int q = k * NB + tid;
+        if (q < n) dp[q] = sm.v[tid];
+      }
+      __syncthreads();
+    }
+    if (k + 1 < T) {
+      const int first = (G > 1) ? 1 : 0, stride = (G > 1) ? G - 1 : 1;
+      if ((int)blockIdx.x >= first) {
+        if ((int)blockIdx.x - first < k) {
+          if (tid < NB) sm.v[tid] = xs[(size_t)(k + 1) * NB + tid];
+          __syncthreads();
+        }
+        for (int i = (int)blockIdx.x - first; i < k; i += stride) {
+          const double t = tile_tmatvec(sm, tile(Lf, k + 1, i), ld, sm.v);
+          if (tid < NB) yv[(size_t)i * NB + tid] -= t;
+        }
+      }
+    }
+    if (k > 0) grid_barrier(bar, bar_base + (unsigned long long)G * (++nbar), fail);
+  }
+  if (blockIdx.x == 0 && tid == 0) {
+    const int f = __hip_atomic_load(fail, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    *ok = f ? 0.0 : 1.0;
+  }
+}
+
+__global__ void k_zero_int(int* p) { *p = 0; }
+
+int launch_factor_solve(psba_ctx* c) {
+  const int T = c->n_pad / NB;
+  if (c->chol_grid <= 0) {
+    int per_cu = 0, ncu = 0;
+    PSBA_HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_chol_solve, 256, 0));
+    PSBA_HIP_TRY(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, c->device));
+    // at most one workgroup per CU (the LDS use allows no more), so that every
+    // workgroup of a launch is resident at once on the stream's device
+    c->chol_resident = std::min(per_cu, 1) * ncu;
+    if (c->chol_resident < 1) return PSBA_ERR_UNSUPPORTED;
+    c->chol_grid = std::max(1, std::min(c->chol_resident, (T - 1) * T / 2));
+  }
+  int G = c->chol_grid;
+  unsigned long long base = c->bar_base;
+  c->bar_base += (unsigned long long)G * (unsigned long long)(2 * T - 1);
+  double* ok = c->d_scal + SC_FACTOR_OK;
+  int ld = c->n_pad, n = c->n;
+  probe_begin(c, PSBA_PROBE_FACTOR);
+  hipLaunchKernelGGL(k_zero_int, dim3(1), dim3(1), 0, c->stream, c->chol_fail);
+  hipLaunchKernelGGL(k_chol_solve, dim3(G), dim3(256), 0, c->stream, c->S, ld, T, c->Lf, c->ea, c->bw, c->yv,
+                     c->xsol, c->dp, n, ok, c->bar, base, c->chol_fail);
+  probe_end(c, PSBA_PROBE_FACTOR);
+  c->L_cur = c->Lf;
   PSBA_LAUNCH_TRY();
   return PSBA_OK;
 }
