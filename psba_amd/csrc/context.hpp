@@ -81,8 +81,13 @@ struct psba_ctx {
   int* lblock_pair = nullptr;   // pair index of each lower block position or -1
   double sum_t2 = 0;            // sum over points of track length squared
 
-  // work buffers
+  bool mixed() const { return precision == PSBA_PREC_MIXED; }
+
+  // work buffers (plain path: FP64; the mixed path uses the *32 FP32 copies of
+  // the per-observation arrays and chunk partials instead)
   double *e = nullptr, *JA = nullptr, *JB = nullptr, *W = nullptr;
+  float *e32 = nullptr, *JA32 = nullptr, *JB32 = nullptr, *W32 = nullptr;
+  float *ucpart32 = nullptr, *schpart32 = nullptr, *eapart32 = nullptr;
   double *U = nullptr, *V = nullptr, *g = nullptr, *uvdiag = nullptr;
   double *Vinv = nullptr, *z = nullptr;
   double *ucpart = nullptr, *schpart = nullptr, *eapart = nullptr;

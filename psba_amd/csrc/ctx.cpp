@@ -70,6 +70,9 @@ static void free_index(psba_ctx* c) {
   dfree(c->ucpart);
   dfree(c->eapart);
   dfree(c->schpart);
+  dfree(c->ucpart32);
+  dfree(c->eapart32);
+  dfree(c->schpart32);
   dfree(c->obs_cam);
   dfree(c->obs_pt);
   c->indexed = false;
@@ -147,7 +150,8 @@ int psba_create(const psba_desc* d, psba_ctx** out) {
     return fail(PSBA_ERR_UNSUPPORTED, "cnp=%d not supported (6, 9, 11, 16)", d->cnp);
   if (d->pnp != 3 || d->mnp != 2) return fail(PSBA_ERR_UNSUPPORTED, "pnp must be 3 and mnp 2");
   if (d->n_cams <= 0 || d->n_pts <= 0 || d->n_obs <= 0) return fail(PSBA_ERR_ARG, "empty problem");
-  if (d->precision != PSBA_PREC_FP64) return fail(PSBA_ERR_UNSUPPORTED, "precision %d not available", d->precision);
+  if (d->precision != PSBA_PREC_FP64 && d->precision != PSBA_PREC_MIXED)
+    return fail(PSBA_ERR_UNSUPPORTED, "precision %d not available", d->precision);
   {
     long n = (long)d->cnp * d->n_cams, n_pad = (n + kNB - 1) / kNB * kNB;
     if (n_pad > kMaxSolveN)
@@ -180,9 +184,16 @@ int psba_create(const psba_desc* d, psba_ctx** out) {
   chk(dalloc(&c->pts, 3 * (size_t)c->np));
   chk(dalloc(&c->fix, (size_t)C * c->nc));
   chk(dalloc(&c->e, 2 * (size_t)c->no));
-  chk(dalloc(&c->JA, 2 * (size_t)C * c->no));
-  chk(dalloc(&c->JB, 6 * (size_t)c->no));
-  chk(dalloc(&c->W, 3 * (size_t)C * c->no));
+  if (c->precision == PSBA_PREC_MIXED) {
+    chk(dalloc(&c->e32, 2 * (size_t)c->no));
+    chk(dalloc(&c->JA32, 2 * (size_t)C * c->no));
+    chk(dalloc(&c->JB32, 6 * (size_t)c->no));
+    chk(dalloc(&c->W32, 3 * (size_t)C * c->no));
+  } else {
+    chk(dalloc(&c->JA, 2 * (size_t)C * c->no));
+    chk(dalloc(&c->JB, 6 * (size_t)c->no));
+    chk(dalloc(&c->W, 3 * (size_t)C * c->no));
+  }
   chk(dalloc(&c->U, (size_t)C * C * c->nc));
   chk(dalloc(&c->V, 6 * (size_t)c->np));
   chk(dalloc(&c->g, (size_t)c->ntot));
@@ -250,6 +261,10 @@ int psba_destroy(psba_ctx* c) {
     if (p) (void)hipFree(p);
   dfree(c->fix);
   dfree(c->counters);
+  dfree(c->e32);
+  dfree(c->JA32);
+  dfree(c->JB32);
+  dfree(c->W32);
   dfree(c->bar);
   dfree(c->chol_fail);
   if (c->h_scal) (void)hipHostFree(c->h_scal);
@@ -464,9 +479,15 @@ int psba_set_index(psba_ctx* c, const int* iidx, const int* jidx) {
   PSBA_TRY(dalloc(&c->schunk_end, se.size()));
   PSBA_TRY(dalloc(&c->pair_schunk_ptr, npairs + 1));
   PSBA_TRY(dalloc(&c->lblock_pair, nlb));
-  PSBA_TRY(dalloc(&c->ucpart, (size_t)c->n_cchunk * NE));
-  PSBA_TRY(dalloc(&c->eapart, (size_t)c->n_cchunk * C));
-  PSBA_TRY(dalloc(&c->schpart, (size_t)c->n_schunk * C * C));
+  if (c->mixed()) {
+    PSBA_TRY(dalloc(&c->ucpart32, (size_t)c->n_cchunk * NE));
+    PSBA_TRY(dalloc(&c->eapart32, (size_t)c->n_cchunk * C));
+    PSBA_TRY(dalloc(&c->schpart32, (size_t)c->n_schunk * C * C));
+  } else {
+    PSBA_TRY(dalloc(&c->ucpart, (size_t)c->n_cchunk * NE));
+    PSBA_TRY(dalloc(&c->eapart, (size_t)c->n_cchunk * C));
+    PSBA_TRY(dalloc(&c->schpart, (size_t)c->n_schunk * C * C));
+  }
   auto up = [&](void* dst, const void* src, size_t bytes) -> int {
     if (bytes == 0) return PSBA_OK;
     PSBA_HIP_TRY(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, c->stream));
@@ -616,6 +637,21 @@ int psba_read_buffer(psba_ctx* c, int which, double* out, long n) {
     PSBA_HIP_TRY(hipStreamSynchronize(c->stream));
     return PSBA_OK;
   };
+  auto f2h = [&](double* dst, const float* src, size_t cnt) -> int {
+    std::vector<float> h(cnt);
+    PSBA_HIP_TRY(hipMemcpyAsync(h.data(), src, sizeof(float) * cnt, hipMemcpyDeviceToHost, c->stream));
+    PSBA_HIP_TRY(hipStreamSynchronize(c->stream));
+    for (size_t q = 0; q < cnt; ++q) dst[q] = h[q];
+    return PSBA_OK;
+  };
+  if (c->mixed()) {
+    switch (which) {
+      case PSBA_BUF_EX: return f2h(out, c->e32, need);
+      case PSBA_BUF_JA: return f2h(out, c->JA32, need);
+      case PSBA_BUF_JB: return f2h(out, c->JB32, need);
+      case PSBA_BUF_W: return f2h(out, c->W32, need);
+    }
+  }
   switch (which) {
     case PSBA_BUF_EX: return d2h(out, c->e, need);
     case PSBA_BUF_JA: return d2h(out, c->JA, need);

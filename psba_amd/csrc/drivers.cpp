@@ -39,6 +39,13 @@ void push(psba_trace_rec* tr, int cap, int* len, const psba_trace_rec& r) {
 int opt_max_iter(const psba_opts* o) { return (o && o->max_iter > 0) ? o->max_iter : 50; }
 int opt_max_pb(const psba_opts* o) { return (o && o->max_pb_tries > 0) ? o->max_pb_tries : 64; }
 
+int fail_unsupported_mixed(const char* what) {
+  char buf[200];
+  snprintf(buf, sizeof(buf), "%s: the dogleg runs on FP64 contexts only (create with PSBA_PREC_FP64)", what);
+  set_last_error(__FILE__, __LINE__, buf);
+  return PSBA_ERR_UNSUPPORTED;
+}
+
 int check_ready(psba_ctx* c) {
   if (!c) return PSBA_ERR_ARG;
   if (!c->uploaded || !c->indexed) return PSBA_ERR_STATE;
@@ -284,6 +291,7 @@ extern "C" {
 int psba_trust_region(psba_ctx* c, const psba_opts* o, int* itno, double* finalErr, psba_trace_rec* trace, int cap,
                       int* len) {
   PSBA_TRY(check_ready(c));
+  if (c->mixed()) return fail_unsupported_mixed("psba_trust_region");
   if (!itno || !finalErr) return PSBA_ERR_ARG;
   const int max_iter = opt_max_iter(o), max_pb = opt_max_pb(o);
   int notgood_cnt = 0, good_iters = 0, nu = 2;
@@ -413,6 +421,7 @@ int psba_trust_region(psba_ctx* c, const psba_opts* o, int* itno, double* finalE
 int psba_hybrid(psba_ctx* c, const psba_opts* o, int* itno, double* initErr, double* finalErr,
                 psba_trace_rec* trace, int cap, int* len) {
   PSBA_TRY(check_ready(c));
+  if (c->mixed()) return fail_unsupported_mixed("psba_hybrid");
   if (initErr) {
     PSBA_TRY(launch_residual(c, false));
     PSBA_TRY(fetch(c));

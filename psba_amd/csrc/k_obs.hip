@@ -72,14 +72,16 @@ int launch_residual(psba_ctx* c, bool trial) {
 // ---------------------------------------------------------------------------
 // Jacobian, W and residual per observation
 // ---------------------------------------------------------------------------
-template <int CNP>
+// R is the storage (and accumulation) type of the normal-equation stages:
+// double on the plain path, float on the mixed path; the projection and its
+// derivatives are always evaluated in FP64.
+template <int CNP, typename R>
 __global__ __launch_bounds__(256) void k_jacobian(const double* __restrict__ K, const double* __restrict__ q0,
                                                   const double* __restrict__ cams, const double* __restrict__ pts,
                                                   const double* __restrict__ meas, const int* __restrict__ obs_cam,
                                                   const int* __restrict__ obs_pt, const unsigned char* __restrict__ fix,
-                                                  int no, double coeff, double* __restrict__ e,
-                                                  double* __restrict__ JA, double* __restrict__ JB,
-                                                  double* __restrict__ W) {
+                                                  int no, double coeff, R* __restrict__ e, R* __restrict__ JA,
+                                                  R* __restrict__ JB, R* __restrict__ W) {
   int o = blockIdx.x * blockDim.x + threadIdx.x;
   if (o >= no) return;
   int j = obs_cam[o], i = obs_pt[o];
@@ -93,33 +95,39 @@ __global__ __launch_bounds__(256) void k_jacobian(const double* __restrict__ K, 
   for (int k = 0; k < CNP; ++k)
     if (fix[j * CNP + k]) ja[k] = ja[CNP + k] = 0.0;
   double e0 = meas[2 * o] - uv[0], e1 = meas[2 * o + 1] - uv[1];
-  e[2 * o] = e0;
-  e[2 * o + 1] = e1;
-  double* pa = JA + (size_t)o * 2 * CNP;
+  e[2 * o] = (R)e0;
+  e[2 * o + 1] = (R)e1;
+  R* pa = JA + (size_t)o * 2 * CNP;
 #pragma unroll
-  for (int k = 0; k < 2 * CNP; ++k) pa[k] = ja[k];
+  for (int k = 0; k < 2 * CNP; ++k) pa[k] = (R)ja[k];
 #pragma unroll
-  for (int k = 0; k < 6; ++k) JB[(size_t)o * 6 + k] = jb[k];
-  double* pw = W + (size_t)o * 3 * CNP;
+  for (int k = 0; k < 6; ++k) JB[(size_t)o * 6 + k] = (R)jb[k];
+  R* pw = W + (size_t)o * 3 * CNP;
+  R jr[2 * CNP], br[6];
+#pragma unroll
+  for (int k = 0; k < 2 * CNP; ++k) jr[k] = (R)ja[k];
+#pragma unroll
+  for (int k = 0; k < 6; ++k) br[k] = (R)jb[k];
 #pragma unroll
   for (int r = 0; r < CNP; ++r)
 #pragma unroll
-    for (int cc = 0; cc < 3; ++cc) pw[3 * r + cc] = coeff * (ja[r] * jb[cc] + ja[CNP + r] * jb[3 + cc]);
+    for (int cc = 0; cc < 3; ++cc) pw[3 * r + cc] = (R)coeff * (jr[r] * br[cc] + jr[CNP + r] * br[3 + cc]);
 }
 
 // V_i (6 unique entries), g_b,i and the V part of UVdiag, one thread per point
 // over its observations in ascending camera order.
-__global__ __launch_bounds__(256) void k_point_normal(const int* __restrict__ pt_ptr, const double* __restrict__ JB,
-                                                      const double* __restrict__ e, int np, double coeff,
+template <typename R>
+__global__ __launch_bounds__(256) void k_point_normal(const int* __restrict__ pt_ptr, const R* __restrict__ JB,
+                                                      const R* __restrict__ e, int np, double coeff,
                                                       double coeff_g, double* __restrict__ V,
                                                       double* __restrict__ gb, double* __restrict__ uvd) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= np) return;
-  double v[6] = {0, 0, 0, 0, 0, 0}, g[3] = {0, 0, 0};
+  R v[6] = {0, 0, 0, 0, 0, 0}, g[3] = {0, 0, 0};
   for (int o = pt_ptr[i]; o < pt_ptr[i + 1]; ++o) {
-    const double* b = JB + (size_t)o * 6;
-    double b0 = b[0], b1 = b[1], b2 = b[2], b3 = b[3], b4 = b[4], b5 = b[5];
-    double e0 = e[2 * o], e1 = e[2 * o + 1];
+    const R* b = JB + (size_t)o * 6;
+    R b0 = b[0], b1 = b[1], b2 = b[2], b3 = b[3], b4 = b[4], b5 = b[5];
+    R e0 = e[2 * o], e1 = e[2 * o + 1];
     v[0] += b0 * b0 + b3 * b3;
     v[1] += b0 * b1 + b3 * b4;
     v[2] += b0 * b2 + b3 * b5;
@@ -131,24 +139,24 @@ __global__ __launch_bounds__(256) void k_point_normal(const int* __restrict__ pt
     g[2] += b2 * e0 + b5 * e1;
   }
 #pragma unroll
-  for (int k = 0; k < 6; ++k) V[(size_t)i * 6 + k] = coeff * v[k];
+  for (int k = 0; k < 6; ++k) V[(size_t)i * 6 + k] = coeff * (double)v[k];
 #pragma unroll
-  for (int k = 0; k < 3; ++k) gb[(size_t)i * 3 + k] = coeff_g * g[k];
-  uvd[(size_t)i * 3 + 0] = coeff * v[0];
-  uvd[(size_t)i * 3 + 1] = coeff * v[3];
-  uvd[(size_t)i * 3 + 2] = coeff * v[5];
+  for (int k = 0; k < 3; ++k) gb[(size_t)i * 3 + k] = coeff_g * (double)g[k];
+  uvd[(size_t)i * 3 + 0] = coeff * (double)v[0];
+  uvd[(size_t)i * 3 + 1] = coeff * (double)v[3];
+  uvd[(size_t)i * 3 + 2] = coeff * (double)v[5];
 }
 
 // Camera-segmented sums of the U_j upper triangle and g_a,j: stage 1.  Each
 // block takes one chunk (<= kCamChunk observations of one camera, ascending
 // point), stages JA and e in LDS, and thread t sums entry t over the chunk.
-template <int CNP>
+template <int CNP, typename R>
 __global__ __launch_bounds__(256) void k_cam_chunks(const int* __restrict__ cam_obs, const int* __restrict__ cb,
-                                                    const int* __restrict__ ce, const double* __restrict__ JA,
-                                                    const double* __restrict__ e, double* __restrict__ part) {
+                                                    const int* __restrict__ ce, const R* __restrict__ JA,
+                                                    const R* __restrict__ e, R* __restrict__ part) {
   constexpr int NU = CNP * (CNP + 1) / 2, NE = NU + CNP, REC = 2 * CNP + 2;
   constexpr int CH = (CNP > 11) ? 128 : kCamChunk;
-  __shared__ double sJ[CH * REC];
+  __shared__ R sJ[CH * REC];
   const int b = blockIdx.x;
   const int beg = cb[b], cnt = ce[b] - beg;
   for (int f = threadIdx.x; f < cnt * REC; f += blockDim.x) {
@@ -158,7 +166,7 @@ __global__ __launch_bounds__(256) void k_cam_chunks(const int* __restrict__ cam_
   }
   __syncthreads();
   for (int t = threadIdx.x; t < NE; t += blockDim.x) {
-    double acc = 0.0;
+    R acc = 0;
     if (t < NU) {
       int r = 0, rem = t;
       while (rem >= CNP - r) {
@@ -167,13 +175,13 @@ __global__ __launch_bounds__(256) void k_cam_chunks(const int* __restrict__ cam_
       }
       int cc = r + rem;
       for (int s = 0; s < cnt; ++s) {
-        const double* a = sJ + s * REC;
+        const R* a = sJ + s * REC;
         acc += a[r] * a[cc] + a[CNP + r] * a[CNP + cc];
       }
     } else {
       int r = t - NU;
       for (int s = 0; s < cnt; ++s) {
-        const double* a = sJ + s * REC;
+        const R* a = sJ + s * REC;
         acc += a[r] * a[2 * CNP] + a[CNP + r] * a[2 * CNP + 1];
       }
     }
@@ -182,15 +190,15 @@ __global__ __launch_bounds__(256) void k_cam_chunks(const int* __restrict__ cam_
 }
 
 // stage 2: per camera, sum its chunk partials in chunk order.
-template <int CNP>
-__global__ __launch_bounds__(256) void k_cam_finish(const int* __restrict__ ccp, const double* __restrict__ part,
+template <int CNP, typename R>
+__global__ __launch_bounds__(256) void k_cam_finish(const int* __restrict__ ccp, const R* __restrict__ part,
                                                     int nc, double coeff, double coeff_g, double* __restrict__ U,
                                                     double* __restrict__ ga, double* __restrict__ uvd) {
   constexpr int NU = CNP * (CNP + 1) / 2, NE = NU + CNP;
   const int j = blockIdx.x;
   for (int t = threadIdx.x; t < NE; t += blockDim.x) {
     double acc = 0.0;
-    for (int q = ccp[j]; q < ccp[j + 1]; ++q) acc += part[(size_t)q * NE + t];
+    for (int q = ccp[j]; q < ccp[j + 1]; ++q) acc += (double)part[(size_t)q * NE + t];
     if (t < NU) {
       int r = 0, rem = t;
       while (rem >= CNP - r) {
@@ -208,19 +216,27 @@ __global__ __launch_bounds__(256) void k_cam_finish(const int* __restrict__ ccp,
   }
 }
 
-template <int CNP>
-static void run_jac_normal(psba_ctx* c, double coeff, double coeff_g) {
+template <int CNP, typename R>
+static void run_jac_normal_t(psba_ctx* c, double coeff, double coeff_g, R* e, R* JA, R* JB, R* W, R* ucpart) {
   probe_begin(c, PSBA_PROBE_JACOBIAN);
-  hipLaunchKernelGGL(k_jacobian<CNP>, dim3(ceil_div(c->no, 256)), dim3(256), 0, c->stream, c->K, c->q0, c->cams,
-                     c->pts, c->meas, c->obs_cam, c->obs_pt, c->fix, c->no, coeff, c->e, c->JA, c->JB, c->W);
+  hipLaunchKernelGGL((k_jacobian<CNP, R>), dim3(ceil_div(c->no, 256)), dim3(256), 0, c->stream, c->K, c->q0,
+                     c->cams, c->pts, c->meas, c->obs_cam, c->obs_pt, c->fix, c->no, coeff, e, JA, JB, W);
   probe_end(c, PSBA_PROBE_JACOBIAN);
-  hipLaunchKernelGGL(k_point_normal, dim3(ceil_div(c->np, 256)), dim3(256), 0, c->stream, c->pt_ptr, c->JB, c->e,
+  hipLaunchKernelGGL(k_point_normal<R>, dim3(ceil_div(c->np, 256)), dim3(256), 0, c->stream, c->pt_ptr, JB, e,
                      c->np, coeff, coeff_g, c->V, c->g + c->n, c->uvdiag + c->n);
   if (c->n_cchunk > 0)
-    hipLaunchKernelGGL(k_cam_chunks<CNP>, dim3(c->n_cchunk), dim3(256), 0, c->stream, c->cam_obs, c->cchunk_begin,
-                       c->cchunk_end, c->JA, c->e, c->ucpart);
-  hipLaunchKernelGGL(k_cam_finish<CNP>, dim3(c->nc), dim3(256), 0, c->stream, c->cam_cchunk_ptr, c->ucpart, c->nc,
+    hipLaunchKernelGGL((k_cam_chunks<CNP, R>), dim3(c->n_cchunk), dim3(256), 0, c->stream, c->cam_obs,
+                       c->cchunk_begin, c->cchunk_end, JA, e, ucpart);
+  hipLaunchKernelGGL((k_cam_finish<CNP, R>), dim3(c->nc), dim3(256), 0, c->stream, c->cam_cchunk_ptr, ucpart, c->nc,
                      coeff, coeff_g, c->U, c->g, c->uvdiag);
+}
+
+template <int CNP>
+static void run_jac_normal(psba_ctx* c, double coeff, double coeff_g) {
+  if (c->mixed())
+    run_jac_normal_t<CNP, float>(c, coeff, coeff_g, c->e32, c->JA32, c->JB32, c->W32, c->ucpart32);
+  else
+    run_jac_normal_t<CNP, double>(c, coeff, coeff_g, c->e, c->JA, c->JB, c->W, c->ucpart);
 }
 
 int launch_jacobian_normal(psba_ctx* c, double coeff, double coeff_g) {
@@ -313,26 +329,26 @@ int launch_vinv(psba_ctx* c, double mu) {
 // ---------------------------------------------------------------------------
 // back-substitution: eb_i = g_b,i - sum_j W_ij^T dpa_j ; dpb_i = V_i^-1 eb_i
 // ---------------------------------------------------------------------------
-template <int CNP>
+template <int CNP, typename R>
 __global__ __launch_bounds__(256) void k_backsub(const int* __restrict__ pt_ptr, const int* __restrict__ obs_cam,
-                                                 const double* __restrict__ W, const double* __restrict__ Vinv,
+                                                 const R* __restrict__ W, const double* __restrict__ Vinv,
                                                  const double* __restrict__ gb, int np, int n,
                                                  double* __restrict__ dp, double* __restrict__ eb) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= np) return;
-  double s[3] = {0, 0, 0};
+  R s[3] = {0, 0, 0};
   for (int o = pt_ptr[i]; o < pt_ptr[i + 1]; ++o) {
-    const double* w = W + (size_t)o * 3 * CNP;
+    const R* w = W + (size_t)o * 3 * CNP;
     const double* da = dp + obs_cam[o] * CNP;
 #pragma unroll
     for (int k = 0; k < CNP; ++k) {
-      double d = da[k];
+      R d = (R)da[k];
       s[0] += w[3 * k] * d;
       s[1] += w[3 * k + 1] * d;
       s[2] += w[3 * k + 2] * d;
     }
   }
-  double e0 = gb[3 * i] - s[0], e1 = gb[3 * i + 1] - s[1], e2 = gb[3 * i + 2] - s[2];
+  double e0 = gb[3 * i] - (double)s[0], e1 = gb[3 * i + 1] - (double)s[1], e2 = gb[3 * i + 2] - (double)s[2];
   eb[3 * i] = e0;
   eb[3 * i + 1] = e1;
   eb[3 * i + 2] = e2;
@@ -344,8 +360,12 @@ __global__ __launch_bounds__(256) void k_backsub(const int* __restrict__ pt_ptr,
 
 template <int CNP>
 static void run_backsub(psba_ctx* c) {
-  hipLaunchKernelGGL(k_backsub<CNP>, dim3(ceil_div(c->np, 256)), dim3(256), 0, c->stream, c->pt_ptr, c->obs_cam,
-                     c->W, c->Vinv, c->g + c->n, c->np, c->n, c->dp, c->eb);
+  if (c->mixed())
+    hipLaunchKernelGGL((k_backsub<CNP, float>), dim3(ceil_div(c->np, 256)), dim3(256), 0, c->stream, c->pt_ptr,
+                       c->obs_cam, c->W32, c->Vinv, c->g + c->n, c->np, c->n, c->dp, c->eb);
+  else
+    hipLaunchKernelGGL((k_backsub<CNP, double>), dim3(ceil_div(c->np, 256)), dim3(256), 0, c->stream, c->pt_ptr,
+                       c->obs_cam, c->W, c->Vinv, c->g + c->n, c->np, c->n, c->dp, c->eb);
 }
 
 int launch_backsub(psba_ctx* c) {

@@ -5,31 +5,34 @@
 // Y = W V^-1 is formed on the fly in LDS (never written to HBM); e_a uses
 // z_i = V_i^-1 g_b,i from the inverse kernel.  Both sums are deterministic
 // two-stage reductions over fixed chunks (item order = ascending point).
+// R is the storage and chunk-accumulation type (double on the plain path,
+// float on the mixed path); V^-1 is always FP64 and the chunk partials are
+// summed in FP64.
 #include "context.hpp"
 
 namespace psba {
 
-template <int CNP>
+template <int CNP, typename R>
 __global__ __launch_bounds__(256) void k_schur_chunks(const int2* __restrict__ items, const int* __restrict__ sb,
                                                       const int* __restrict__ se, const int* __restrict__ obs_pt,
-                                                      const double* __restrict__ W, const double* __restrict__ Vinv,
-                                                      double* __restrict__ part) {
+                                                      const R* __restrict__ W, const double* __restrict__ Vinv,
+                                                      R* __restrict__ part) {
   constexpr int C2 = CNP * CNP, REC = 3 * CNP;
   constexpr int G = (256 / C2) > 0 ? (256 / C2) : 1;
-  __shared__ double sY[kPairChunk * REC];
-  __shared__ double sW[kPairChunk * REC];
-  __shared__ double sG[G * C2];
+  __shared__ R sY[kPairChunk * REC];
+  __shared__ R sW[kPairChunk * REC];
+  __shared__ R sG[G * C2];
   const int b = blockIdx.x;
   const int beg = sb[b], cnt = se[b] - beg;
   // Y rows: task = (item, row r) -> Y[it][r][0..2] = W_a[r][:] V^-1
   for (int task = threadIdx.x; task < cnt * CNP; task += blockDim.x) {
     int it = task / CNP, r = task - it * CNP;
     int2 pr = items[beg + it];
-    const double* w = W + (size_t)pr.x * REC + 3 * r;
+    const R* w = W + (size_t)pr.x * REC + 3 * r;
     const double* vi = Vinv + (size_t)obs_pt[pr.x] * 6;
-    double w0 = w[0], w1 = w[1], w2 = w[2];
-    double v00 = vi[0], v01 = vi[1], v02 = vi[2], v11 = vi[3], v12 = vi[4], v22 = vi[5];
-    double* y = sY + it * REC + 3 * r;
+    R w0 = w[0], w1 = w[1], w2 = w[2];
+    R v00 = (R)vi[0], v01 = (R)vi[1], v02 = (R)vi[2], v11 = (R)vi[3], v12 = (R)vi[4], v22 = (R)vi[5];
+    R* y = sY + it * REC + 3 * r;
     y[0] = w0 * v00 + w1 * v01 + w2 * v02;
     y[1] = w0 * v01 + w1 * v11 + w2 * v12;
     y[2] = w0 * v02 + w1 * v12 + w2 * v22;
@@ -42,17 +45,17 @@ __global__ __launch_bounds__(256) void k_schur_chunks(const int2* __restrict__ i
   const int grp = threadIdx.x / C2, ent = threadIdx.x - grp * C2;
   if (grp < G) {
     const int r = ent / CNP, cc = ent - r * CNP;
-    double acc = 0.0;
+    R acc = 0;
     for (int it = grp; it < cnt; it += G) {
-      const double* y = sY + it * REC + 3 * r;
-      const double* w = sW + it * REC + 3 * cc;
+      const R* y = sY + it * REC + 3 * r;
+      const R* w = sW + it * REC + 3 * cc;
       acc += y[0] * w[0] + y[1] * w[1] + y[2] * w[2];
     }
     sG[grp * C2 + ent] = acc;
   }
   __syncthreads();
   for (int t = threadIdx.x; t < C2; t += blockDim.x) {
-    double s = 0.0;
+    R s = 0;
     for (int q = 0; q < G; ++q) s += sG[q * C2 + t];
     part[(size_t)b * C2 + t] = s;
   }
@@ -62,9 +65,9 @@ __global__ __launch_bounds__(256) void k_schur_chunks(const int2* __restrict__ i
 // Blocks [0, nlb) write the lower block positions; the remaining blocks
 // rewrite the padding rows n..n_pad-1 as identity rows (a failed
 // factorisation may have left anything there).
-template <int CNP>
+template <int CNP, typename R>
 __global__ __launch_bounds__(256) void k_schur_finish(const int* __restrict__ lblock_pair,
-                                                      const int* __restrict__ psp, const double* __restrict__ part,
+                                                      const int* __restrict__ psp, const R* __restrict__ part,
                                                       const double* __restrict__ U,
                                                       const unsigned char* __restrict__ fix, double mu, int ld,
                                                       int nlb, int n, double* __restrict__ S) {
@@ -84,7 +87,7 @@ __global__ __launch_bounds__(256) void k_schur_finish(const int* __restrict__ lb
     const int r = t / CNP, cc = t - r * CNP;
     double acc = 0.0;
     if (q >= 0)
-      for (int h = psp[q]; h < psp[q + 1]; ++h) acc += part[(size_t)h * C2 + t];
+      for (int h = psp[q]; h < psp[q + 1]; ++h) acc += (double)part[(size_t)h * C2 + t];
     double val;
     if (k == l) {
       double u = U[(size_t)k * C2 + t];
@@ -99,57 +102,65 @@ __global__ __launch_bounds__(256) void k_schur_finish(const int* __restrict__ lb
 }
 
 // e_a chunks over the camera chunks: p_o = W_o z_i, column sums
-template <int CNP>
+template <int CNP, typename R>
 __global__ __launch_bounds__(256) void k_ea_chunks(const int* __restrict__ cam_obs, const int* __restrict__ cb,
                                                    const int* __restrict__ ce, const int* __restrict__ obs_pt,
-                                                   const double* __restrict__ W, const double* __restrict__ z,
-                                                   double* __restrict__ part) {
+                                                   const R* __restrict__ W, const double* __restrict__ z,
+                                                   R* __restrict__ part) {
   constexpr int CH = (CNP > 11) ? 128 : kCamChunk;
-  __shared__ double sP[CH * CNP];
+  __shared__ R sP[CH * CNP];
   const int b = blockIdx.x;
   const int beg = cb[b], cnt = ce[b] - beg;
   for (int task = threadIdx.x; task < cnt * CNP; task += blockDim.x) {
     int s = task / CNP, r = task - s * CNP;
     int o = cam_obs[beg + s];
-    const double* w = W + (size_t)o * 3 * CNP + 3 * r;
+    const R* w = W + (size_t)o * 3 * CNP + 3 * r;
     const double* zi = z + (size_t)obs_pt[o] * 3;
-    sP[task] = w[0] * zi[0] + w[1] * zi[1] + w[2] * zi[2];
+    sP[task] = w[0] * (R)zi[0] + w[1] * (R)zi[1] + w[2] * (R)zi[2];
   }
   __syncthreads();
   if (threadIdx.x < CNP) {
-    double acc = 0.0;
+    R acc = 0;
     for (int s = 0; s < cnt; ++s) acc += sP[s * CNP + threadIdx.x];
     part[(size_t)b * CNP + threadIdx.x] = acc;
   }
 }
 
-template <int CNP>
-__global__ __launch_bounds__(64) void k_ea_finish(const int* __restrict__ ccp, const double* __restrict__ part,
+template <int CNP, typename R>
+__global__ __launch_bounds__(64) void k_ea_finish(const int* __restrict__ ccp, const R* __restrict__ part,
                                                   const double* __restrict__ ga, int nc, double* __restrict__ ea) {
   int t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= nc * CNP) return;
   int j = t / CNP, r = t - j * CNP;
   double acc = 0.0;
-  for (int q = ccp[j]; q < ccp[j + 1]; ++q) acc += part[(size_t)q * CNP + r];
+  for (int q = ccp[j]; q < ccp[j + 1]; ++q) acc += (double)part[(size_t)q * CNP + r];
   ea[t] = ga[t] - acc;
+}
+
+template <int CNP, typename R>
+static void run_schur_t(psba_ctx* c, double mu, const R* W, R* schpart, R* eapart) {
+  if (c->n_schunk > 0) {
+    probe_begin(c, PSBA_PROBE_SCHUR);
+    hipLaunchKernelGGL((k_schur_chunks<CNP, R>), dim3(c->n_schunk), dim3(256), 0, c->stream, c->pair_items,
+                       c->schunk_begin, c->schunk_end, c->obs_pt, W, c->Vinv, schpart);
+    probe_end(c, PSBA_PROBE_SCHUR);
+  }
+  hipLaunchKernelGGL((k_schur_finish<CNP, R>), dim3(c->n_lblocks + (c->n_pad - c->n)),
+                     dim3(CNP * CNP > 128 ? 256 : 128), 0, c->stream, c->lblock_pair, c->pair_schunk_ptr, schpart, c->U,
+                     c->fix, mu, c->n_pad, c->n_lblocks, c->n, c->S);
+  if (c->n_cchunk > 0)
+    hipLaunchKernelGGL((k_ea_chunks<CNP, R>), dim3(c->n_cchunk), dim3(256), 0, c->stream, c->cam_obs,
+                       c->cchunk_begin, c->cchunk_end, c->obs_pt, W, c->z, eapart);
+  hipLaunchKernelGGL((k_ea_finish<CNP, R>), dim3(ceil_div((long)c->nc * CNP, 64)), dim3(64), 0, c->stream,
+                     c->cam_cchunk_ptr, eapart, c->g, c->nc, c->ea);
 }
 
 template <int CNP>
 static void run_schur(psba_ctx* c, double mu) {
-  if (c->n_schunk > 0) {
-    probe_begin(c, PSBA_PROBE_SCHUR);
-    hipLaunchKernelGGL(k_schur_chunks<CNP>, dim3(c->n_schunk), dim3(256), 0, c->stream, c->pair_items,
-                       c->schunk_begin, c->schunk_end, c->obs_pt, c->W, c->Vinv, c->schpart);
-    probe_end(c, PSBA_PROBE_SCHUR);
-  }
-  hipLaunchKernelGGL(k_schur_finish<CNP>, dim3(c->n_lblocks + (c->n_pad - c->n)), dim3(CNP * CNP > 128 ? 256 : 128),
-                     0, c->stream, c->lblock_pair, c->pair_schunk_ptr, c->schpart, c->U, c->fix, mu, c->n_pad,
-                     c->n_lblocks, c->n, c->S);
-  if (c->n_cchunk > 0)
-    hipLaunchKernelGGL(k_ea_chunks<CNP>, dim3(c->n_cchunk), dim3(256), 0, c->stream, c->cam_obs, c->cchunk_begin,
-                       c->cchunk_end, c->obs_pt, c->W, c->z, c->eapart);
-  hipLaunchKernelGGL(k_ea_finish<CNP>, dim3(ceil_div((long)c->nc * CNP, 64)), dim3(64), 0, c->stream,
-                     c->cam_cchunk_ptr, c->eapart, c->g, c->nc, c->ea);
+  if (c->mixed())
+    run_schur_t<CNP, float>(c, mu, c->W32, c->schpart32, c->eapart32);
+  else
+    run_schur_t<CNP, double>(c, mu, c->W, c->schpart, c->eapart);
 }
 
 int launch_schur(psba_ctx* c, double mu) {

@@ -288,3 +288,46 @@ def test_bad_index_is_rejected(P, gpu):
     with pytest.raises(P.PsbaError):
         c.levmar()  # no index: state error, nothing launched
 
+
+
+# ---- mixed precision (FP32 Jacobian / W / V / Schur products, FP64 factor and cost) ----
+
+@pytest.mark.parametrize("config,kw", [(0, {}), (2, dict(n_pts=3000, n_cams=20))], ids=["c0-cnp6", "c2-cnp9"])
+def test_mixed_stages_track_fp64(P, O, gpu, config, kw):
+    """FP32 storage and chunk accumulation: the normal-equation blocks and the
+    reduced system agree with the FP64 oracle to FP32 accuracy."""
+    sc = P.Scene(config, 5100 + config, **kw)
+    g = P.Context.from_scene(sc, precision=P.PREC_MIXED)
+    o = O.Oracle.from_scene(sc)
+    g.residual()
+    o.residual()
+    g.jacobian_normal()
+    o.jacobian_normal()
+    for b in ("JA", "JB", "W", "U", "V", "G"):
+        assert rel(g.buffer(b), o.buffer(b)) < 2e-5, b
+    mu = 1e-3 * o.max_uvdiag()
+    g.schur(mu)
+    o.schur(mu)
+    n = sc.cnp * sc.n_cams
+    tl = np.tril_indices(n)
+    assert rel(g.buffer("S").reshape(n, n)[tl], o.buffer("S").reshape(n, n)[tl]) < 2e-5
+    assert rel(g.buffer("EA"), o.buffer("EA")) < 2e-5
+
+
+def test_mixed_solve_reaches_fp64_minimum(P, O, gpu):
+    """The mixed path converges to the FP64 oracle's minimum: final cost (and so
+    RMS reprojection error) within 1e-6 relative, outlier-free sparse scene."""
+    sc = P.Scene(2, 5200, n_pts=8000, n_cams=40, outlier_frac=0.0)
+    g = P.Context.from_scene(sc, precision=P.PREC_MIXED)
+    o = O.Oracle.from_scene(sc)
+    rg = g.levmar(max_iter=100, flags=P.FLAG_NO_HANDOFF, rel_tol=1e-8)
+    ro = o.levmar(max_iter=100, flags=O.FLAG_NO_HANDOFF, rel_tol=1e-8)
+    assert rg["code"] in (P.ITER_DP_NO_CHANGE, P.ITER_ERR_SMALL_ENOUGH)
+    assert abs(rg["final_err"] - ro["final_err"]) <= 1e-6 * ro["final_err"]
+
+
+def test_mixed_context_rejects_dogleg(P, gpu):
+    sc = P.Scene(0, 1)
+    g = P.Context.from_scene(sc, precision=P.PREC_MIXED)
+    with pytest.raises(P.PsbaError, match="FP64 contexts only"):
+        g.trust_region()
