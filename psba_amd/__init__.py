@@ -94,6 +94,10 @@ class SbaData(ct.Structure):
                 ("covimgpts", ct.POINTER(ct.c_double)), ("vmask", ct.POINTER(ct.c_char))]
 
 
+class BatchDesc(ct.Structure):
+    _fields_ = [("cnp", ct.c_int), ("precision", ct.c_int), ("device", ct.c_int), ("max_problems", ct.c_int)]
+
+
 _lib = None
 
 
@@ -144,6 +148,20 @@ def lib():
                                      ct.POINTER(SbaData)]),
         "psba_sba_free": (None, [ct.POINTER(SbaData)]),
         "psba_quat2vec": (None, [dp, ct.c_int, dp, ct.c_int]),
+        "psba_problem_bytes": (ct.c_longlong, [ct.c_int, ct.c_int, ct.c_int, ct.c_int, ct.c_int, ct.c_longlong]),
+        "psba_batch_create": (ct.c_int, [ct.POINTER(BatchDesc), ct.POINTER(vp)]),
+        "psba_batch_destroy": (ct.c_int, [vp]),
+        "psba_batch_bytes": (ct.c_longlong, [vp]),
+        "psba_batch_reset": (ct.c_int, [vp]),
+        "psba_batch_add": (ct.c_int, [vp, ct.c_int, ct.c_int, ct.c_int, dp, dp, dp, dp, dp, ip, ip,
+                                      ct.POINTER(ct.c_ubyte), ip]),
+        "psba_batch_commit": (ct.c_int, [vp]),
+        "psba_batch_launch": (ct.c_int, [vp, ct.POINTER(Opts)]),
+        "psba_batch_poll": (ct.c_int, [vp]),
+        "psba_batch_solve": (ct.c_int, [vp, ct.POINTER(Opts), ip]),
+        "psba_batch_result": (ct.c_int, [vp, ct.c_int, ip, ip, dp, dp, dp, dp]),
+        "psba_batch_elapsed": (ct.c_int, [vp, dp, ip]),
+        "psba_batch_autotune": (ct.c_int, [vp, ip, ip]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -413,3 +431,113 @@ def generate_idxs(n_cams, n_pts, n_obs, vmask, with_comm=False):
                                   _iptr(jidx), _iptr(blk))
     _check(rc, "generate_idxs")
     return iidx, jidx, blk, cnt, comm
+
+
+def problem_bytes(cnp, n_cams, n_pts, n_obs, pair_items, precision=PREC_FP64):
+    """Device bytes one problem takes inside a batch (an upper bound)."""
+    return lib().psba_problem_bytes(cnp, precision, n_cams, n_pts, n_obs, int(pair_items))
+
+
+def pair_items(iidx):
+    """sum over points of t(t+1)/2, t = track length (iidx point-major)."""
+    t = np.bincount(np.asarray(iidx, dtype=np.int64))
+    return int((t * (t + 1) // 2).sum())
+
+
+class Batch:
+    """Many independent problems solved together (include/psba_batch.h)."""
+
+    def __init__(self, cnp, precision=PREC_FP64, device=0, max_problems=0):
+        L = lib()
+        self._h = ct.c_void_p()
+        d = BatchDesc(cnp, precision, device, max_problems)
+        _check(L.psba_batch_create(ct.byref(d), ct.byref(self._h)), "psba_batch_create")
+        self.cnp = cnp
+        self.sizes = []
+
+    def close(self):
+        if self._h:
+            lib().psba_batch_destroy(self._h)
+            self._h = ct.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def add(self, K, impts, initrot, cams, pts, iidx, jidx, fixmask=None):
+        a = [np.ascontiguousarray(x, dtype=np.float64) for x in (K, impts, initrot, cams, pts)]
+        i = np.ascontiguousarray(iidx, dtype=np.int32)
+        j = np.ascontiguousarray(jidx, dtype=np.int32)
+        nc, npt, no = a[3].size // self.cnp, a[4].size // 3, i.size
+        fm = None
+        if fixmask is not None:
+            fm = np.ascontiguousarray(fixmask, dtype=np.uint8)
+        slot = ct.c_int(-1)
+        _check(lib().psba_batch_add(self._h, nc, npt, no, *[_dptr(x) for x in a], _iptr(i), _iptr(j),
+                                    None if fm is None else fm.ctypes.data_as(ct.POINTER(ct.c_ubyte)),
+                                    ct.byref(slot)), "psba_batch_add")
+        self.sizes.append((nc, npt, no))
+        return slot.value
+
+    def add_scene(self, sc, fix=True):
+        return self.add(sc.K, sc.meas, sc.q0, sc.cams, sc.pts, sc.iidx, sc.jidx, sc.fixmask if fix else None)
+
+    def reset(self):
+        _check(lib().psba_batch_reset(self._h), "psba_batch_reset")
+        self.sizes = []
+
+    def commit(self):
+        _check(lib().psba_batch_commit(self._h), "psba_batch_commit")
+
+    def nbytes(self):
+        return lib().psba_batch_bytes(self._h)
+
+    @staticmethod
+    def _opts(max_iter, rel_tol):
+        return Opts(max_iter, 0, 0, 0, rel_tol)
+
+    def launch(self, max_iter=100, rel_tol=1e-8):
+        _check(lib().psba_batch_launch(self._h, ct.byref(self._opts(max_iter, rel_tol))), "psba_batch_launch")
+
+    def poll(self):
+        return _check(lib().psba_batch_poll(self._h), "psba_batch_poll") == 1
+
+    def solve(self, max_iter=100, rel_tol=1e-8):
+        r = ct.c_int(0)
+        _check(lib().psba_batch_solve(self._h, ct.byref(self._opts(max_iter, rel_tol)), ct.byref(r)),
+               "psba_batch_solve")
+        return r.value
+
+    def result(self, slot, params=True):
+        if not 0 <= slot < len(self.sizes):
+            params = False  # the library reports the bad slot
+        nc, npt, _ = self.sizes[slot] if params else (0, 0, 0)
+        code, it = ct.c_int(0), ct.c_int(0)
+        ie, fe = ct.c_double(0), ct.c_double(0)
+        cams = np.empty(self.cnp * nc) if params else None
+        pts = np.empty(3 * npt) if params else None
+        _check(lib().psba_batch_result(self._h, slot, ct.byref(code), ct.byref(it), ct.byref(ie), ct.byref(fe),
+                                       _dptr(cams) if params else None, _dptr(pts) if params else None),
+               "psba_batch_result")
+        out = dict(code=code.value, itno=it.value, init_err=ie.value, final_err=fe.value)
+        if params:
+            out["cams"], out["pts"] = cams, pts
+        return out
+
+    def elapsed(self):
+        ms, r = ct.c_double(0), ct.c_int(0)
+        _check(lib().psba_batch_elapsed(self._h, ct.byref(ms), ct.byref(r)), "psba_batch_elapsed")
+        return ms.value, r.value
+
+    def autotune(self):
+        ob, pb = ct.c_int(0), ct.c_int(0)
+        _check(lib().psba_batch_autotune(self._h, ct.byref(ob), ct.byref(pb)), "psba_batch_autotune")
+        return ob.value, pb.value

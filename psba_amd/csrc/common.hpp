@@ -112,4 +112,15 @@ __device__ inline void grid_sum_finish(double (&v)[K], double* partials, unsigne
 
 inline unsigned ceil_div(long a, long b) { return (unsigned)((a + b - 1) / b); }
 
+// call FN<cnp>(args...) for the supported camera models, else return
+// PSBA_ERR_UNSUPPORTED from the enclosing function
+#define PSBA_DISPATCH_CNP(cnp, FN, ...)   \
+  switch (cnp) {                          \
+    case 6: FN<6>(__VA_ARGS__); break;    \
+    case 9: FN<9>(__VA_ARGS__); break;    \
+    case 11: FN<11>(__VA_ARGS__); break;  \
+    case 16: FN<16>(__VA_ARGS__); break;  \
+    default: return PSBA_ERR_UNSUPPORTED; \
+  }
+
 }  // namespace psba

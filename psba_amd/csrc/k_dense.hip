@@ -830,3 +830,130 @@ int launch_factor_solve(psba_ctx* c) {
 }
 
 }  // namespace psba
+
+// ---------------------------------------------------------------------------
+// batched problems: one workgroup factors and solves one problem's reduced
+// system with the same tile tasks as k_chol_solve, run in sequence (the
+// diagonal task of each phase last, so its factor stays in LDS for the next
+// phase).  Arithmetic per tile is the same as in the single-problem kernel.
+// ---------------------------------------------------------------------------
+#include "batch.hpp"
+
+namespace psba {
+
+__global__ __launch_bounds__(256) void k_b_chol_solve(double* __restrict__ S, double* __restrict__ Lf,
+                                                      const long* __restrict__ S_off, const int* __restrict__ npad,
+                                                      const int* __restrict__ vec_off, const int* __restrict__ cam_off,
+                                                      int cnp, const double* __restrict__ ea,
+                                                      double* __restrict__ bw, double* __restrict__ yv,
+                                                      double* __restrict__ xs, double* __restrict__ dp,
+                                                      double* __restrict__ fok, const unsigned char* __restrict__ run) {
+  __shared__ CholSmem sm;
+  const int p = blockIdx.x;
+  if (!run[p]) return;
+  const int tid = threadIdx.x;
+  const int ld = npad[p], T = ld / NB, n = (cam_off[p + 1] - cam_off[p]) * cnp;
+  double* A = S + S_off[p];
+  double* L = Lf + S_off[p];
+  const double* b = ea + (size_t)cam_off[p] * cnp;
+  double* bwp = bw + vec_off[p];
+  double* yvp = yv + vec_off[p];
+  double* xsp = xs + vec_off[p];
+  double* dpp = dp + (size_t)cam_off[p] * cnp;
+  auto tile = [&](double* base, int i, int j) { return base + (size_t)i * NB * ld + (size_t)j * NB; };
+  int bad = 0;
+  for (int q = tid; q < T * NB; q += blockDim.x) bwp[q] = (q < n) ? b[q] : 0.0;
+  for (int f = tid; f < NB * NB; f += blockDim.x) {
+    const int r = f >> 6, c = f & 63;
+    sm.Y[r * TP + c] = A[(size_t)r * ld + c];
+  }
+  if (tid < NB) sm.part[0][tid] = (tid < n) ? b[tid] : 0.0;
+  __syncthreads();
+  bad |= tile_potrf(sm);
+  store_tile(sm.Y, L, ld);
+  tile_forward(sm.Y, sm.part[0], yvp, sm.w);
+  __syncthreads();
+
+  for (int k = 0; k + 1 < T; ++k) {
+    for (int f = tid; f < NB * NB; f += blockDim.x) {
+      const int r = f >> 6, c = f & 63;
+      sm.P[r * TP + c] = sm.Y[r * TP + c];
+    }
+    if (tid < NB) sm.v[tid] = sm.w[tid];
+    __syncthreads();
+    if (tid < NB) sm.rd[tid] = 1.0 / sm.P[tid * TP + tid];
+    __syncthreads();
+    const int m = T - 1 - k, ntasks = m * (m + 1) / 2;
+    // tasks in column order, the diagonal (k+1, k+1) moved to the end
+    for (int tt = 1; tt <= ntasks; ++tt) {
+      const int t = (tt == ntasks) ? 0 : tt;
+      int a = 0, cum = 0;
+      while (cum + (m - a) <= t) { cum += m - a; ++a; }
+      const int j = k + 1 + a, i = j + (t - cum);
+      dvec4 acc[2][2];
+      acc_load(acc, tile(A, i, j), ld);
+      trsm_rows(sm.X, tile(A, i, k), ld, sm.P, sm.rd);
+      if (i != j) trsm_rows(sm.Y, tile(A, j, k), ld, sm.P, sm.rd);
+      __syncthreads();
+      acc_update(acc, sm.X, (i != j) ? sm.Y : sm.X, i == j);
+      if (j == k + 1) {
+        store_tile(sm.X, tile(L, i, k), ld);
+        const int r = tid >> 2, pp = tid & 3;
+        double s = 0.0;
+#pragma unroll
+        for (int c = 0; c < 16; ++c) s += sm.X[r * TP + 16 * pp + c] * sm.v[16 * pp + c];
+        s += __shfl_xor(s, 1, 64);
+        s += __shfl_xor(s, 2, 64);
+        if (pp == 0) {
+          const double bi = bwp[(size_t)i * NB + r] - s;
+          bwp[(size_t)i * NB + r] = bi;
+          if (i == j) sm.part[0][r] = bi;
+        }
+      }
+      if (i == j && j == k + 1) {
+        __syncthreads();
+        acc_store(acc, sm.Y, TP);
+        __syncthreads();
+        bad |= tile_potrf(sm);
+        store_tile(sm.Y, tile(L, i, i), ld);
+        tile_forward(sm.Y, sm.part[0], yvp + (size_t)i * NB, sm.w);
+      } else {
+        acc_store(acc, tile(A, i, j), ld);
+      }
+      __syncthreads();
+    }
+  }
+
+  // back substitution, right-looking as in k_chol_solve: step k applies
+  // x_{k+1} (in sm.w) to r_k, solves x_k, then applies x_{k+1} to r_i, i < k
+  for (int k = T - 1; k >= 0; --k) {
+    load_lower(sm.P, tile(L, k, k), ld);
+    double br = yvp[(size_t)k * NB + (tid & 63)];
+    if (k + 1 < T) br -= tile_tmatvec(sm, tile(L, k + 1, k), ld, sm.w);
+    __syncthreads();
+    tile_backward(sm.P, br, xsp + (size_t)k * NB, sm.v);
+    __syncthreads();
+    if (tid < NB) {
+      const int q = k * NB + tid;
+      if (q < n) dpp[q] = sm.v[tid];
+    }
+    if (k + 1 < T)
+      for (int i = 0; i < k; ++i) {
+        const double t = tile_tmatvec(sm, tile(L, k + 1, i), ld, sm.w);
+        if (tid < NB) yvp[(size_t)i * NB + tid] -= t;
+      }
+    __syncthreads();
+    if (tid < NB) sm.w[tid] = sm.v[tid];
+    __syncthreads();
+  }
+  if (tid == 0) fok[p] = bad ? 0.0 : 1.0;
+}
+
+int launch_b_factor_solve(psba_batch* b) {
+  hipLaunchKernelGGL(k_b_chol_solve, dim3(b->P), dim3(256), 0, b->stream, b->S, b->Lf, b->S_off, b->npad, b->vec_off,
+                     b->cam_off, b->cnp, b->ea, b->bw, b->yv, b->xs, b->dp, b->fok, b->active);
+  PSBA_LAUNCH_TRY();
+  return PSBA_OK;
+}
+
+}  // namespace psba

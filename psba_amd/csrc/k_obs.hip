@@ -13,15 +13,6 @@
 
 namespace psba {
 
-#define PSBA_DISPATCH_CNP(cnp, FN, ...)        \
-  switch (cnp) {                               \
-    case 6: FN<6>(__VA_ARGS__); break;         \
-    case 9: FN<9>(__VA_ARGS__); break;         \
-    case 11: FN<11>(__VA_ARGS__); break;       \
-    case 16: FN<16>(__VA_ARGS__); break;       \
-    default: return PSBA_ERR_UNSUPPORTED;      \
-  }
-
 // ---------------------------------------------------------------------------
 // residual and cost
 // ---------------------------------------------------------------------------
@@ -81,9 +72,11 @@ __global__ __launch_bounds__(256) void k_jacobian(const double* __restrict__ K, 
                                                   const double* __restrict__ meas, const int* __restrict__ obs_cam,
                                                   const int* __restrict__ obs_pt, const unsigned char* __restrict__ fix,
                                                   int no, double coeff, R* __restrict__ e, R* __restrict__ JA,
-                                                  R* __restrict__ JB, R* __restrict__ W) {
+                                                  R* __restrict__ JB, R* __restrict__ W,
+                                                  const int* __restrict__ ownr, const unsigned char* __restrict__ run) {
   int o = blockIdx.x * blockDim.x + threadIdx.x;
   if (o >= no) return;
+  if (run && !run[ownr[o]]) return;  // batched problems: only those that need it
   int j = obs_cam[o], i = obs_pt[o];
   double cam[CNP], M[3], uv[2], ja[2 * CNP], jb[6];
 #pragma unroll
@@ -120,9 +113,12 @@ template <typename R>
 __global__ __launch_bounds__(256) void k_point_normal(const int* __restrict__ pt_ptr, const R* __restrict__ JB,
                                                       const R* __restrict__ e, int np, double coeff,
                                                       double coeff_g, double* __restrict__ V,
-                                                      double* __restrict__ gb, double* __restrict__ uvd) {
+                                                      double* __restrict__ gb, double* __restrict__ uvd,
+                                                      const int* __restrict__ ownr,
+                                                      const unsigned char* __restrict__ run) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= np) return;
+  if (run && !run[ownr[i]]) return;
   R v[6] = {0, 0, 0, 0, 0, 0}, g[3] = {0, 0, 0};
   for (int o = pt_ptr[i]; o < pt_ptr[i + 1]; ++o) {
     const R* b = JB + (size_t)o * 6;
@@ -153,11 +149,14 @@ __global__ __launch_bounds__(256) void k_point_normal(const int* __restrict__ pt
 template <int CNP, typename R>
 __global__ __launch_bounds__(256) void k_cam_chunks(const int* __restrict__ cam_obs, const int* __restrict__ cb,
                                                     const int* __restrict__ ce, const R* __restrict__ JA,
-                                                    const R* __restrict__ e, R* __restrict__ part) {
+                                                    const R* __restrict__ e, R* __restrict__ part,
+                                                    const int* __restrict__ ownr,
+                                                    const unsigned char* __restrict__ run) {
   constexpr int NU = CNP * (CNP + 1) / 2, NE = NU + CNP, REC = 2 * CNP + 2;
   constexpr int CH = (CNP > 11) ? 128 : kCamChunk;
   __shared__ R sJ[CH * REC];
   const int b = blockIdx.x;
+  if (run && !run[ownr[b]]) return;
   const int beg = cb[b], cnt = ce[b] - beg;
   for (int f = threadIdx.x; f < cnt * REC; f += blockDim.x) {
     int s = f / REC, el = f - s * REC;
@@ -193,9 +192,12 @@ __global__ __launch_bounds__(256) void k_cam_chunks(const int* __restrict__ cam_
 template <int CNP, typename R>
 __global__ __launch_bounds__(256) void k_cam_finish(const int* __restrict__ ccp, const R* __restrict__ part,
                                                     int nc, double coeff, double coeff_g, double* __restrict__ U,
-                                                    double* __restrict__ ga, double* __restrict__ uvd) {
+                                                    double* __restrict__ ga, double* __restrict__ uvd,
+                                                    const int* __restrict__ ownr,
+                                                    const unsigned char* __restrict__ run) {
   constexpr int NU = CNP * (CNP + 1) / 2, NE = NU + CNP;
   const int j = blockIdx.x;
+  if (run && !run[ownr[j]]) return;
   for (int t = threadIdx.x; t < NE; t += blockDim.x) {
     double acc = 0.0;
     for (int q = ccp[j]; q < ccp[j + 1]; ++q) acc += (double)part[(size_t)q * NE + t];
@@ -220,15 +222,18 @@ template <int CNP, typename R>
 static void run_jac_normal_t(psba_ctx* c, double coeff, double coeff_g, R* e, R* JA, R* JB, R* W, R* ucpart) {
   probe_begin(c, PSBA_PROBE_JACOBIAN);
   hipLaunchKernelGGL((k_jacobian<CNP, R>), dim3(ceil_div(c->no, 256)), dim3(256), 0, c->stream, c->K, c->q0,
-                     c->cams, c->pts, c->meas, c->obs_cam, c->obs_pt, c->fix, c->no, coeff, e, JA, JB, W);
+                     c->cams, c->pts, c->meas, c->obs_cam, c->obs_pt, c->fix, c->no, coeff, e, JA, JB, W,
+                     (const int*)nullptr, (const unsigned char*)nullptr);
   probe_end(c, PSBA_PROBE_JACOBIAN);
   hipLaunchKernelGGL(k_point_normal<R>, dim3(ceil_div(c->np, 256)), dim3(256), 0, c->stream, c->pt_ptr, JB, e,
-                     c->np, coeff, coeff_g, c->V, c->g + c->n, c->uvdiag + c->n);
+                     c->np, coeff, coeff_g, c->V, c->g + c->n, c->uvdiag + c->n, (const int*)nullptr,
+                     (const unsigned char*)nullptr);
   if (c->n_cchunk > 0)
     hipLaunchKernelGGL((k_cam_chunks<CNP, R>), dim3(c->n_cchunk), dim3(256), 0, c->stream, c->cam_obs,
-                       c->cchunk_begin, c->cchunk_end, JA, e, ucpart);
+                       c->cchunk_begin, c->cchunk_end, JA, e, ucpart, (const int*)nullptr,
+                       (const unsigned char*)nullptr);
   hipLaunchKernelGGL((k_cam_finish<CNP, R>), dim3(c->nc), dim3(256), 0, c->stream, c->cam_cchunk_ptr, ucpart, c->nc,
-                     coeff, coeff_g, c->U, c->g, c->uvdiag);
+                     coeff, coeff_g, c->U, c->g, c->uvdiag, (const int*)nullptr, (const unsigned char*)nullptr);
 }
 
 template <int CNP>
@@ -272,10 +277,19 @@ int launch_maxdiag(psba_ctx* c) {
 // ---------------------------------------------------------------------------
 // Regular branch: adjugate over T = -det (PSBA/CL_files/compute_Vinv.cl:29,76-86).
 // |T| < 1e-16: the reference's pivoted-elimination divisor (compute_Vinv.cl:31-73).
+// Batched problems pass mu_p (damping per problem) with ownr (point -> problem)
+// and run (problem -> still iterating); the single-problem path passes mu.
 __global__ __launch_bounds__(256) void k_vinv(const double* __restrict__ V, const double* __restrict__ gb, int np,
-                                              double mu, double* __restrict__ Vinv, double* __restrict__ z) {
+                                              double mu, double* __restrict__ Vinv, double* __restrict__ z,
+                                              const double* __restrict__ mu_p, const int* __restrict__ ownr,
+                                              const unsigned char* __restrict__ run) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= np) return;
+  if (mu_p) {
+    const int p = ownr[i];
+    if (!run[p]) return;
+    mu = mu_p[p];
+  }
   const double* v = V + (size_t)i * 6;
   double a11 = v[0] + mu, a12 = v[1], a13 = v[2], a22 = v[3] + mu, a23 = v[4], a33 = v[5] + mu;
   double T = a33 * a12 * a12 - 2 * a12 * a13 * a23 + a22 * a13 * a13 + a11 * a23 * a23 - a11 * a22 * a33;
@@ -321,7 +335,7 @@ __global__ __launch_bounds__(256) void k_vinv(const double* __restrict__ V, cons
 
 int launch_vinv(psba_ctx* c, double mu) {
   hipLaunchKernelGGL(k_vinv, dim3(ceil_div(c->np, 256)), dim3(256), 0, c->stream, c->V, c->g + c->n, c->np, mu,
-                     c->Vinv, c->z);
+                     c->Vinv, c->z, (const double*)nullptr, (const int*)nullptr, (const unsigned char*)nullptr);
   PSBA_LAUNCH_TRY();
   return PSBA_OK;
 }
@@ -333,9 +347,11 @@ template <int CNP, typename R>
 __global__ __launch_bounds__(256) void k_backsub(const int* __restrict__ pt_ptr, const int* __restrict__ obs_cam,
                                                  const R* __restrict__ W, const double* __restrict__ Vinv,
                                                  const double* __restrict__ gb, int np, int n,
-                                                 double* __restrict__ dp, double* __restrict__ eb) {
+                                                 double* __restrict__ dp, double* __restrict__ eb,
+                                                 const int* __restrict__ ownr, const unsigned char* __restrict__ run) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= np) return;
+  if (run && !run[ownr[i]]) return;
   R s[3] = {0, 0, 0};
   for (int o = pt_ptr[i]; o < pt_ptr[i + 1]; ++o) {
     const R* w = W + (size_t)o * 3 * CNP;
@@ -362,10 +378,12 @@ template <int CNP>
 static void run_backsub(psba_ctx* c) {
   if (c->mixed())
     hipLaunchKernelGGL((k_backsub<CNP, float>), dim3(ceil_div(c->np, 256)), dim3(256), 0, c->stream, c->pt_ptr,
-                       c->obs_cam, c->W32, c->Vinv, c->g + c->n, c->np, c->n, c->dp, c->eb);
+                       c->obs_cam, c->W32, c->Vinv, c->g + c->n, c->np, c->n, c->dp, c->eb, (const int*)nullptr,
+                       (const unsigned char*)nullptr);
   else
     hipLaunchKernelGGL((k_backsub<CNP, double>), dim3(ceil_div(c->np, 256)), dim3(256), 0, c->stream, c->pt_ptr,
-                       c->obs_cam, c->W, c->Vinv, c->g + c->n, c->np, c->n, c->dp, c->eb);
+                       c->obs_cam, c->W, c->Vinv, c->g + c->n, c->np, c->n, c->dp, c->eb, (const int*)nullptr,
+                       (const unsigned char*)nullptr);
 }
 
 int launch_backsub(psba_ctx* c) {
@@ -505,6 +523,71 @@ __global__ __launch_bounds__(256) void k_axpby(double* __restrict__ out, double 
 
 int launch_axpby(psba_ctx* c, double* out, double alpha, const double* a, double beta, const double* b, long n) {
   hipLaunchKernelGGL(k_axpby, dim3(ceil_div(n, 256)), dim3(256), 0, c->stream, out, alpha, a, beta, b, n);
+  PSBA_LAUNCH_TRY();
+  return PSBA_OK;
+}
+
+}  // namespace psba
+
+// ---------------------------------------------------------------------------
+// batched problems: the same stage kernels over the concatenated arrays,
+// masked per problem
+// ---------------------------------------------------------------------------
+#include "batch.hpp"
+
+namespace psba {
+
+template <int CNP, typename R>
+static void run_b_jac_t(psba_batch* b) {
+  R *e = (R*)b->e, *JA = (R*)b->JA, *JB = (R*)b->JB, *W = (R*)b->W, *ucpart = (R*)b->ucpart;
+  const int bs = b->obs_block;
+  hipLaunchKernelGGL((k_jacobian<CNP, R>), dim3(ceil_div(b->NO, bs)), dim3(bs), 0, b->stream, b->K, b->q0, b->cams,
+                     b->pts, b->meas, b->obs_cam, b->obs_pt, b->fix, (int)b->NO, 1.0, e, JA, JB, W, b->obs_prob,
+                     b->need_jac);
+  hipLaunchKernelGGL(k_point_normal<R>, dim3(ceil_div(b->NP, b->pt_block)), dim3(b->pt_block), 0, b->stream, b->pt_ptr, JB, e,
+                     (int)b->NP, 1.0, 1.0, b->V, b->g + b->N, b->uvdiag + b->N, b->pt_prob, b->need_jac);
+  if (b->n_cchunk > 0)
+    hipLaunchKernelGGL((k_cam_chunks<CNP, R>), dim3(b->n_cchunk), dim3(256), 0, b->stream, b->cam_obs, b->cchunk_b,
+                       b->cchunk_e, JA, e, ucpart, b->cchunk_prob, b->need_jac);
+  hipLaunchKernelGGL((k_cam_finish<CNP, R>), dim3(b->NC), dim3(256), 0, b->stream, b->ccp, ucpart, (int)b->NC, 1.0,
+                     1.0, b->U, b->g, b->uvdiag, b->cam_prob, b->need_jac);
+}
+
+template <int CNP>
+static void run_b_jac(psba_batch* b) {
+  if (b->mixed())
+    run_b_jac_t<CNP, float>(b);
+  else
+    run_b_jac_t<CNP, double>(b);
+}
+
+int launch_b_jacobian(psba_batch* b) {
+  PSBA_DISPATCH_CNP(b->cnp, run_b_jac, b);
+  PSBA_LAUNCH_TRY();
+  return PSBA_OK;
+}
+
+int launch_b_vinv(psba_batch* b) {
+  hipLaunchKernelGGL(k_vinv, dim3(ceil_div(b->NP, b->pt_block)), dim3(b->pt_block), 0, b->stream, b->V, b->g + b->N, (int)b->NP, 0.0,
+                     b->Vinv, b->z, b->mu, b->pt_prob, b->active);
+  PSBA_LAUNCH_TRY();
+  return PSBA_OK;
+}
+
+template <int CNP>
+static void run_b_backsub(psba_batch* b) {
+  if (b->mixed())
+    hipLaunchKernelGGL((k_backsub<CNP, float>), dim3(ceil_div(b->NP, b->pt_block)), dim3(b->pt_block), 0, b->stream, b->pt_ptr,
+                       b->obs_cam, (const float*)b->W, b->Vinv, b->g + b->N, (int)b->NP, (int)b->N, b->dp, b->eb,
+                       b->pt_prob, b->active);
+  else
+    hipLaunchKernelGGL((k_backsub<CNP, double>), dim3(ceil_div(b->NP, b->pt_block)), dim3(b->pt_block), 0, b->stream, b->pt_ptr,
+                       b->obs_cam, (const double*)b->W, b->Vinv, b->g + b->N, (int)b->NP, (int)b->N, b->dp, b->eb,
+                       b->pt_prob, b->active);
+}
+
+int launch_b_backsub(psba_batch* b) {
+  PSBA_DISPATCH_CNP(b->cnp, run_b_backsub, b);
   PSBA_LAUNCH_TRY();
   return PSBA_OK;
 }

@@ -16,13 +16,15 @@ template <int CNP, typename R>
 __global__ __launch_bounds__(256) void k_schur_chunks(const int2* __restrict__ items, const int* __restrict__ sb,
                                                       const int* __restrict__ se, const int* __restrict__ obs_pt,
                                                       const R* __restrict__ W, const double* __restrict__ Vinv,
-                                                      R* __restrict__ part) {
+                                                      R* __restrict__ part, const int* __restrict__ ownr,
+                                                      const unsigned char* __restrict__ run) {
   constexpr int C2 = CNP * CNP, REC = 3 * CNP;
   constexpr int G = (256 / C2) > 0 ? (256 / C2) : 1;
   __shared__ R sY[kPairChunk * REC];
   __shared__ R sW[kPairChunk * REC];
   __shared__ R sG[G * C2];
   const int b = blockIdx.x;
+  if (run && !run[ownr[b]]) return;
   const int beg = sb[b], cnt = se[b] - beg;
   // Y rows: task = (item, row r) -> Y[it][r][0..2] = W_a[r][:] V^-1
   for (int task = threadIdx.x; task < cnt * CNP; task += blockDim.x) {
@@ -106,10 +108,12 @@ template <int CNP, typename R>
 __global__ __launch_bounds__(256) void k_ea_chunks(const int* __restrict__ cam_obs, const int* __restrict__ cb,
                                                    const int* __restrict__ ce, const int* __restrict__ obs_pt,
                                                    const R* __restrict__ W, const double* __restrict__ z,
-                                                   R* __restrict__ part) {
+                                                   R* __restrict__ part, const int* __restrict__ ownr,
+                                                   const unsigned char* __restrict__ run) {
   constexpr int CH = (CNP > 11) ? 128 : kCamChunk;
   __shared__ R sP[CH * CNP];
   const int b = blockIdx.x;
+  if (run && !run[ownr[b]]) return;
   const int beg = cb[b], cnt = ce[b] - beg;
   for (int task = threadIdx.x; task < cnt * CNP; task += blockDim.x) {
     int s = task / CNP, r = task - s * CNP;
@@ -128,10 +132,12 @@ __global__ __launch_bounds__(256) void k_ea_chunks(const int* __restrict__ cam_o
 
 template <int CNP, typename R>
 __global__ __launch_bounds__(64) void k_ea_finish(const int* __restrict__ ccp, const R* __restrict__ part,
-                                                  const double* __restrict__ ga, int nc, double* __restrict__ ea) {
+                                                  const double* __restrict__ ga, int nc, double* __restrict__ ea,
+                                                  const int* __restrict__ ownr, const unsigned char* __restrict__ run) {
   int t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= nc * CNP) return;
   int j = t / CNP, r = t - j * CNP;
+  if (run && !run[ownr[j]]) return;
   double acc = 0.0;
   for (int q = ccp[j]; q < ccp[j + 1]; ++q) acc += (double)part[(size_t)q * CNP + r];
   ea[t] = ga[t] - acc;
@@ -142,7 +148,8 @@ static void run_schur_t(psba_ctx* c, double mu, const R* W, R* schpart, R* eapar
   if (c->n_schunk > 0) {
     probe_begin(c, PSBA_PROBE_SCHUR);
     hipLaunchKernelGGL((k_schur_chunks<CNP, R>), dim3(c->n_schunk), dim3(256), 0, c->stream, c->pair_items,
-                       c->schunk_begin, c->schunk_end, c->obs_pt, W, c->Vinv, schpart);
+                       c->schunk_begin, c->schunk_end, c->obs_pt, W, c->Vinv, schpart, (const int*)nullptr,
+                       (const unsigned char*)nullptr);
     probe_end(c, PSBA_PROBE_SCHUR);
   }
   hipLaunchKernelGGL((k_schur_finish<CNP, R>), dim3(c->n_lblocks + (c->n_pad - c->n)),
@@ -150,9 +157,10 @@ static void run_schur_t(psba_ctx* c, double mu, const R* W, R* schpart, R* eapar
                      c->fix, mu, c->n_pad, c->n_lblocks, c->n, c->S);
   if (c->n_cchunk > 0)
     hipLaunchKernelGGL((k_ea_chunks<CNP, R>), dim3(c->n_cchunk), dim3(256), 0, c->stream, c->cam_obs,
-                       c->cchunk_begin, c->cchunk_end, c->obs_pt, W, c->z, eapart);
+                       c->cchunk_begin, c->cchunk_end, c->obs_pt, W, c->z, eapart, (const int*)nullptr,
+                       (const unsigned char*)nullptr);
   hipLaunchKernelGGL((k_ea_finish<CNP, R>), dim3(ceil_div((long)c->nc * CNP, 64)), dim3(64), 0, c->stream,
-                     c->cam_cchunk_ptr, eapart, c->g, c->nc, c->ea);
+                     c->cam_cchunk_ptr, eapart, c->g, c->nc, c->ea, (const int*)nullptr, (const unsigned char*)nullptr);
 }
 
 template <int CNP>
@@ -171,6 +179,99 @@ int launch_schur(psba_ctx* c, double mu) {
     case 16: run_schur<16>(c, mu); break;
     default: return PSBA_ERR_UNSUPPORTED;
   }
+  PSBA_LAUNCH_TRY();
+  return PSBA_OK;
+}
+
+}  // namespace psba
+
+// ---------------------------------------------------------------------------
+// batched problems
+// ---------------------------------------------------------------------------
+#include "batch.hpp"
+
+namespace psba {
+
+// one block per lower block (problem p, local cameras k >= l)
+template <int CNP, typename R>
+__global__ __launch_bounds__(256) void k_b_schur_finish(const int* __restrict__ lb_prob, const int* __restrict__ lb_k,
+                                                        const int* __restrict__ lb_l, const int* __restrict__ lb_pair,
+                                                        const int* __restrict__ psp, const R* __restrict__ part,
+                                                        const double* __restrict__ U,
+                                                        const unsigned char* __restrict__ fix,
+                                                        const double* __restrict__ mu_p, const int* __restrict__ cam_off,
+                                                        const int* __restrict__ npad, const long* __restrict__ S_off,
+                                                        const unsigned char* __restrict__ run, double* __restrict__ S) {
+  constexpr int C2 = CNP * CNP;
+  const int b = blockIdx.x;
+  const int p = lb_prob[b];
+  if (!run[p]) return;
+  const int k = lb_k[b], l = lb_l[b], q = lb_pair[b];
+  const int kg = cam_off[p] + k, ld = npad[p];
+  const double mu = mu_p[p];
+  double* Sp = S + S_off[p];
+  for (int t = threadIdx.x; t < C2; t += blockDim.x) {
+    const int r = t / CNP, cc = t - r * CNP;
+    double acc = 0.0;
+    if (q >= 0)
+      for (int h = psp[q]; h < psp[q + 1]; ++h) acc += (double)part[(size_t)h * C2 + t];
+    double val;
+    if (k == l) {
+      double u = U[(size_t)kg * C2 + t];
+      if (r == cc) u = u + mu;
+      val = u - acc;
+      if (r == cc && fix[(size_t)kg * CNP + r]) val = 1.0;
+    } else {
+      val = -acc;
+    }
+    Sp[(size_t)(k * CNP + r) * ld + l * CNP + cc] = val;
+  }
+}
+
+// padding rows n..n_pad-1 of every problem's S: identity
+__global__ __launch_bounds__(256) void k_b_pad(const int* __restrict__ cam_off, const int* __restrict__ npad,
+                                               const long* __restrict__ S_off, int cnp,
+                                               const unsigned char* __restrict__ run, double* __restrict__ S) {
+  const int p = blockIdx.x;
+  if (!run[p]) return;
+  const int n = (cam_off[p + 1] - cam_off[p]) * cnp, ld = npad[p];
+  double* Sp = S + S_off[p];
+  for (long f = threadIdx.x; f < (long)(ld - n) * ld; f += blockDim.x) {
+    const int r = n + (int)(f / ld), q = (int)(f % ld);
+    if (q <= r) Sp[(size_t)r * ld + q] = (q == r) ? 1.0 : 0.0;
+  }
+}
+
+template <int CNP, typename R>
+static void run_b_schur_t(psba_batch* b) {
+  const R* W = (const R*)b->W;
+  R *schpart = (R*)b->schpart, *eapart = (R*)b->eapart;
+  if (b->n_schunk > 0)
+    hipLaunchKernelGGL((k_schur_chunks<CNP, R>), dim3(b->n_schunk), dim3(256), 0, b->stream, b->items, b->schunk_b,
+                       b->schunk_e, b->obs_pt, W, b->Vinv, schpart, b->schunk_prob, b->active);
+  if (b->nlb > 0)
+    hipLaunchKernelGGL((k_b_schur_finish<CNP, R>), dim3(b->nlb), dim3(CNP * CNP > 128 ? 256 : 128), 0, b->stream,
+                       b->lb_prob, b->lb_k, b->lb_l, b->lb_pair, b->psp, schpart, b->U, b->fix, b->mu, b->cam_off,
+                       b->npad, b->S_off, b->active, b->S);
+  hipLaunchKernelGGL(k_b_pad, dim3(b->P), dim3(256), 0, b->stream, b->cam_off, b->npad, b->S_off, CNP, b->active,
+                     b->S);
+  if (b->n_cchunk > 0)
+    hipLaunchKernelGGL((k_ea_chunks<CNP, R>), dim3(b->n_cchunk), dim3(256), 0, b->stream, b->cam_obs, b->cchunk_b,
+                       b->cchunk_e, b->obs_pt, W, b->z, eapart, b->cchunk_prob, b->active);
+  hipLaunchKernelGGL((k_ea_finish<CNP, R>), dim3(ceil_div(b->NC * CNP, 64)), dim3(64), 0, b->stream, b->ccp, eapart,
+                     b->g, (int)b->NC, b->ea, b->cam_prob, b->active);
+}
+
+template <int CNP>
+static void run_b_schur(psba_batch* b) {
+  if (b->mixed())
+    run_b_schur_t<CNP, float>(b);
+  else
+    run_b_schur_t<CNP, double>(b);
+}
+
+int launch_b_schur(psba_batch* b) {
+  PSBA_DISPATCH_CNP(b->cnp, run_b_schur, b);
   PSBA_LAUNCH_TRY();
   return PSBA_OK;
 }

@@ -1,0 +1,160 @@
+"""Batched solves (include/psba_batch.h) through the C-ABI.
+
+* Each problem's result is the same alone (a batch of one) and inside a
+  mixed batch, in any order, bit for bit: every reduction of the batched path
+  has an order fixed by the problem alone, and stopped problems are skipped.
+* Against the single-problem psba_levmar with the same rules (no handoff,
+  relative-decrease stop, 100 iterations) and against the oracle's LM, the
+  stop code and iteration count agree and the final cost agrees within 1e-9
+  relative (the reduction orders differ, the arithmetic does not).
+* Polling, autotuning and re-committing into a used arena change nothing.
+* The mixed-precision batch reaches the FP64 batch's final cost within 1e-6
+  relative.
+"""
+import numpy as np
+import pytest
+
+from conftest import rel
+
+pytestmark = pytest.mark.gpu
+
+SEED = 1003
+OPTS = dict(max_iter=100, rel_tol=1e-8)
+
+
+def window(P, k, **kw):
+    return P.Scene(3, SEED, problem=k, **kw)
+
+
+def solve_batch(P, scenes, precision=0, **opts):
+    b = P.Batch(scenes[0].cnp, precision=precision)
+    for sc in scenes:
+        b.add_scene(sc)
+    b.commit()
+    b.solve(**(opts or OPTS))
+    out = [b.result(s) for s in range(len(scenes))]
+    b.close()
+    return out
+
+
+def same(a, b):
+    assert a["code"] == b["code"]
+    assert a["itno"] == b["itno"]
+    assert a["init_err"] == b["init_err"]
+    assert a["final_err"] == b["final_err"]
+    assert np.array_equal(a["cams"], b["cams"])
+    assert np.array_equal(a["pts"], b["pts"])
+
+
+def test_alone_equals_in_mix(P, gpu):
+    scenes = [window(P, k) for k in range(8)]
+    mix = solve_batch(P, scenes)
+    rev = solve_batch(P, scenes[::-1])[::-1]
+    b = P.Batch(6)
+    for k, sc in enumerate(scenes):
+        b.reset()
+        b.add_scene(sc)
+        b.commit()  # re-commits into the same arena
+        b.solve(**OPTS)
+        alone = b.result(0)
+        same(alone, mix[k])
+        same(alone, rev[k])
+        assert alone["final_err"] < alone["init_err"]
+    b.close()
+
+
+def test_matches_levmar_and_oracle(P, O, gpu):
+    scenes = [window(P, k, n_pts=600) for k in (11, 12, 13)]
+    res = solve_batch(P, scenes)
+    for sc, r in zip(scenes, res):
+        c = P.Context.from_scene(sc)
+        g = c.levmar(max_iter=100, flags=P.FLAG_NO_HANDOFF, rel_tol=1e-8)
+        o = O.Oracle.from_scene(sc).levmar(max_iter=100, flags=O.FLAG_NO_HANDOFF, rel_tol=1e-8)
+        for ref in (g, o):
+            assert r["code"] == ref["code"]
+            assert r["itno"] == ref["itno"]
+            assert abs(r["final_err"] - ref["final_err"]) <= 1e-9 * ref["final_err"]
+        cams, pts = c.read_params()
+        assert rel(r["cams"], cams) < 1e-7
+        assert rel(r["pts"], pts) < 1e-7
+        c.close()
+
+
+@pytest.mark.parametrize("config,kw", [(1, dict(n_pts=400, n_cams=12)), (1, dict(cnp=16, n_pts=400, n_cams=10)),
+                                       (2, dict(n_pts=1500, n_cams=12, outlier_frac=0.0))],
+                         ids=["cnp11", "cnp16", "cnp9"])
+def test_camera_models(P, gpu, config, kw):
+    """Fixed budget (8 iterations, before any stopping test can fire) and
+    convergence by the relative-decrease rule.  The reference's own |dp| test
+    (|dp| < 1e-12 |p|) is left out: at convergence it compares rounding noise,
+    so the iteration it fires on is not a property of the algorithm."""
+    scenes = [P.Scene(config, 77 + k, **kw) for k in range(3)]
+    short = solve_batch(P, scenes, max_iter=8, rel_tol=0.0)
+    full = solve_batch(P, scenes, **OPTS)
+    for sc, r8, r in zip(scenes, short, full):
+        c = P.Context.from_scene(sc)
+        g8 = c.levmar(max_iter=8, flags=P.FLAG_NO_HANDOFF)
+        cams, pts = c.read_params()
+        c.close()
+        assert (r8["itno"], r8["code"]) == (g8["itno"], g8["code"]) == (8, P.ITER_CONTINUE)
+        assert abs(r8["final_err"] - g8["final_err"]) <= 1e-9 * g8["final_err"]
+        assert rel(r8["cams"], cams) < 1e-7 and rel(r8["pts"], pts) < 1e-7
+        c = P.Context.from_scene(sc)
+        g = c.levmar(flags=P.FLAG_NO_HANDOFF, **OPTS)
+        c.close()
+        info = (r["itno"], g["itno"], r["code"], g["code"], r["final_err"], g["final_err"])
+        assert r["code"] == g["code"], info
+        assert abs(r["itno"] - g["itno"]) <= 1, info
+        assert abs(r["final_err"] - g["final_err"]) <= 1e-9 * g["final_err"], info
+
+
+def test_poll_and_autotune(P, gpu):
+    scenes = [window(P, k) for k in range(20, 26)]
+    ref = solve_batch(P, scenes)
+    b = P.Batch(6)
+    for sc in scenes:
+        b.add_scene(sc)
+    b.commit()
+    ob, pb = b.autotune()
+    assert ob in (64, 128, 256) and pb in (64, 128, 256)
+    b.launch(**OPTS)
+    spins = 0
+    while not b.poll():
+        spins += 1
+    ms, rounds = b.elapsed()
+    assert ms > 0 and rounds >= max(r["itno"] for r in ref)
+    for s in range(len(scenes)):
+        same(b.result(s), ref[s])
+    b.close()
+
+
+def test_mixed_precision(P, gpu):
+    scenes = [window(P, k) for k in range(30, 34)]
+    ref = solve_batch(P, scenes)
+    mix = solve_batch(P, scenes, precision=P.PREC_MIXED)
+    for a, m in zip(ref, mix):
+        assert m["code"] in (P.ITER_DP_NO_CHANGE, P.ITER_CONTINUE)
+        assert abs(m["final_err"] - a["final_err"]) <= 1e-6 * a["final_err"]
+
+
+def test_errors(P, gpu):
+    b = P.Batch(6)
+    with pytest.raises(P.PsbaError):
+        b.launch()
+    sc = window(P, 40)
+    bad = sc.iidx.copy()
+    bad[0] = sc.n_pts
+    with pytest.raises(P.PsbaError):
+        b.add(sc.K, sc.meas, sc.q0, sc.cams, sc.pts, bad, sc.jidx)
+    b.add_scene(sc)
+    with pytest.raises(P.PsbaError):
+        b.result(0)
+    b.commit()
+    with pytest.raises(P.PsbaError):
+        b.result(0)
+    b.solve(max_iter=3, rel_tol=0.0)
+    r = b.result(0)
+    assert r["itno"] == 3 and r["code"] == P.ITER_CONTINUE
+    with pytest.raises(P.PsbaError):
+        b.result(1)
+    b.close()
