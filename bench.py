@@ -59,6 +59,117 @@ def cpu_baseline(scene, kw):
                       "%.1f s, final cost %.10g)" % (r["itno"], len(r["trace"]), dt, r["final_err"])}
 
 
+def batch_section(P, seed=1003, n=1024, reps=3):
+    """1,024 independent windows (config 3) per launch: every LM stage is one
+    launch over all problems, rounds are replayed HIP graphs, launch shapes
+    autotuned; mixed precision, with plain FP64 beside it."""
+    scenes = [P.Scene(3, seed, problem=k) for k in range(n)]
+    n_obs = sum(s.n_obs for s in scenes)
+    out = {"problems": n, "n_obs": n_obs}
+    finals = {}
+    for name, prec in (("mixed", P.PREC_MIXED), ("fp64", P.PREC_FP64)):
+        b = P.Batch(6, precision=prec)
+        for sc in scenes:
+            b.add_scene(sc)
+        b.commit()
+        shapes = b.autotune()
+        b.solve()  # warm-up
+        t = []
+        for _ in range(reps):
+            t0 = time.perf_counter()
+            rounds = b.solve()
+            t.append(time.perf_counter() - t0)
+        dev_ms, _ = b.elapsed()
+        res = [b.result(k, params=False) for k in range(n)]
+        its = np.array([r["itno"] for r in res])
+        finals[name] = np.array([r["final_err"] for r in res])
+        wall = float(np.median(t))
+        out[name] = {"solves_per_s": n / wall, "ms_per_batch": 1e3 * wall, "device_ms": dev_ms,
+                     "lm_rounds": rounds, "mean_iterations": float(its.mean()), "max_iterations": int(its.max()),
+                     "jacobian_obs_per_s": float(n_obs * rounds / wall),
+                     "autotuned_blocks": {"obs": shapes[0], "point": shapes[1]}, "bytes": b.nbytes()}
+        b.close()
+        log("batch %s: %d problems, %.1f ms/batch (%.0f solves/s), %d rounds" % (
+            name, n, 1e3 * wall, n / wall, rounds))
+    rms_m = np.sqrt(finals["mixed"] / np.array([2 * s.n_obs for s in scenes]))
+    rms_p = np.sqrt(finals["fp64"] / np.array([2 * s.n_obs for s in scenes]))
+    out["max_rel_rms_mixed_vs_fp64"] = float(np.max(np.abs(rms_m - rms_p) / rms_p))
+    out["speedup_mixed_vs_fp64"] = out["mixed"]["solves_per_s"] / out["fp64"]["solves_per_s"]
+    return out
+
+
+def stream_section(P, seed=1004, rates=(200, 1000, 2000), duration=2.0, serial_n=100):
+    """Open-loop Poisson request stream (config 4 mix: 90% small, 9% medium,
+    1% large) through the request loop, against one-at-a-time plain FP64
+    solves of the same mix."""
+    rng = np.random.default_rng(seed)
+    pool = {0: [], 1: [], 2: []}
+    for c, count in ((0, 300), (1, 30), (2, 6)):
+        for k in range(count):
+            pool[c].append(P.Scene(4, seed, problem=10000 * c + k, size_class=c))
+
+    def draw(m):
+        u = rng.random(m)
+        cls = np.where(u < 0.90, 0, np.where(u < 0.99, 1, 2))
+        return [pool[c][rng.integers(len(pool[c]))] for c in cls], cls
+
+    kw = dict(max_iter=100, flags=P.FLAG_NO_HANDOFF, rel_tol=1e-8)
+    sample, _ = draw(serial_n)
+    serial = []
+    t0 = time.perf_counter()
+    for sc in sample:
+        c = P.Context.from_scene(sc)
+        r = c.levmar(**kw)
+        c.read_params()
+        c.close()
+        serial.append(r["final_err"])
+    serial_rate = serial_n / (time.perf_counter() - t0)
+    out = {"serial_plain_fp64_solves_per_s": serial_rate, "serial_sample": serial_n, "precision": "mixed",
+           "lanes": 4, "hbm_budget_bytes": 200e9, "rates": []}
+    log("stream: serial plain FP64 %.1f solves/s" % serial_rate)
+
+    def run(reqs, arr):
+        srv = P.Server(6, precision=P.PREC_MIXED, lanes=4)
+        ids, done = {}, {}
+        T0 = P.server_clock() + 0.01
+        i, m = 0, len(arr)
+        while len(done) < m:
+            now = P.server_clock() - T0
+            while i < m and arr[i] <= now:
+                ids[srv.submit_scene(reqs[i])] = i
+                i += 1
+            srv.step()
+            for rid in srv.completed():
+                done[ids[rid]] = srv.result(rid, params=False)
+        st = srv.stats()
+        srv.close()
+        lat = np.array([done[k]["t_done"] - (T0 + arr[k]) for k in range(m)])
+        span = max(done[k]["t_done"] for k in range(m)) - T0
+        return done, lat, span, st
+
+    for rate in rates:
+        gaps = rng.exponential(1.0 / rate, int(rate * duration * 1.5) + 10)
+        arr = np.cumsum(gaps)
+        arr = arr[arr < duration]
+        reqs, cls = draw(len(arr))
+        done, lat, span, st = run(reqs, arr)
+        rec = {"offered_per_s": rate, "requests": len(arr), "solves_per_s": len(arr) / span,
+               "p50_ms": 1e3 * float(np.percentile(lat, 50)), "p99_ms": 1e3 * float(np.percentile(lat, 99)),
+               "p99_ms_by_class": [1e3 * float(np.percentile(lat[cls == c], 99)) if np.any(cls == c) else None
+                                   for c in (0, 1, 2)],
+               "batches": st["batches"], "peak_bytes": st["bytes_peak"]}
+        rec["vs_serial"] = rec["solves_per_s"] / serial_rate
+        out["rates"].append(rec)
+        log("stream %d req/s: %.1f solves/s (%.1fx serial), p50 %.1f ms, p99 %.1f ms" % (
+            rate, rec["solves_per_s"], rec["vs_serial"], rec["p50_ms"], rec["p99_ms"]))
+    # accuracy: the serial sample as one burst through the loop
+    done, lat, span, st = run(sample, np.zeros(len(sample)))
+    rel = [abs(done[k]["final_err"] - serial[k]) / serial[k] for k in range(len(sample))]
+    out["max_rel_cost_vs_serial"] = float(max(rel))
+    out["burst_solves_per_s"] = len(sample) / span
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -173,6 +284,8 @@ def main():
     log("mixed: %.3f ms/solve (%.2fx), rms %.6f (rel %.2e); outlier-free: plain %.3f ms, mixed %.3f ms (%.2fx), "
         "rms rel %.2e" % (1e3 * t_mix, (dt / a.steps) / t_mix, rms_mix, abs(rms_mix - rms) / rms, 1e3 * t_p0,
                           1e3 * t_m0, t_p0 / t_m0, abs(rms_m0 - rms_p0) / rms_p0))
+    out["batch"] = batch_section(P)
+    out["stream"] = stream_section(P)
     if not a.no_cpu:
         out["cpu_baseline"] = cpu_baseline(sc, kw)
     print(json.dumps(out), flush=True)

@@ -98,6 +98,18 @@ class BatchDesc(ct.Structure):
     _fields_ = [("cnp", ct.c_int), ("precision", ct.c_int), ("device", ct.c_int), ("max_problems", ct.c_int)]
 
 
+class ServerDesc(ct.Structure):
+    _fields_ = [("cnp", ct.c_int), ("precision", ct.c_int), ("device", ct.c_int), ("lanes", ct.c_int),
+                ("max_batch", ct.c_int), ("hbm_budget", ct.c_longlong), ("max_iter", ct.c_int),
+                ("rel_tol", ct.c_double)]
+
+
+class ServerStats(ct.Structure):
+    _fields_ = [("submitted", ct.c_longlong), ("completed", ct.c_longlong), ("rejected", ct.c_longlong),
+                ("batches", ct.c_longlong), ("bytes_held", ct.c_longlong), ("bytes_peak", ct.c_longlong),
+                ("queued", ct.c_int), ("running", ct.c_int)]
+
+
 _lib = None
 
 
@@ -162,6 +174,16 @@ def lib():
         "psba_batch_result": (ct.c_int, [vp, ct.c_int, ip, ip, dp, dp, dp, dp]),
         "psba_batch_elapsed": (ct.c_int, [vp, dp, ip]),
         "psba_batch_autotune": (ct.c_int, [vp, ip, ip]),
+        "psba_server_create": (ct.c_int, [ct.POINTER(ServerDesc), ct.POINTER(vp)]),
+        "psba_server_destroy": (ct.c_int, [vp]),
+        "psba_server_clock": (ct.c_double, []),
+        "psba_server_submit": (ct.c_int, [vp, ct.c_int, ct.c_int, ct.c_int, dp, dp, dp, dp, dp, ip, ip,
+                                          ct.POINTER(ct.c_ubyte), ct.POINTER(ct.c_longlong)]),
+        "psba_server_step": (ct.c_int, [vp]),
+        "psba_server_drain": (ct.c_int, [vp]),
+        "psba_server_result": (ct.c_int, [vp, ct.c_longlong, ip, ip, dp, dp, dp, dp, dp, dp]),
+        "psba_server_get_stats": (ct.c_int, [vp, ct.POINTER(ServerStats)]),
+        "psba_server_completed": (ct.c_int, [vp, ct.POINTER(ct.c_longlong), ct.c_int]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -541,3 +563,90 @@ class Batch:
         ob, pb = ct.c_int(0), ct.c_int(0)
         _check(lib().psba_batch_autotune(self._h, ct.byref(ob), ct.byref(pb)), "psba_batch_autotune")
         return ob.value, pb.value
+
+
+def server_clock():
+    """Seconds on the steady clock the server stamps requests with."""
+    return lib().psba_server_clock()
+
+
+class Server:
+    """Request loop over batches (include/psba_server.h)."""
+
+    def __init__(self, cnp, precision=PREC_FP64, device=0, lanes=0, max_batch=0, hbm_budget=0, max_iter=100,
+                 rel_tol=1e-8):
+        L = lib()
+        self._h = ct.c_void_p()
+        d = ServerDesc(cnp, precision, device, lanes, max_batch, int(hbm_budget), max_iter, rel_tol)
+        _check(L.psba_server_create(ct.byref(d), ct.byref(self._h)), "psba_server_create")
+        self.cnp = cnp
+        self._sizes = {}
+
+    def close(self):
+        if self._h:
+            lib().psba_server_destroy(self._h)
+            self._h = ct.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def submit(self, K, impts, initrot, cams, pts, iidx, jidx, fixmask=None):
+        a = [np.ascontiguousarray(x, dtype=np.float64) for x in (K, impts, initrot, cams, pts)]
+        i = np.ascontiguousarray(iidx, dtype=np.int32)
+        j = np.ascontiguousarray(jidx, dtype=np.int32)
+        nc, npt, no = a[3].size // self.cnp, a[4].size // 3, i.size
+        fm = None if fixmask is None else np.ascontiguousarray(fixmask, dtype=np.uint8)
+        rid = ct.c_longlong(0)
+        _check(lib().psba_server_submit(self._h, nc, npt, no, *[_dptr(x) for x in a], _iptr(i), _iptr(j),
+                                        None if fm is None else fm.ctypes.data_as(ct.POINTER(ct.c_ubyte)),
+                                        ct.byref(rid)), "psba_server_submit")
+        self._sizes[rid.value] = (nc, npt)
+        return rid.value
+
+    def submit_scene(self, sc, fix=True):
+        return self.submit(sc.K, sc.meas, sc.q0, sc.cams, sc.pts, sc.iidx, sc.jidx, sc.fixmask if fix else None)
+
+    def step(self):
+        return _check(lib().psba_server_step(self._h), "psba_server_step")
+
+    def drain(self):
+        _check(lib().psba_server_drain(self._h), "psba_server_drain")
+
+    def result(self, rid, params=True):
+        """The request's result (and release), or None while it runs."""
+        nc, npt = self._sizes.get(rid, (0, 0))
+        code, it = ct.c_int(0), ct.c_int(0)
+        ie, fe, ts, td = ct.c_double(0), ct.c_double(0), ct.c_double(0), ct.c_double(0)
+        cams = np.empty(self.cnp * nc) if params else None
+        pts = np.empty(3 * npt) if params else None
+        r = _check(lib().psba_server_result(self._h, rid, ct.byref(code), ct.byref(it), ct.byref(ie), ct.byref(fe),
+                                            _dptr(cams) if params else None, _dptr(pts) if params else None,
+                                            ct.byref(ts), ct.byref(td)), "psba_server_result")
+        if r == 0:
+            return None
+        self._sizes.pop(rid, None)
+        out = dict(code=code.value, itno=it.value, init_err=ie.value, final_err=fe.value, t_submit=ts.value,
+                   t_done=td.value)
+        if params:
+            out["cams"], out["pts"] = cams, pts
+        return out
+
+    def completed(self, cap=4096):
+        """Ids completed since the last call, in completion order."""
+        buf = (ct.c_longlong * cap)()
+        n = _check(lib().psba_server_completed(self._h, buf, cap), "psba_server_completed")
+        return [buf[k] for k in range(n)]
+
+    def stats(self):
+        st = ServerStats()
+        _check(lib().psba_server_get_stats(self._h, ct.byref(st)), "psba_server_get_stats")
+        return {k: getattr(st, k) for k, _ in ServerStats._fields_}

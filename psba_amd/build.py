@@ -23,7 +23,7 @@ ORACLE_LIB = os.path.join(ORACLE_DIR, "liboracle.so")
 ARCH = os.environ.get("PSBA_OFFLOAD_ARCH", "gfx950")
 HIPCC = os.environ.get("HIPCC", shutil.which("hipcc") or "/opt/rocm/bin/hipcc")
 
-SOURCES = ["ctx.cpp", "drivers.cpp", "dropin.cpp", "scene.cpp", "sba_io.cpp", "batch.cpp",
+SOURCES = ["ctx.cpp", "drivers.cpp", "dropin.cpp", "scene.cpp", "sba_io.cpp", "batch.cpp", "server.cpp",
            "k_obs.hip", "k_schur.hip", "k_dense.hip", "k_tr.hip", "k_batch.hip"]
 
 HIP_FLAGS = [

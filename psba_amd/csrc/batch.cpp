@@ -125,17 +125,16 @@ struct Carver {
   }
 };
 
-size_t carve(psba_batch* b, const Sizes& z, int C, size_t R, char* base) {
+// Views in this order: the inputs (uploaded from the staging image), then the
+// working parameters, work arrays and per-problem state.  *input_end receives
+// the size of the input part.
+size_t carve(psba_batch* t, const Sizes& z, int C, size_t R, char* base, size_t* input_end = nullptr) {
   Carver cv(base);
-  psba_batch tmp;
-  psba_batch* t = base ? b : &tmp;
   const long NE = (long)C * (C + 1) / 2 + C;
   const long ntot = z.N + 3 * z.NP;
   cv.take(&t->K, 5 * z.NC);
   cv.take(&t->q0, 4 * z.NC);
   cv.take(&t->meas, 2 * z.NO);
-  cv.take(&t->cams, z.N);
-  cv.take(&t->pts, 3 * z.NP);
   cv.take(&t->cams0, z.N);
   cv.take(&t->pts0, 3 * z.NP);
   cv.take(&t->fix, z.N);
@@ -169,6 +168,10 @@ size_t carve(psba_batch* b, const Sizes& z, int C, size_t R, char* base) {
   cv.take(&t->npad, z.P);
   cv.take(&t->vec_off, z.P + 1);
   cv.take(&t->S_off, z.P + 1);
+  cv.take(&t->all_on, z.P);
+  if (input_end) *input_end = cv.off;
+  cv.take(&t->cams, z.N);
+  cv.take(&t->pts, 3 * z.NP);
   cv.take_raw(&t->e, 2 * z.NO, R);
   cv.take_raw(&t->JA, 2 * C * z.NO, R);
   cv.take_raw(&t->JB, 6 * z.NO, R);
@@ -206,8 +209,7 @@ size_t carve(psba_batch* b, const Sizes& z, int C, size_t R, char* base) {
   cv.take(&t->need_jac, z.P);
   cv.take(&t->accept, z.P);
   cv.take(&t->first, z.P);
-  cv.take(&t->all_on, z.P);
-  cv.take(&t->active_count, 1);
+  cv.take(&t->stat, z.P + 1);
   return cv.off;
 }
 
@@ -232,16 +234,19 @@ Sizes problem_sizes(int C, int nc, int np, int no, long pair_items, long npairs)
 }
 
 int free_graph(psba_batch* b) {
-  if (b->graph_exec) (void)hipGraphExecDestroy(b->graph_exec);
-  if (b->graph) (void)hipGraphDestroy(b->graph);
-  b->graph_exec = nullptr;
-  b->graph = nullptr;
+  for (int k = 0; k < 2; ++k) {
+    if (b->graph_exec[k]) (void)hipGraphExecDestroy(b->graph_exec[k]);
+    if (b->graph[k]) (void)hipGraphDestroy(b->graph[k]);
+    b->graph_exec[k] = nullptr;
+    b->graph[k] = nullptr;
+  }
   b->graph_iter = -1;
   return PSBA_OK;
 }
 
-// one LM round: every problem still running makes one trial
-int enqueue_round(psba_batch* b) {
+// one LM round: every problem still running makes one trial; the round ends
+// with the per-problem status copied to pinned status buffer `parity`
+int enqueue_round(psba_batch* b, int parity) {
   PSBA_TRY(launch_b_jacobian(b));
   PSBA_TRY(launch_b_lm(b, BLM_MU0));
   PSBA_TRY(launch_b_vinv(b));
@@ -252,31 +257,34 @@ int enqueue_round(psba_batch* b) {
   PSBA_TRY(launch_b_lm(b, BLM_TRIAL_COST));
   PSBA_TRY(launch_b_lm(b, BLM_CONTROL));
   PSBA_TRY(launch_b_lm(b, BLM_COMMIT));
-  PSBA_HIP_TRY(hipMemcpyAsync(b->h_active, b->active_count, sizeof(int), hipMemcpyDeviceToHost, b->stream));
+  PSBA_HIP_TRY(hipMemcpyAsync(b->h_stat + (size_t)parity * (b->P + 1), b->stat, sizeof(BatchStat) * (size_t)(b->P + 1),
+                              hipMemcpyDeviceToHost, b->stream));
   return PSBA_OK;
 }
 
 int capture_round(psba_batch* b) {
-  if (b->graph_exec && b->graph_iter == b->max_iter && b->graph_tol == b->rel_tol) return PSBA_OK;
+  if (b->graph_exec[0] && b->graph_iter == b->max_iter && b->graph_tol == b->rel_tol) return PSBA_OK;
   free_graph(b);
-  PSBA_HIP_TRY(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
-  int rc = enqueue_round(b);
-  hipGraph_t g = nullptr;
-  hipError_t e = hipStreamEndCapture(b->stream, &g);
-  if (rc != PSBA_OK) {
-    if (g) (void)hipGraphDestroy(g);
-    return rc;
+  for (int k = 0; k < 2; ++k) {
+    PSBA_HIP_TRY(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+    int rc = enqueue_round(b, k);
+    hipGraph_t g = nullptr;
+    hipError_t e = hipStreamEndCapture(b->stream, &g);
+    if (rc != PSBA_OK) {
+      if (g) (void)hipGraphDestroy(g);
+      return rc;
+    }
+    PSBA_HIP_TRY(e);
+    b->graph[k] = g;
+    PSBA_HIP_TRY(hipGraphInstantiate(&b->graph_exec[k], b->graph[k], nullptr, nullptr, 0));
   }
-  PSBA_HIP_TRY(e);
-  b->graph = g;
-  PSBA_HIP_TRY(hipGraphInstantiate(&b->graph_exec, b->graph, nullptr, nullptr, 0));
   b->graph_iter = b->max_iter;
   b->graph_tol = b->rel_tol;
   return PSBA_OK;
 }
 
 int launch_round(psba_batch* b) {
-  PSBA_HIP_TRY(hipGraphLaunch(b->graph_exec, b->stream));
+  PSBA_HIP_TRY(hipGraphLaunch(b->graph_exec[b->launched % 2], b->stream));
   PSBA_HIP_TRY(hipEventRecord(b->ev_round[b->launched % psba_batch::kDepth], b->stream));
   b->launched++;
   return PSBA_OK;
@@ -294,9 +302,61 @@ int finish(psba_batch* b) {
   return PSBA_OK;
 }
 
-// advance: check the oldest queued round, keep kDepth rounds queued;
-// blocking waits for it.  Returns 1 when every problem has stopped.
-int advance(psba_batch* b, bool blocking) {
+int threads_for(long work, int items) {
+  const int hw = (int)std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+  const int want = (int)std::min<long>(hw, std::max<long>(1, work / 200000));
+  return std::max(1, std::min(want, items));
+}
+
+template <typename F>
+void parallel_for(int n, int nt, F&& fn) {
+  if (nt <= 1) {
+    for (int p = 0; p < n; ++p) fn(p);
+    return;
+  }
+  std::vector<std::thread> th;
+  for (int t = 0; t < nt; ++t)
+    th.emplace_back([&, t] {
+      for (int p = t; p < n; p += nt) fn(p);
+    });
+  for (auto& x : th) x.join();
+}
+
+int check_problem(const psba_batch* b, int n_cams, int n_pts, int n_obs, const int* iidx, const int* jidx) {
+  if (n_cams <= 0 || n_pts <= 0 || n_obs <= 0) return bfail(PSBA_ERR_ARG, "empty problem");
+  if (b->cap_p > 0 && (int)b->staged.size() >= b->cap_p)
+    return bfail(PSBA_ERR_ARG, "batch holds at most %d problems", b->cap_p);
+  const long n = (long)b->cnp * n_cams;
+  if (pad64(n) > kMaxSolveN)
+    return bfail(PSBA_ERR_UNSUPPORTED, "reduced camera system of order %ld exceeds the limit %d", n, kMaxSolveN);
+  for (int k = 0; k < n_obs; ++k) {
+    if (iidx[k] < 0 || iidx[k] >= n_pts || jidx[k] < 0 || jidx[k] >= n_cams)
+      return bfail(PSBA_ERR_ARG, "observation %d has index out of range", k);
+    if (k > 0 && (iidx[k] < iidx[k - 1] || (iidx[k] == iidx[k - 1] && jidx[k] <= jidx[k - 1])))
+      return bfail(PSBA_ERR_ARG, "observations must be point-major and camera-ascending (obs %d)", k);
+  }
+  return PSBA_OK;
+}
+
+}  // namespace
+
+namespace psba {
+
+int batch_add_staged(psba_batch* b, psba_batch::Staged&& s) {
+  b->staged.push_back(std::move(s));
+  b->committed = false;
+  return (int)b->staged.size() - 1;
+}
+
+long long staged_bytes(int cnp, int precision, const psba_batch::Staged& s) {
+  std::vector<int> t(s.np, 0);
+  for (int k = 0; k < s.no; ++k) t[s.iidx[k]]++;
+  long long items = 0;
+  for (int v : t) items += (long long)v * (v + 1) / 2;
+  return psba_problem_bytes(cnp, precision, s.nc, s.np, s.no, items);
+}
+
+int batch_advance(psba_batch* b, bool blocking, void (*cb)(void*, const BatchStat*), void* ctx) {
   if (!b->running) return b->finished ? 1 : 0;
   while (b->launched - b->checked < psba_batch::kDepth && b->launched < b->max_rounds) PSBA_TRY(launch_round(b));
   hipEvent_t ev = b->ev_round[b->checked % psba_batch::kDepth];
@@ -307,9 +367,12 @@ int advance(psba_batch* b, bool blocking) {
     if (q == hipErrorNotReady) return 0;
     PSBA_HIP_TRY(q);
   }
+  // round `checked` is complete and the next launch would be round checked + 2,
+  // so its status buffer is stable until then
+  const BatchStat* st = b->h_stat + (size_t)(b->checked % 2) * (b->P + 1);
   b->checked++;
-  // the pinned count is from this round or a later one; it never grows
-  if (*(volatile int*)b->h_active == 0 || b->checked >= b->max_rounds) {
+  if (cb) cb(ctx, st);
+  if (st[0].active == 0 || b->checked >= b->max_rounds) {
     PSBA_TRY(finish(b));
     return 1;
   }
@@ -317,7 +380,24 @@ int advance(psba_batch* b, bool blocking) {
   return 0;
 }
 
-}  // namespace
+const BatchStat* batch_final_stat(const psba_batch* b) {
+  return b->h_stat + (size_t)((b->launched - 1) % 2) * (b->P + 1);
+}
+
+int batch_release(psba_batch* b) {
+  if (b->running) return bfail(PSBA_ERR_STATE, "a solve is in flight");
+  PSBA_HIP_TRY(hipSetDevice(b->device));
+  PSBA_HIP_TRY(hipStreamSynchronize(b->stream));
+  free_graph(b);
+  if (b->arena) (void)hipFree(b->arena);
+  b->arena = nullptr;
+  b->arena_bytes = 0;
+  b->committed = false;
+  b->finished = false;
+  return PSBA_OK;
+}
+
+}  // namespace psba
 
 extern "C" {
 
@@ -327,8 +407,8 @@ long long psba_problem_bytes(int cnp, int precision, int n_cams, int n_pts, int 
   const long npairs = std::min<long long>((long long)n_cams * (n_cams + 1) / 2, pair_items);
   const Sizes z = problem_sizes(cnp, n_cams, n_pts, n_obs, (long)pair_items, npairs);
   const size_t R = precision == PSBA_PREC_MIXED ? sizeof(float) : sizeof(double);
-  // every view of a one-problem layout may add up to 255 bytes of alignment
-  return (long long)carve(nullptr, z, cnp, R, nullptr);
+  psba_batch tmp;
+  return (long long)carve(&tmp, z, cnp, R, nullptr);
 }
 
 int psba_batch_create(const psba_batch_desc* d, psba_batch** out) {
@@ -349,10 +429,10 @@ int psba_batch_create(const psba_batch_desc* d, psba_batch** out) {
   b->device = d->device;
   b->cap_p = d->max_problems;
   hipError_t e = hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking);
-  if (e == hipSuccess) e = hipHostMalloc((void**)&b->h_active, sizeof(int), hipHostMallocDefault);
   for (int k = 0; k < psba_batch::kDepth && e == hipSuccess; ++k) e = hipEventCreate(&b->ev_round[k]);
   if (e == hipSuccess) e = hipEventCreate(&b->ev_start);
   if (e == hipSuccess) e = hipEventCreate(&b->ev_stop);
+  if (e == hipSuccess) e = hipEventCreateWithFlags(&b->ev_staged, hipEventDisableTiming);
   if (e != hipSuccess) {
     psba_batch_destroy(b);
     return bfail(PSBA_ERR_HIP, "batch creation: %s", hipGetErrorString(e));
@@ -367,11 +447,13 @@ int psba_batch_destroy(psba_batch* b) {
   if (b->stream) (void)hipStreamSynchronize(b->stream);
   free_graph(b);
   if (b->arena) (void)hipFree(b->arena);
-  if (b->h_active) (void)hipHostFree(b->h_active);
+  if (b->staging) (void)hipHostFree(b->staging);
+  if (b->h_stat) (void)hipHostFree(b->h_stat);
   for (auto& ev : b->ev_round)
     if (ev) (void)hipEventDestroy(ev);
   if (b->ev_start) (void)hipEventDestroy(b->ev_start);
   if (b->ev_stop) (void)hipEventDestroy(b->ev_stop);
+  if (b->ev_staged) (void)hipEventDestroy(b->ev_staged);
   if (b->stream) (void)hipStreamDestroy(b->stream);
   delete b;
   return PSBA_OK;
@@ -394,18 +476,7 @@ int psba_batch_add(psba_batch* b, int n_cams, int n_pts, int n_obs, const double
                    const unsigned char* fixmask, int* slot) {
   if (!b || !K || !impts || !initrot || !cams || !pts || !iidx || !jidx) return bfail(PSBA_ERR_ARG, "null argument");
   if (b->running) return bfail(PSBA_ERR_STATE, "a solve is in flight");
-  if (n_cams <= 0 || n_pts <= 0 || n_obs <= 0) return bfail(PSBA_ERR_ARG, "empty problem");
-  if (b->cap_p > 0 && (int)b->staged.size() >= b->cap_p)
-    return bfail(PSBA_ERR_ARG, "batch holds at most %d problems", b->cap_p);
-  const long n = (long)b->cnp * n_cams;
-  if (pad64(n) > kMaxSolveN)
-    return bfail(PSBA_ERR_UNSUPPORTED, "reduced camera system of order %ld exceeds the limit %d", n, kMaxSolveN);
-  for (int k = 0; k < n_obs; ++k) {
-    if (iidx[k] < 0 || iidx[k] >= n_pts || jidx[k] < 0 || jidx[k] >= n_cams)
-      return bfail(PSBA_ERR_ARG, "observation %d has index out of range", k);
-    if (k > 0 && (iidx[k] < iidx[k - 1] || (iidx[k] == iidx[k - 1] && jidx[k] <= jidx[k - 1])))
-      return bfail(PSBA_ERR_ARG, "observations must be point-major and camera-ascending (obs %d)", k);
-  }
+  PSBA_TRY(check_problem(b, n_cams, n_pts, n_obs, iidx, jidx));
   psba_batch::Staged s;
   s.nc = n_cams;
   s.np = n_pts;
@@ -422,9 +493,8 @@ int psba_batch_add(psba_batch* b, int n_cams, int n_pts, int n_obs, const double
     s.fix.assign(fixmask, fixmask + (size_t)C * n_cams);
   else
     s.fix.assign((size_t)C * n_cams, 0);
-  if (slot) *slot = (int)b->staged.size();
-  b->staged.push_back(std::move(s));
-  b->committed = false;
+  const int k = batch_add_staged(b, std::move(s));
+  if (slot) *slot = k;
   return PSBA_OK;
 }
 
@@ -435,18 +505,13 @@ int psba_batch_commit(psba_batch* b) {
   if (P == 0) return bfail(PSBA_ERR_STATE, "no problems added");
   PSBA_HIP_TRY(hipSetDevice(b->device));
   const int C = b->cnp;
-  // per-problem index, built in parallel
+  long total_obs = 0;
+  for (const auto& s : b->staged) total_obs += s.no;
+  const int nt = threads_for(total_obs, P);
+  // per-problem index in its own numbering
   std::vector<LocalIndex> loc(P);
   std::vector<int> rcs(P, PSBA_OK);
-  {
-    const int nt = std::max(1, std::min<int>(std::min(16u, std::max(1u, std::thread::hardware_concurrency())), P));
-    std::vector<std::thread> th;
-    for (int t = 0; t < nt; ++t)
-      th.emplace_back([&, t] {
-        for (int p = t; p < P; p += nt) rcs[p] = build_local(b->staged[p], C, loc[p]);
-      });
-    for (auto& x : th) x.join();
-  }
+  parallel_for(P, nt, [&](int p) { rcs[p] = build_local(b->staged[p], C, loc[p]); });
   for (int p = 0; p < P; ++p)
     if (rcs[p] != PSBA_OK) return rcs[p];
 
@@ -457,16 +522,9 @@ int psba_batch_commit(psba_batch* b) {
   for (int p = 0; p < P; ++p) {
     const auto& s = b->staged[p];
     const auto& L = loc[p];
-    auto add = [&](long a, long d, const char* what) -> long {
-      if (a + d > 0x7fffffffL) return -1;
-      (void)what;
-      return a + d;
-    };
-    long v;
-    if ((v = add(cam_off[p], s.nc, "cameras")) < 0 || (v = add(pt_off[p], s.np, "points")) < 0 ||
-        (v = add(obs_off[p], s.no, "observations")) < 0 || (v = add((long)C * cam_off[p], (long)C * s.nc, "")) < 0 ||
-        (v = add(item_off[p], (long)L.items.size(), "items")) < 0 ||
-        (v = add(3L * pt_off[p] + (long)C * cam_off[p], 3L * s.np + (long)C * s.nc, "")) < 0)
+    const long lim = 0x7fffffffL;
+    if ((long)obs_off[p] + s.no > lim || item_off[p] + (long)L.items.size() > lim ||
+        3L * (pt_off[p] + s.np) + (long)C * (cam_off[p] + s.nc) > lim)
       return bfail(PSBA_ERR_UNSUPPORTED, "batch exceeds 32-bit indexing at problem %d", p);
     cam_off[p + 1] = cam_off[p] + s.nc;
     pt_off[p + 1] = pt_off[p] + s.np;
@@ -496,84 +554,13 @@ int psba_batch_commit(psba_batch* b) {
   z.S_len = S_off[P];
   z.V_len = vec_off[P];
 
-  // host images of the global arrays
-  std::vector<double> K(5 * z.NC), q0(4 * z.NC), meas(2 * z.NO), cams(z.N), pts(3 * z.NP);
-  std::vector<unsigned char> fix(z.N);
-  std::vector<int> obs_cam(z.NO), obs_pt(z.NO), obs_prob(z.NO), pt_ptr(z.NP + 1), pt_prob(z.NP), cam_obs(z.NO),
-      cam_prob(z.NC), cb(z.n_cchunk), ce(z.n_cchunk), ccp(z.NC + 1), cchp(z.n_cchunk), sb(z.n_schunk),
-      se(z.n_schunk), psp(z.npairs + 1), schp(z.n_schunk), lbp(z.nlb), lbk(z.nlb), lbl(z.nlb), lbq(z.nlb),
-      rb(z.n_rchunk), re(z.n_rchunk), rp(z.n_rchunk), rcp(P + 1);
-  std::vector<int2> items((size_t)z.items);
-  {
-    const int nt = std::max(1, std::min<int>(std::min(16u, std::max(1u, std::thread::hardware_concurrency())), P));
-    std::vector<std::thread> th;
-    for (int t = 0; t < nt; ++t)
-      th.emplace_back([&, t] {
-        for (int p = t; p < P; p += nt) {
-          const auto& s = b->staged[p];
-          const auto& L = loc[p];
-          const int co = cam_off[p], po = pt_off[p], oo = obs_off[p];
-          std::copy(s.K.begin(), s.K.end(), K.begin() + 5L * co);
-          std::copy(s.q0.begin(), s.q0.end(), q0.begin() + 4L * co);
-          std::copy(s.meas.begin(), s.meas.end(), meas.begin() + 2L * oo);
-          std::copy(s.cams.begin(), s.cams.end(), cams.begin() + (long)C * co);
-          std::copy(s.pts.begin(), s.pts.end(), pts.begin() + 3L * po);
-          std::copy(s.fix.begin(), s.fix.end(), fix.begin() + (long)C * co);
-          for (int k = 0; k < s.no; ++k) {
-            obs_cam[oo + k] = co + s.jidx[k];
-            obs_pt[oo + k] = po + s.iidx[k];
-            obs_prob[oo + k] = p;
-            cam_obs[oo + k] = oo + L.cam_obs[k];
-          }
-          for (int i = 0; i < s.np; ++i) {
-            pt_ptr[po + i] = oo + L.pt_ptr[i];
-            pt_prob[po + i] = p;
-          }
-          for (int j = 0; j < s.nc; ++j) {
-            cam_prob[co + j] = p;
-            ccp[co + j] = cch_off[p] + L.ccp[j];
-          }
-          for (size_t q = 0; q < L.cb.size(); ++q) {
-            cb[cch_off[p] + q] = oo + L.cb[q];
-            ce[cch_off[p] + q] = oo + L.ce[q];
-            cchp[cch_off[p] + q] = p;
-          }
-          const long io = item_off[p];
-          for (size_t q = 0; q < L.items.size(); ++q)
-            items[(size_t)io + q] = make_int2(oo + L.items[q].x, oo + L.items[q].y);
-          for (size_t q = 0; q < L.sb.size(); ++q) {
-            sb[sch_off[p] + q] = (int)(io + L.sb[q]);
-            se[sch_off[p] + q] = (int)(io + L.se[q]);
-            schp[sch_off[p] + q] = p;
-          }
-          for (int q = 0; q < L.npairs; ++q) psp[pair_off[p] + q] = sch_off[p] + L.psp[q];
-          for (int k = 0, f = lb_off[p]; k < s.nc; ++k)
-            for (int l = 0; l <= k; ++l, ++f) {
-              const int lq = L.lbpair[(size_t)k * (k + 1) / 2 + l];
-              lbp[f] = p;
-              lbk[f] = k;
-              lbl[f] = l;
-              lbq[f] = lq < 0 ? -1 : pair_off[p] + lq;
-            }
-          for (int q = 0, a = 0; a < s.no; ++q, a += kResidChunk) {
-            rb[rch_off[p] + q] = oo + a;
-            re[rch_off[p] + q] = oo + std::min(a + kResidChunk, s.no);
-            rp[rch_off[p] + q] = p;
-          }
-          rcp[p] = rch_off[p];
-        }
-      });
-    for (auto& x : th) x.join();
-  }
-  pt_ptr[z.NP] = (int)z.NO;
-  ccp[z.NC] = (int)z.n_cchunk;
-  psp[z.npairs] = (int)z.n_schunk;
-  rcp[P] = (int)z.n_rchunk;
-
-  // arena
+  // device arena (grows only) and pinned staging image of its input part
   const size_t R = b->mixed() ? sizeof(float) : sizeof(double);
-  const size_t need = carve(b, z, C, R, nullptr);
+  size_t in_bytes = 0;
+  psba_batch hv;  // views into the staging image
+  const size_t need = carve(&hv, z, C, R, nullptr, &in_bytes);
   free_graph(b);
+  PSBA_HIP_TRY(hipEventSynchronize(b->ev_staged));  // the previous upload has left the staging image
   if (need > b->arena_bytes) {
     PSBA_HIP_TRY(hipStreamSynchronize(b->stream));
     if (b->arena) (void)hipFree(b->arena);
@@ -586,7 +573,103 @@ int psba_batch_commit(psba_batch* b) {
     }
     b->arena_bytes = need;
   }
+  if (in_bytes > b->staging_bytes) {
+    if (b->staging) (void)hipHostFree(b->staging);
+    b->staging = nullptr;
+    b->staging_bytes = 0;
+    hipError_t e = hipHostMalloc((void**)&b->staging, in_bytes, hipHostMallocDefault);
+    if (e != hipSuccess) {
+      b->staging = nullptr;
+      return bfail(PSBA_ERR_NOMEM, "pinned allocation of %zu bytes failed: %s", in_bytes, hipGetErrorString(e));
+    }
+    b->staging_bytes = in_bytes;
+  }
+  if (2 * (size_t)(P + 1) > b->h_stat_cap) {
+    if (b->h_stat) (void)hipHostFree(b->h_stat);
+    b->h_stat = nullptr;
+    b->h_stat_cap = 0;
+    hipError_t e = hipHostMalloc((void**)&b->h_stat, sizeof(BatchStat) * 2 * (size_t)(P + 1), hipHostMallocDefault);
+    if (e != hipSuccess) {
+      b->h_stat = nullptr;
+      return bfail(PSBA_ERR_NOMEM, "pinned allocation failed: %s", hipGetErrorString(e));
+    }
+    b->h_stat_cap = 2 * (size_t)(P + 1);
+  }
+  carve(&hv, z, C, R, b->staging);
+
+  // fill the staging image with the global arrays
+  parallel_for(P, nt, [&](int p) {
+    const auto& s = b->staged[p];
+    const auto& L = loc[p];
+    const int co = cam_off[p], po = pt_off[p], oo = obs_off[p];
+    std::copy(s.K.begin(), s.K.end(), hv.K + 5L * co);
+    std::copy(s.q0.begin(), s.q0.end(), hv.q0 + 4L * co);
+    std::copy(s.meas.begin(), s.meas.end(), hv.meas + 2L * oo);
+    std::copy(s.cams.begin(), s.cams.end(), hv.cams0 + (long)C * co);
+    std::copy(s.pts.begin(), s.pts.end(), hv.pts0 + 3L * po);
+    std::copy(s.fix.begin(), s.fix.end(), hv.fix + (long)C * co);
+    for (int k = 0; k < s.no; ++k) {
+      hv.obs_cam[oo + k] = co + s.jidx[k];
+      hv.obs_pt[oo + k] = po + s.iidx[k];
+      hv.obs_prob[oo + k] = p;
+      hv.cam_obs[oo + k] = oo + L.cam_obs[k];
+    }
+    for (int i = 0; i < s.np; ++i) {
+      hv.pt_ptr[po + i] = oo + L.pt_ptr[i];
+      hv.pt_prob[po + i] = p;
+    }
+    for (int j = 0; j < s.nc; ++j) {
+      hv.cam_prob[co + j] = p;
+      hv.ccp[co + j] = cch_off[p] + L.ccp[j];
+    }
+    for (size_t q = 0; q < L.cb.size(); ++q) {
+      hv.cchunk_b[cch_off[p] + q] = oo + L.cb[q];
+      hv.cchunk_e[cch_off[p] + q] = oo + L.ce[q];
+      hv.cchunk_prob[cch_off[p] + q] = p;
+    }
+    const long io = item_off[p];
+    for (size_t q = 0; q < L.items.size(); ++q)
+      hv.items[(size_t)io + q] = make_int2(oo + L.items[q].x, oo + L.items[q].y);
+    for (size_t q = 0; q < L.sb.size(); ++q) {
+      hv.schunk_b[sch_off[p] + q] = (int)(io + L.sb[q]);
+      hv.schunk_e[sch_off[p] + q] = (int)(io + L.se[q]);
+      hv.schunk_prob[sch_off[p] + q] = p;
+    }
+    for (int q = 0; q < L.npairs; ++q) hv.psp[pair_off[p] + q] = sch_off[p] + L.psp[q];
+    for (int k = 0, f = lb_off[p]; k < s.nc; ++k)
+      for (int l = 0; l <= k; ++l, ++f) {
+        const int lq = L.lbpair[(size_t)k * (k + 1) / 2 + l];
+        hv.lb_prob[f] = p;
+        hv.lb_k[f] = k;
+        hv.lb_l[f] = l;
+        hv.lb_pair[f] = lq < 0 ? -1 : pair_off[p] + lq;
+      }
+    for (int q = 0, a = 0; a < s.no; ++q, a += kResidChunk) {
+      hv.rchunk_b[rch_off[p] + q] = oo + a;
+      hv.rchunk_e[rch_off[p] + q] = oo + std::min(a + kResidChunk, s.no);
+      hv.rchunk_prob[rch_off[p] + q] = p;
+    }
+    hv.rcp[p] = rch_off[p];
+    hv.cam_off[p] = co;
+    hv.pt_off[p] = po;
+    hv.obs_off[p] = oo;
+    hv.npad[p] = npad[p];
+    hv.vec_off[p] = vec_off[p];
+    hv.S_off[p] = S_off[p];
+    hv.all_on[p] = 1;
+  });
+  hv.pt_ptr[z.NP] = (int)z.NO;
+  hv.ccp[z.NC] = (int)z.n_cchunk;
+  hv.psp[z.npairs] = (int)z.n_schunk;
+  hv.rcp[P] = (int)z.n_rchunk;
+  hv.cam_off[P] = cam_off[P];
+  hv.pt_off[P] = pt_off[P];
+  hv.obs_off[P] = obs_off[P];
+  hv.vec_off[P] = vec_off[P];
+  hv.S_off[P] = S_off[P];
+
   carve(b, z, C, R, b->arena);
+  b->input_bytes = in_bytes;
   b->P = P;
   b->NC = z.NC;
   b->NP = z.NP;
@@ -604,55 +687,13 @@ int psba_batch_commit(psba_batch* b) {
   b->h_pt_off = pt_off;
   b->h_obs_off = obs_off;
 
-  auto up = [&](void* dst, const void* src, size_t bytes) -> int {
-    if (bytes) PSBA_HIP_TRY(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, b->stream));
-    return PSBA_OK;
-  };
-  PSBA_TRY(up(b->K, K.data(), sizeof(double) * K.size()));
-  PSBA_TRY(up(b->q0, q0.data(), sizeof(double) * q0.size()));
-  PSBA_TRY(up(b->meas, meas.data(), sizeof(double) * meas.size()));
-  PSBA_TRY(up(b->cams0, cams.data(), sizeof(double) * cams.size()));
-  PSBA_TRY(up(b->pts0, pts.data(), sizeof(double) * pts.size()));
-  PSBA_TRY(up(b->fix, fix.data(), fix.size()));
-  PSBA_TRY(up(b->obs_cam, obs_cam.data(), sizeof(int) * obs_cam.size()));
-  PSBA_TRY(up(b->obs_pt, obs_pt.data(), sizeof(int) * obs_pt.size()));
-  PSBA_TRY(up(b->obs_prob, obs_prob.data(), sizeof(int) * obs_prob.size()));
-  PSBA_TRY(up(b->pt_ptr, pt_ptr.data(), sizeof(int) * pt_ptr.size()));
-  PSBA_TRY(up(b->pt_prob, pt_prob.data(), sizeof(int) * pt_prob.size()));
-  PSBA_TRY(up(b->cam_obs, cam_obs.data(), sizeof(int) * cam_obs.size()));
-  PSBA_TRY(up(b->cam_prob, cam_prob.data(), sizeof(int) * cam_prob.size()));
-  PSBA_TRY(up(b->cchunk_b, cb.data(), sizeof(int) * cb.size()));
-  PSBA_TRY(up(b->cchunk_e, ce.data(), sizeof(int) * ce.size()));
-  PSBA_TRY(up(b->ccp, ccp.data(), sizeof(int) * ccp.size()));
-  PSBA_TRY(up(b->cchunk_prob, cchp.data(), sizeof(int) * cchp.size()));
-  PSBA_TRY(up(b->items, items.data(), sizeof(int2) * items.size()));
-  PSBA_TRY(up(b->schunk_b, sb.data(), sizeof(int) * sb.size()));
-  PSBA_TRY(up(b->schunk_e, se.data(), sizeof(int) * se.size()));
-  PSBA_TRY(up(b->psp, psp.data(), sizeof(int) * psp.size()));
-  PSBA_TRY(up(b->schunk_prob, schp.data(), sizeof(int) * schp.size()));
-  PSBA_TRY(up(b->lb_prob, lbp.data(), sizeof(int) * lbp.size()));
-  PSBA_TRY(up(b->lb_k, lbk.data(), sizeof(int) * lbk.size()));
-  PSBA_TRY(up(b->lb_l, lbl.data(), sizeof(int) * lbl.size()));
-  PSBA_TRY(up(b->lb_pair, lbq.data(), sizeof(int) * lbq.size()));
-  PSBA_TRY(up(b->rchunk_b, rb.data(), sizeof(int) * rb.size()));
-  PSBA_TRY(up(b->rchunk_e, re.data(), sizeof(int) * re.size()));
-  PSBA_TRY(up(b->rchunk_prob, rp.data(), sizeof(int) * rp.size()));
-  PSBA_TRY(up(b->rcp, rcp.data(), sizeof(int) * rcp.size()));
-  PSBA_TRY(up(b->cam_off, cam_off.data(), sizeof(int) * cam_off.size()));
-  PSBA_TRY(up(b->pt_off, pt_off.data(), sizeof(int) * pt_off.size()));
-  PSBA_TRY(up(b->obs_off, obs_off.data(), sizeof(int) * obs_off.size()));
-  PSBA_TRY(up(b->npad, npad.data(), sizeof(int) * npad.size()));
-  PSBA_TRY(up(b->vec_off, vec_off.data(), sizeof(int) * vec_off.size()));
-  PSBA_TRY(up(b->S_off, S_off.data(), sizeof(long) * S_off.size()));
-  {
-    std::vector<unsigned char> on(P, 1);
-    PSBA_TRY(up(b->all_on, on.data(), P));
-  }
-  // the dense blocks start as zeros so that untouched upper parts are finite
+  // one upload; the dense blocks start as zeros so that untouched upper parts
+  // are finite
+  PSBA_HIP_TRY(hipMemcpyAsync(b->arena, b->staging, in_bytes, hipMemcpyHostToDevice, b->stream));
+  PSBA_HIP_TRY(hipEventRecord(b->ev_staged, b->stream));
   PSBA_HIP_TRY(hipMemsetAsync(b->S, 0, sizeof(double) * (size_t)z.S_len, b->stream));
   PSBA_HIP_TRY(hipMemsetAsync(b->Lf, 0, sizeof(double) * (size_t)z.S_len, b->stream));
   PSBA_HIP_TRY(hipMemsetAsync(b->dp, 0, sizeof(double) * (size_t)(z.N + 3 * z.NP), b->stream));
-  PSBA_HIP_TRY(hipStreamSynchronize(b->stream));
   b->committed = true;
   b->finished = false;
   return PSBA_OK;
@@ -669,12 +710,11 @@ int psba_batch_launch(psba_batch* b, const psba_opts* o) {
   // doublings, so each iteration takes at most 31 rounds
   b->max_rounds = b->max_iter * 31 + 2;
   PSBA_TRY(capture_round(b));
-  const size_t nbytes = sizeof(double);
   PSBA_HIP_TRY(hipEventRecord(b->ev_start, b->stream));
-  PSBA_HIP_TRY(hipMemcpyAsync(b->cams, b->cams0, nbytes * (size_t)b->N, hipMemcpyDeviceToDevice, b->stream));
-  PSBA_HIP_TRY(hipMemcpyAsync(b->pts, b->pts0, nbytes * 3 * (size_t)b->NP, hipMemcpyDeviceToDevice, b->stream));
+  PSBA_HIP_TRY(hipMemcpyAsync(b->cams, b->cams0, sizeof(double) * (size_t)b->N, hipMemcpyDeviceToDevice, b->stream));
+  PSBA_HIP_TRY(
+      hipMemcpyAsync(b->pts, b->pts0, sizeof(double) * 3 * (size_t)b->NP, hipMemcpyDeviceToDevice, b->stream));
   PSBA_TRY(launch_b_lm(b, BLM_INIT));
-  *b->h_active = -1;
   b->launched = 0;
   b->checked = 0;
   b->running = true;
@@ -689,7 +729,7 @@ int psba_batch_poll(psba_batch* b) {
   PSBA_HIP_TRY(hipSetDevice(b->device));
   // drain every round that has completed
   for (;;) {
-    const int r = advance(b, false);
+    const int r = batch_advance(b, false);
     if (r != 0) return r;
     if (b->launched == b->checked) return 0;
     if (hipEventQuery(b->ev_round[b->checked % psba_batch::kDepth]) == hipErrorNotReady) return 0;
@@ -699,7 +739,7 @@ int psba_batch_poll(psba_batch* b) {
 int psba_batch_solve(psba_batch* b, const psba_opts* o, int* rounds) {
   PSBA_TRY(psba_batch_launch(b, o));
   for (;;) {
-    const int r = advance(b, true);
+    const int r = batch_advance(b, true);
     if (r < 0) return r;
     if (r == 1) break;
   }
@@ -713,16 +753,11 @@ int psba_batch_result(psba_batch* b, int slot, int* code, int* itno, double* ini
   if (!b->finished) return bfail(PSBA_ERR_STATE, "no finished solve");
   if (slot < 0 || slot >= b->P) return bfail(PSBA_ERR_ARG, "slot %d out of range", slot);
   PSBA_HIP_TRY(hipSetDevice(b->device));
-  int ci[2];
-  double cd[2];
-  PSBA_HIP_TRY(hipMemcpy(&ci[0], b->code + slot, sizeof(int), hipMemcpyDeviceToHost));
-  PSBA_HIP_TRY(hipMemcpy(&ci[1], b->itno + slot, sizeof(int), hipMemcpyDeviceToHost));
-  PSBA_HIP_TRY(hipMemcpy(&cd[0], b->init_err + slot, sizeof(double), hipMemcpyDeviceToHost));
-  PSBA_HIP_TRY(hipMemcpy(&cd[1], b->cost + slot, sizeof(double), hipMemcpyDeviceToHost));
-  if (code) *code = ci[0];
-  if (itno) *itno = ci[1];
-  if (init_err) *init_err = cd[0];
-  if (final_err) *final_err = cd[1];
+  const BatchStat& st = batch_final_stat(b)[slot + 1];
+  if (code) *code = st.code;
+  if (itno) *itno = st.itno;
+  if (init_err) *init_err = st.init_err;
+  if (final_err) *final_err = st.cost;
   const int C = b->cnp;
   if (cams)
     PSBA_HIP_TRY(hipMemcpy(cams, b->cams + (size_t)C * b->h_cam_off[slot],

@@ -15,6 +15,13 @@
 #include "common.hpp"
 #include "psba_batch.h"
 
+// Per-problem outcome, copied to pinned host memory at the end of every round
+// (entry 0 is a header whose `active` field counts the problems still running).
+struct BatchStat {
+  double cost, init_err;
+  int itno, code, active, pad;
+};
+
 struct psba_batch {
   int cnp = 0, precision = 0, device = 0;
   int cap_p = 0;
@@ -36,9 +43,13 @@ struct psba_batch {
   int n_cchunk = 0, n_schunk = 0, nlb = 0, npairs = 0, n_rchunk = 0;
   bool committed = false;
 
-  // device arena
+  // device arena; the inputs occupy its first input_bytes and are uploaded
+  // from a pinned staging image with one copy
   char* arena = nullptr;
-  size_t arena_bytes = 0;
+  size_t arena_bytes = 0, input_bytes = 0;
+  char* staging = nullptr;
+  size_t staging_bytes = 0;
+  hipEvent_t ev_staged = nullptr;
 
   // problem data
   double *K = nullptr, *q0 = nullptr, *meas = nullptr, *cams = nullptr, *pts = nullptr;
@@ -69,8 +80,9 @@ struct psba_batch {
   int *nu = nullptr, *itno = nullptr, *code = nullptr;
   unsigned char *active = nullptr, *need_jac = nullptr, *accept = nullptr, *first = nullptr;
   unsigned char* all_on = nullptr;  // every problem (initial cost)
-  int* active_count = nullptr;      // device
-  int* h_active = nullptr;          // pinned host copy, written by every round
+  BatchStat* stat = nullptr;        // device, P + 1 entries
+  BatchStat* h_stat = nullptr;      // pinned host copies, 2 x (P + 1) entries
+  size_t h_stat_cap = 0;
 
   // solve in flight
   bool running = false, finished = false;
@@ -78,8 +90,11 @@ struct psba_batch {
   double rel_tol = 1e-8;
   int graph_iter = -1;
   double graph_tol = -1;
-  hipGraph_t graph = nullptr;
-  hipGraphExec_t graph_exec = nullptr;
+  // two captures of the round, identical but for the pinned status buffer
+  // they write (round r writes buffer r % 2), so the status of a checked
+  // round stays stable until the round two later is launched
+  hipGraph_t graph[2] = {nullptr, nullptr};
+  hipGraphExec_t graph_exec[2] = {nullptr, nullptr};
   static constexpr int kDepth = 2;  // rounds queued ahead of the host's check
   hipEvent_t ev_round[kDepth] = {nullptr, nullptr};
   int launched = 0, checked = 0;
@@ -105,5 +120,16 @@ int launch_b_backsub(psba_batch* b);        // dp point part
 int launch_b_lm(psba_batch* b, int stage);  // per-problem LM stages (k_batch.hip)
 
 enum BatchLmStage { BLM_INIT = 0, BLM_MU0, BLM_STEP_STATS, BLM_TRIAL_COST, BLM_CONTROL, BLM_COMMIT, BLM_COUNT };
+
+// host side, for the serving loop (batch.cpp)
+int batch_add_staged(psba_batch* b, psba_batch::Staged&& s);  // checked by the caller
+// Check the oldest queued round (blocking: wait for it), hand its status to
+// cb (if non-null) while it is stable, then keep kDepth rounds queued.
+// Returns 1 once every problem has stopped, 0 otherwise.
+int batch_advance(psba_batch* b, bool blocking, void (*cb)(void*, const BatchStat*) = nullptr,
+                  void* ctx = nullptr);
+const BatchStat* batch_final_stat(const psba_batch* b);  // after the solve finished
+long long staged_bytes(int cnp, int precision, const psba_batch::Staged& s);
+int batch_release(psba_batch* b);                             // free the arena (idle batch)
 
 }  // namespace psba

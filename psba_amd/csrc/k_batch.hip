@@ -72,7 +72,8 @@ __global__ void k_b_init(int P, const int* __restrict__ rcp, const double* __res
                          double* __restrict__ cost, double* __restrict__ init_err, double* __restrict__ mu,
                          int* __restrict__ nu, int* __restrict__ itno, int* __restrict__ code,
                          unsigned char* __restrict__ active, unsigned char* __restrict__ need_jac,
-                         unsigned char* __restrict__ first, unsigned char* __restrict__ accept, int max_iter) {
+                         unsigned char* __restrict__ first, unsigned char* __restrict__ accept, int max_iter,
+                         BatchStat* __restrict__ stat) {
   const int p = blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= P) return;
   double s = 0.0;
@@ -88,6 +89,8 @@ __global__ void k_b_init(int P, const int* __restrict__ rcp, const double* __res
   need_jac[p] = go;
   first[p] = 1;
   accept[p] = 0;
+  stat[p + 1] = BatchStat{s, s, 0, PSBA_ITER_CONTINUE, go ? 1 : 0, 0};
+  if (p == 0) stat[0].active = go ? P : 0;
 }
 
 // mu0 = 1e-3 max diag(J^T J) on a problem's first iteration (one block per problem)
@@ -156,7 +159,7 @@ __global__ void k_b_control(int P, const int* __restrict__ rcp, const double* __
                             int* __restrict__ nu, int* __restrict__ itno, int* __restrict__ code,
                             unsigned char* __restrict__ active, unsigned char* __restrict__ need_jac,
                             unsigned char* __restrict__ accept, int max_iter, double rel_tol,
-                            int* __restrict__ active_count) {
+                            const double* __restrict__ init_err, BatchStat* __restrict__ stat) {
   const int p = blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= P) return;
   accept[p] = 0;
@@ -213,7 +216,8 @@ __global__ void k_b_control(int P, const int* __restrict__ rcp, const double* __
       need_jac[p] = 1;
     }
   }
-  if (still) atomicAdd(active_count, 1);
+  stat[p + 1] = BatchStat{cost[p], init_err[p], itno[p], still ? PSBA_ITER_CONTINUE : code[p], still ? 1 : 0, 0};
+  if (still) atomicAdd(&stat[0].active, 1);
 }
 
 // p += dp on accepted problems; p_L2 = |p|^2 for the next |dp| test
@@ -243,7 +247,7 @@ __global__ __launch_bounds__(256) void k_b_commit(double* __restrict__ cams, dou
   if (threadIdx.x == 0) p_L2[p] = acc[0];
 }
 
-__global__ void k_b_zero(int* x) { *x = 0; }
+__global__ void k_b_zero(BatchStat* st) { st->active = 0; }
 
 template <int CNP>
 static void run_b_resid(psba_batch* b, bool trial) {
@@ -258,7 +262,8 @@ int launch_b_lm(psba_batch* b, int stage) {
     case BLM_INIT:
       if (b->n_rchunk > 0) PSBA_DISPATCH_CNP(b->cnp, run_b_resid, b, false);
       hipLaunchKernelGGL(k_b_init, dim3(pb), dim3(64), 0, b->stream, b->P, b->rcp, b->rpart, b->cost, b->init_err,
-                         b->mu, b->nu, b->itno, b->code, b->active, b->need_jac, b->first, b->accept, b->max_iter);
+                         b->mu, b->nu, b->itno, b->code, b->active, b->need_jac, b->first, b->accept, b->max_iter,
+                         b->stat);
       break;
     case BLM_MU0:
       hipLaunchKernelGGL(k_b_mu0, dim3(b->P), dim3(256), 0, b->stream, b->uvdiag, b->cam_off, b->pt_off, b->cnp, b->N,
@@ -272,10 +277,10 @@ int launch_b_lm(psba_batch* b, int stage) {
       if (b->n_rchunk > 0) PSBA_DISPATCH_CNP(b->cnp, run_b_resid, b, true);
       break;
     case BLM_CONTROL:
-      hipLaunchKernelGGL(k_b_zero, dim3(1), dim3(1), 0, b->stream, b->active_count);
+      hipLaunchKernelGGL(k_b_zero, dim3(1), dim3(1), 0, b->stream, b->stat);
       hipLaunchKernelGGL(k_b_control, dim3(pb), dim3(64), 0, b->stream, b->P, b->rcp, b->rpart, b->fok, b->dp2,
                          b->rhoden, b->p_L2, b->cost, b->tcost, b->mu, b->nu, b->itno, b->code, b->active,
-                         b->need_jac, b->accept, b->max_iter, b->rel_tol, b->active_count);
+                         b->need_jac, b->accept, b->max_iter, b->rel_tol, b->init_err, b->stat);
       break;
     case BLM_COMMIT:
       hipLaunchKernelGGL(k_b_commit, dim3(b->P), dim3(256), 0, b->stream, b->cams, b->pts, b->dp, b->cam_off,
