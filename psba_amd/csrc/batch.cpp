@@ -29,12 +29,6 @@ int bfail(int code, const char* fmt, ...) {
 long pad64(long n) { return (n + kNB - 1) / kNB * kNB; }
 
 // index of one problem in its own numbering (the same rules as psba_set_index)
-struct LocalIndex {
-  std::vector<int> pt_ptr, cam_obs, cb, ce, ccp, sb, se, psp, lbpair;
-  std::vector<int2> items;
-  int npairs = 0;
-};
-
 int build_local(const psba_batch::Staged& s, int C, LocalIndex& L) {
   const int nc = s.nc, np = s.np, no = s.no;
   const int* iidx = s.iidx.data();
@@ -103,7 +97,7 @@ int build_local(const psba_batch::Staged& s, int C, LocalIndex& L) {
 // sizes that fix the arena layout
 struct Sizes {
   long P = 0, NC = 0, NP = 0, NO = 0, N = 0, items = 0, n_schunk = 0, n_cchunk = 0, nlb = 0, npairs = 0,
-       n_rchunk = 0, S_len = 0, V_len = 0;
+       n_rchunk = 0, S_len = 0, V_len = 0, T_max = 0;
 };
 
 // arena layout: every view 256-byte aligned; with base == nullptr only the
@@ -169,6 +163,8 @@ size_t carve(psba_batch* t, const Sizes& z, int C, size_t R, char* base, size_t*
   cv.take(&t->vec_off, z.P + 1);
   cv.take(&t->S_off, z.P + 1);
   cv.take(&t->all_on, z.P);
+  cv.take(&t->chol_fo, std::max(0L, z.T_max - 1) * (z.P + 1));
+  cv.take(&t->chol_bo, z.T_max * (z.P + 1));
   if (input_end) *input_end = cv.off;
   cv.take(&t->cams, z.N);
   cv.take(&t->pts, 3 * z.NP);
@@ -230,6 +226,7 @@ Sizes problem_sizes(int C, int nc, int np, int no, long pair_items, long npairs)
   const long np_ = pad64(z.N);
   z.S_len = np_ * np_;
   z.V_len = np_;
+  z.T_max = np_ / kNB;
   return z;
 }
 
@@ -290,12 +287,21 @@ int launch_round(psba_batch* b) {
   return PSBA_OK;
 }
 
+// wait for the rounds queued behind the last check and read the solve's time
+int settle(psba_batch* b) {
+  if (!b->stop_pending) return PSBA_OK;
+  PSBA_HIP_TRY(hipEventSynchronize(b->ev_stop));
+  float t = 0;
+  PSBA_HIP_TRY(hipEventElapsedTime(&t, b->ev_start, b->ev_stop));
+  b->last_ms = t;
+  b->stop_pending = false;
+  return PSBA_OK;
+}
+
+// the solve has stopped; rounds still queued behind the check end as no-ops
 int finish(psba_batch* b) {
   PSBA_HIP_TRY(hipEventRecord(b->ev_stop, b->stream));
-  PSBA_HIP_TRY(hipEventSynchronize(b->ev_stop));
-  float ms = 0;
-  PSBA_HIP_TRY(hipEventElapsedTime(&ms, b->ev_start, b->ev_stop));
-  b->last_ms = ms;
+  b->stop_pending = true;
   b->rounds = b->checked;
   b->running = false;
   b->finished = true;
@@ -341,6 +347,8 @@ int check_problem(const psba_batch* b, int n_cams, int n_pts, int n_obs, const i
 }  // namespace
 
 namespace psba {
+
+int build_local_index(const psba_batch::Staged& s, int cnp, LocalIndex& L) { return build_local(s, cnp, L); }
 
 int batch_add_staged(psba_batch* b, psba_batch::Staged&& s) {
   b->staged.push_back(std::move(s));
@@ -511,9 +519,12 @@ int psba_batch_commit(psba_batch* b) {
   // per-problem index in its own numbering
   std::vector<LocalIndex> loc(P);
   std::vector<int> rcs(P, PSBA_OK);
-  parallel_for(P, nt, [&](int p) { rcs[p] = build_local(b->staged[p], C, loc[p]); });
+  parallel_for(P, nt, [&](int p) {
+    if (!b->staged[p].index) rcs[p] = build_local(b->staged[p], C, loc[p]);
+  });
   for (int p = 0; p < P; ++p)
     if (rcs[p] != PSBA_OK) return rcs[p];
+  auto L_of = [&](int p) -> const LocalIndex& { return b->staged[p].index ? *b->staged[p].index : loc[p]; };
 
   // offsets
   std::vector<int> cam_off(P + 1, 0), pt_off(P + 1, 0), obs_off(P + 1, 0), npad(P), vec_off(P + 1, 0);
@@ -521,7 +532,7 @@ int psba_batch_commit(psba_batch* b) {
   std::vector<long> item_off(P + 1, 0), S_off(P + 1, 0);
   for (int p = 0; p < P; ++p) {
     const auto& s = b->staged[p];
-    const auto& L = loc[p];
+    const auto& L = L_of(p);
     const long lim = 0x7fffffffL;
     if ((long)obs_off[p] + s.no > lim || item_off[p] + (long)L.items.size() > lim ||
         3L * (pt_off[p] + s.np) + (long)C * (cam_off[p] + s.nc) > lim)
@@ -553,6 +564,7 @@ int psba_batch_commit(psba_batch* b) {
   z.n_rchunk = rch_off[P];
   z.S_len = S_off[P];
   z.V_len = vec_off[P];
+  for (int p = 0; p < P; ++p) z.T_max = std::max(z.T_max, (long)(npad[p] / kNB));
 
   // device arena (grows only) and pinned staging image of its input part
   const size_t R = b->mixed() ? sizeof(float) : sizeof(double);
@@ -585,6 +597,7 @@ int psba_batch_commit(psba_batch* b) {
     b->staging_bytes = in_bytes;
   }
   if (2 * (size_t)(P + 1) > b->h_stat_cap) {
+    PSBA_HIP_TRY(hipStreamSynchronize(b->stream));  // a trailing round may still write it
     if (b->h_stat) (void)hipHostFree(b->h_stat);
     b->h_stat = nullptr;
     b->h_stat_cap = 0;
@@ -600,7 +613,7 @@ int psba_batch_commit(psba_batch* b) {
   // fill the staging image with the global arrays
   parallel_for(P, nt, [&](int p) {
     const auto& s = b->staged[p];
-    const auto& L = loc[p];
+    const auto& L = L_of(p);
     const int co = cam_off[p], po = pt_off[p], oo = obs_off[p];
     std::copy(s.K.begin(), s.K.end(), hv.K + 5L * co);
     std::copy(s.q0.begin(), s.q0.end(), hv.q0 + 4L * co);
@@ -667,6 +680,27 @@ int psba_batch_commit(psba_batch* b) {
   hv.obs_off[P] = obs_off[P];
   hv.vec_off[P] = vec_off[P];
   hv.S_off[P] = S_off[P];
+  // task tables of the phase-by-phase dense solve
+  const int Tm = (int)z.T_max;
+  std::vector<int> Gf(std::max(0, Tm - 1), 0), Gb(Tm, 0);
+  for (int k = 0; k + 1 < Tm; ++k) {
+    int* fo = hv.chol_fo + (size_t)k * (P + 1);
+    fo[0] = 0;
+    for (int p = 0; p < P; ++p) {
+      const int m = npad[p] / kNB - 1 - k;
+      fo[p + 1] = fo[p] + (m >= 1 ? m * (m + 1) / 2 : 0);
+    }
+    Gf[k] = fo[P];
+  }
+  for (int s = 0; s < Tm; ++s) {
+    int* bo = hv.chol_bo + (size_t)s * (P + 1);
+    bo[0] = 0;
+    for (int p = 0; p < P; ++p) {
+      const int T = npad[p] / kNB;
+      bo[p + 1] = bo[p] + ((T >= 2 && T > s) ? (s == 0 ? 1 : T - s) : 0);
+    }
+    Gb[s] = bo[P];
+  }
 
   carve(b, z, C, R, b->arena);
   b->input_bytes = in_bytes;
@@ -686,6 +720,10 @@ int psba_batch_commit(psba_batch* b) {
   b->h_cam_off = cam_off;
   b->h_pt_off = pt_off;
   b->h_obs_off = obs_off;
+  b->T_max = Tm;
+  b->chol_phased = Tm >= kPhasedMinT;
+  b->h_Gf = Gf;
+  b->h_Gb = Gb;
 
   // one upload; the dense blocks start as zeros so that untouched upper parts
   // are finite
@@ -710,6 +748,7 @@ int psba_batch_launch(psba_batch* b, const psba_opts* o) {
   // doublings, so each iteration takes at most 31 rounds
   b->max_rounds = b->max_iter * 31 + 2;
   PSBA_TRY(capture_round(b));
+  b->stop_pending = false;  // an unread time of the previous solve is dropped
   PSBA_HIP_TRY(hipEventRecord(b->ev_start, b->stream));
   PSBA_HIP_TRY(hipMemcpyAsync(b->cams, b->cams0, sizeof(double) * (size_t)b->N, hipMemcpyDeviceToDevice, b->stream));
   PSBA_HIP_TRY(
@@ -753,6 +792,7 @@ int psba_batch_result(psba_batch* b, int slot, int* code, int* itno, double* ini
   if (!b->finished) return bfail(PSBA_ERR_STATE, "no finished solve");
   if (slot < 0 || slot >= b->P) return bfail(PSBA_ERR_ARG, "slot %d out of range", slot);
   PSBA_HIP_TRY(hipSetDevice(b->device));
+  PSBA_TRY(settle(b));
   const BatchStat& st = batch_final_stat(b)[slot + 1];
   if (code) *code = st.code;
   if (itno) *itno = st.itno;
@@ -771,6 +811,7 @@ int psba_batch_result(psba_batch* b, int slot, int* code, int* itno, double* ini
 int psba_batch_elapsed(psba_batch* b, double* ms, int* rounds) {
   if (!b) return bfail(PSBA_ERR_ARG, "null batch");
   if (!b->finished) return bfail(PSBA_ERR_STATE, "no finished solve");
+  PSBA_TRY(settle(b));
   if (ms) *ms = b->last_ms;
   if (rounds) *rounds = b->rounds;
   return PSBA_OK;
@@ -781,6 +822,7 @@ int psba_batch_autotune(psba_batch* b, int* obs_block, int* pt_block) {
   if (!b->committed) return bfail(PSBA_ERR_STATE, "batch not committed");
   if (b->running) return bfail(PSBA_ERR_STATE, "a solve is in flight");
   PSBA_HIP_TRY(hipSetDevice(b->device));
+  PSBA_TRY(settle(b));
   // state as at the start of a solve: every problem needs its Jacobian
   b->max_iter = 100;
   PSBA_HIP_TRY(hipMemcpyAsync(b->cams, b->cams0, sizeof(double) * (size_t)b->N, hipMemcpyDeviceToDevice, b->stream));

@@ -10,10 +10,21 @@
 
 #include <hip/hip_runtime.h>
 
+#include <memory>
 #include <vector>
 
 #include "common.hpp"
 #include "psba_batch.h"
+
+namespace psba {
+// index of one problem in its own numbering: point CSR, camera-ordered
+// observations, camera chunks, camera-pair items and chunks, lower-block pairs
+struct LocalIndex {
+  std::vector<int> pt_ptr, cam_obs, cb, ce, ccp, sb, se, psp, lbpair;
+  std::vector<int2> items;
+  int npairs = 0;
+};
+}  // namespace psba
 
 // Per-problem outcome, copied to pinned host memory at the end of every round
 // (entry 0 is a header whose `active` field counts the problems still running).
@@ -33,6 +44,7 @@ struct psba_batch {
     std::vector<double> K, meas, q0, cams, pts;
     std::vector<int> iidx, jidx;
     std::vector<unsigned char> fix;
+    std::shared_ptr<psba::LocalIndex> index;  // built ahead of the commit (optional)
   };
   std::vector<Staged> staged;
 
@@ -66,6 +78,13 @@ struct psba_batch {
   int *cam_off = nullptr, *pt_off = nullptr, *obs_off = nullptr, *npad = nullptr, *vec_off = nullptr;
   long* S_off = nullptr;
   std::vector<int> h_cam_off, h_pt_off, h_obs_off;
+  // dense solve: one workgroup per problem, or (chol_phased) one launch per
+  // phase with the task tables chol_fo ((T_max-1) x (P+1)) and chol_bo
+  // (T_max x (P+1)) and per-launch grid sizes h_Gf / h_Gb
+  int T_max = 0;
+  bool chol_phased = false;
+  int *chol_fo = nullptr, *chol_bo = nullptr;
+  std::vector<int> h_Gf, h_Gb;
 
   // work (R = double on the plain path, float on the mixed path)
   void *e = nullptr, *JA = nullptr, *JB = nullptr, *W = nullptr;
@@ -99,6 +118,7 @@ struct psba_batch {
   hipEvent_t ev_round[kDepth] = {nullptr, nullptr};
   int launched = 0, checked = 0;
   hipEvent_t ev_start = nullptr, ev_stop = nullptr;
+  bool stop_pending = false;  // ev_stop recorded, elapsed time not read yet
   double last_ms = 0;
 
   // launch shapes chosen by psba_batch_autotune
@@ -110,6 +130,7 @@ struct psba_batch {
 namespace psba {
 
 constexpr int kResidChunk = 1024;  // observations per residual chunk of a batch
+constexpr int kPhasedMinT = 3;     // dense tiles from which a batch's solve runs phase by phase
 
 // stage launchers for batches (defined next to the kernels they launch)
 int launch_b_jacobian(psba_batch* b);       // e, JA, JB, W, U, V, g, uvdiag for need_jac problems
@@ -130,6 +151,7 @@ int batch_advance(psba_batch* b, bool blocking, void (*cb)(void*, const BatchSta
                   void* ctx = nullptr);
 const BatchStat* batch_final_stat(const psba_batch* b);  // after the solve finished
 long long staged_bytes(int cnp, int precision, const psba_batch::Staged& s);
+int build_local_index(const psba_batch::Staged& s, int cnp, LocalIndex& L);
 int batch_release(psba_batch* b);                             // free the arena (idle batch)
 
 }  // namespace psba

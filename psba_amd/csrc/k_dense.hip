@@ -949,9 +949,184 @@ __global__ __launch_bounds__(256) void k_b_chol_solve(double* __restrict__ S, do
   if (tid == 0) fok[p] = bad ? 0.0 : 1.0;
 }
 
+// Batches whose largest problem has several tiles run the same tile tasks with
+// one launch per phase instead, every task of a phase (over all problems) on
+// its own workgroup: the kernel boundary is the barrier between phases, so
+// workgroups never wait for each other.  Task tables: phase k of problem p
+// owns workgroups fo[k][p] .. fo[k][p+1]-1 (forward), back-substitution step s
+// owns bo[s][p] .. bo[s][p+1]-1.
+
+// p with off[p] <= t < off[p+1]
+__device__ __forceinline__ int find_owner(const int* __restrict__ off, int P, int t) {
+  int lo = 0, hi = P - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (off[mid] <= t)
+      lo = mid;
+    else
+      hi = mid - 1;
+  }
+  return lo;
+}
+
+// phase -1: right-hand side, factor of tile (0,0), y_0; one-tile problems
+// also solve x_0 here
+__global__ __launch_bounds__(256) void k_b_chol_init(double* __restrict__ S, double* __restrict__ Lf,
+                                                     const long* __restrict__ S_off, const int* __restrict__ npad,
+                                                     const int* __restrict__ vec_off,
+                                                     const int* __restrict__ cam_off, int cnp,
+                                                     const double* __restrict__ ea, double* __restrict__ bw,
+                                                     double* __restrict__ yv, double* __restrict__ xs,
+                                                     double* __restrict__ dp, double* __restrict__ fok,
+                                                     const unsigned char* __restrict__ run) {
+  __shared__ CholSmem sm;
+  const int p = blockIdx.x;
+  if (!run[p]) return;
+  const int tid = threadIdx.x;
+  const int ld = npad[p], T = ld / NB, n = (cam_off[p + 1] - cam_off[p]) * cnp;
+  const double* A = S + S_off[p];
+  double* L = Lf + S_off[p];
+  const double* b = ea + (size_t)cam_off[p] * cnp;
+  double* bwp = bw + vec_off[p];
+  for (int q = tid; q < T * NB; q += blockDim.x) bwp[q] = (q < n) ? b[q] : 0.0;
+  for (int f = tid; f < NB * NB; f += blockDim.x) {
+    const int r = f >> 6, c = f & 63;
+    sm.Y[r * TP + c] = A[(size_t)r * ld + c];
+  }
+  if (tid < NB) sm.part[0][tid] = (tid < n) ? b[tid] : 0.0;
+  __syncthreads();
+  const int bad = tile_potrf(sm);
+  store_tile(sm.Y, L, ld);
+  tile_forward(sm.Y, sm.part[0], yv + vec_off[p], sm.w);
+  __syncthreads();
+  if (T == 1) {
+    const double br = sm.w[tid & 63];
+    __syncthreads();
+    tile_backward(sm.Y, br, xs + vec_off[p], sm.v);
+    __syncthreads();
+    if (tid < n) dp[(size_t)cam_off[p] * cnp + tid] = sm.v[tid];
+  }
+  if (tid == 0) fok[p] = bad ? 0.0 : 1.0;
+}
+
+// forward phase k: one tile task (i, j), j >= k + 1, per workgroup
+__global__ __launch_bounds__(256) void k_b_chol_phase(int k, const int* __restrict__ fo, int P,
+                                                      double* __restrict__ S, double* __restrict__ Lf,
+                                                      const long* __restrict__ S_off, const int* __restrict__ npad,
+                                                      const int* __restrict__ vec_off, double* __restrict__ bw,
+                                                      double* __restrict__ yv, double* __restrict__ fok,
+                                                      const unsigned char* __restrict__ run) {
+  __shared__ CholSmem sm;
+  const int p = find_owner(fo, P, blockIdx.x);
+  if (!run[p]) return;
+  const int t = blockIdx.x - fo[p];
+  const int tid = threadIdx.x;
+  const int ld = npad[p], T = ld / NB, m = T - 1 - k;
+  double* A = S + S_off[p];
+  double* L = Lf + S_off[p];
+  double* bwp = bw + vec_off[p];
+  double* yvp = yv + vec_off[p];
+  auto tile = [&](double* base, int i, int j) { return base + (size_t)i * NB * ld + (size_t)j * NB; };
+  load_lower(sm.P, tile(L, k, k), ld);
+  if (tid < NB) sm.v[tid] = yvp[(size_t)k * NB + tid];
+  __syncthreads();
+  if (tid < NB) sm.rd[tid] = 1.0 / sm.P[tid * TP + tid];
+  __syncthreads();
+  int a = 0, cum = 0;
+  while (cum + (m - a) <= t) { cum += m - a; ++a; }
+  const int j = k + 1 + a, i = j + (t - cum);
+  dvec4 acc[2][2];
+  acc_load(acc, tile(A, i, j), ld);
+  trsm_rows(sm.X, tile(A, i, k), ld, sm.P, sm.rd);
+  if (i != j) trsm_rows(sm.Y, tile(A, j, k), ld, sm.P, sm.rd);
+  __syncthreads();
+  acc_update(acc, sm.X, (i != j) ? sm.Y : sm.X, i == j);
+  if (j == k + 1) {
+    store_tile(sm.X, tile(L, i, k), ld);
+    const int r = tid >> 2, pp = tid & 3;
+    double s = 0.0;
+#pragma unroll
+    for (int c = 0; c < 16; ++c) s += sm.X[r * TP + 16 * pp + c] * sm.v[16 * pp + c];
+    s += __shfl_xor(s, 1, 64);
+    s += __shfl_xor(s, 2, 64);
+    if (pp == 0) {
+      const double bi = bwp[(size_t)i * NB + r] - s;
+      bwp[(size_t)i * NB + r] = bi;
+      if (i == j) sm.part[0][r] = bi;
+    }
+  }
+  if (i == j && j == k + 1) {
+    __syncthreads();
+    acc_store(acc, sm.Y, TP);
+    __syncthreads();
+    const int bad = tile_potrf(sm);
+    store_tile(sm.Y, tile(L, i, i), ld);
+    tile_forward(sm.Y, sm.part[0], yvp + (size_t)i * NB, sm.w);
+    if (bad && tid == 0) fok[p] = 0.0;
+  } else {
+    acc_store(acc, tile(A, i, j), ld);
+  }
+}
+
+// back-substitution step s (block row k = T - 1 - s of each problem with
+// T >= 2): workgroup 0 of a problem applies x_{k+1} to r_k and solves x_k,
+// workgroup 1 + i applies x_{k+1} to r_i, i < k
+__global__ __launch_bounds__(256) void k_b_chol_back(int s, const int* __restrict__ bo, int P,
+                                                     const double* __restrict__ Lf, const long* __restrict__ S_off,
+                                                     const int* __restrict__ npad, const int* __restrict__ vec_off,
+                                                     const int* __restrict__ cam_off, int cnp,
+                                                     double* __restrict__ yv, double* __restrict__ xs,
+                                                     double* __restrict__ dp, const unsigned char* __restrict__ run) {
+  __shared__ CholSmem sm;
+  const int p = find_owner(bo, P, blockIdx.x);
+  if (!run[p]) return;
+  const int idx = blockIdx.x - bo[p];
+  const int tid = threadIdx.x;
+  const int ld = npad[p], T = ld / NB, k = T - 1 - s, n = (cam_off[p + 1] - cam_off[p]) * cnp;
+  const double* L = Lf + S_off[p];
+  double* yvp = yv + vec_off[p];
+  double* xsp = xs + vec_off[p];
+  auto tile = [&](const double* base, int i, int j) { return base + (size_t)i * NB * ld + (size_t)j * NB; };
+  if (s > 0 && tid < NB) sm.w[tid] = xsp[(size_t)(k + 1) * NB + tid];
+  __syncthreads();
+  if (idx == 0) {
+    load_lower(sm.P, tile(L, k, k), ld);
+    double br = yvp[(size_t)k * NB + (tid & 63)];
+    if (s > 0) br -= tile_tmatvec(sm, tile(L, k + 1, k), ld, sm.w);
+    __syncthreads();
+    tile_backward(sm.P, br, xsp + (size_t)k * NB, sm.v);
+    __syncthreads();
+    if (tid < NB) {
+      const int q = k * NB + tid;
+      if (q < n) dp[(size_t)cam_off[p] * cnp + q] = sm.v[tid];
+    }
+  } else {
+    const int i = idx - 1;
+    const double t = tile_tmatvec(sm, tile(L, k + 1, i), ld, sm.w);
+    if (tid < NB) yvp[(size_t)i * NB + tid] -= t;
+  }
+}
+
 int launch_b_factor_solve(psba_batch* b) {
-  hipLaunchKernelGGL(k_b_chol_solve, dim3(b->P), dim3(256), 0, b->stream, b->S, b->Lf, b->S_off, b->npad, b->vec_off,
+  if (!b->chol_phased) {
+    hipLaunchKernelGGL(k_b_chol_solve, dim3(b->P), dim3(256), 0, b->stream, b->S, b->Lf, b->S_off, b->npad,
+                       b->vec_off, b->cam_off, b->cnp, b->ea, b->bw, b->yv, b->xs, b->dp, b->fok, b->active);
+    PSBA_LAUNCH_TRY();
+    return PSBA_OK;
+  }
+  const int P = b->P;
+  hipLaunchKernelGGL(k_b_chol_init, dim3(P), dim3(256), 0, b->stream, b->S, b->Lf, b->S_off, b->npad, b->vec_off,
                      b->cam_off, b->cnp, b->ea, b->bw, b->yv, b->xs, b->dp, b->fok, b->active);
+  for (int k = 0; k + 1 < b->T_max; ++k)
+    if (b->h_Gf[k] > 0)
+      hipLaunchKernelGGL(k_b_chol_phase, dim3(b->h_Gf[k]), dim3(256), 0, b->stream, k,
+                         b->chol_fo + (size_t)k * (P + 1), P, b->S, b->Lf, b->S_off, b->npad, b->vec_off, b->bw,
+                         b->yv, b->fok, b->active);
+  for (int s = 0; s < b->T_max; ++s)
+    if (b->h_Gb[s] > 0)
+      hipLaunchKernelGGL(k_b_chol_back, dim3(b->h_Gb[s]), dim3(256), 0, b->stream, s,
+                         b->chol_bo + (size_t)s * (P + 1), P, b->Lf, b->S_off, b->npad, b->vec_off, b->cam_off,
+                         b->cnp, b->yv, b->xs, b->dp, b->active);
   PSBA_LAUNCH_TRY();
   return PSBA_OK;
 }

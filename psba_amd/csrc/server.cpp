@@ -7,10 +7,15 @@
 // round copies to pinned memory), copies their parameters back on the lane's
 // copy stream, and admits queued requests into idle lanes within the budget.
 #include <algorithm>
+#include <atomic>
 #include <chrono>
+#include <condition_variable>
 #include <cstdarg>
 #include <cstring>
 #include <deque>
+#include <functional>
+#include <mutex>
+#include <thread>
 #include <unordered_map>
 #include <vector>
 
@@ -41,6 +46,8 @@ enum ReqState { RQ_QUEUED = 0, RQ_RUNNING, RQ_COPYING, RQ_DONE };
 struct Request {
   long long id = 0;
   psba_batch::Staged data;
+  std::atomic<int> ready{0};  // index built (by a worker) and bytes known
+  int build_rc = PSBA_OK;
   long long bytes = 0;
   int cls = 0;
   int state = RQ_QUEUED;
@@ -75,6 +82,47 @@ int size_class(int cnp, int nc) {
   return T <= 2 ? 0 : (T <= 8 ? 1 : 2);
 }
 
+// host workers that build each request's index while earlier requests run
+struct Workers {
+  std::vector<std::thread> th;
+  std::mutex m;
+  std::condition_variable cv;
+  std::deque<std::function<void()>> jobs;
+  bool stop = false;
+  void start(int n) {
+    for (int k = 0; k < n; ++k)
+      th.emplace_back([this] {
+        for (;;) {
+          std::function<void()> job;
+          {
+            std::unique_lock<std::mutex> lk(m);
+            cv.wait(lk, [this] { return stop || !jobs.empty(); });
+            if (jobs.empty()) return;
+            job = std::move(jobs.front());
+            jobs.pop_front();
+          }
+          job();
+        }
+      });
+  }
+  void push(std::function<void()> f) {
+    {
+      std::lock_guard<std::mutex> lk(m);
+      jobs.push_back(std::move(f));
+    }
+    cv.notify_one();
+  }
+  void join() {
+    {
+      std::lock_guard<std::mutex> lk(m);
+      stop = true;
+    }
+    cv.notify_all();
+    for (auto& t : th) t.join();
+    th.clear();
+  }
+};
+
 }  // namespace
 
 struct psba_server {
@@ -87,6 +135,7 @@ struct psba_server {
   long long next_id = 1;
   psba_server_stats st{};
   psba_opts opts{};
+  Workers workers;
 };
 
 namespace {
@@ -210,32 +259,43 @@ int service_lane(psba_server* s, Lane& L) {
 
 // form and launch a batch on an idle lane; returns 1 if one was launched
 int admit(psba_server* s, Lane& L) {
-  // requests larger than the whole budget can never run
-  while (!s->queue.empty()) {
-    Request& h = s->reqs[s->queue.front()];
-    if (h.bytes <= s->d.hbm_budget) break;
-    h.state = RQ_DONE;
-    h.code = PSBA_ERR_NOMEM;
-    h.t_done = now_s();
-    s->done_ids.push_back(h.id);
-    s->queue.pop_front();
-    s->st.rejected++;
+  // the oldest request whose index is built; requests that can never run
+  // (index failure, larger than the whole budget) complete with an error
+  Request* head = nullptr;
+  for (auto it = s->queue.begin(); it != s->queue.end();) {
+    Request& q = s->reqs[*it];
+    if (!q.ready.load(std::memory_order_acquire)) {
+      ++it;
+      continue;
+    }
+    if (q.build_rc != PSBA_OK || q.bytes > s->d.hbm_budget) {
+      q.state = RQ_DONE;
+      q.code = q.build_rc != PSBA_OK ? q.build_rc : PSBA_ERR_NOMEM;
+      q.t_done = now_s();
+      s->done_ids.push_back(q.id);
+      s->st.rejected++;
+      it = s->queue.erase(it);
+      continue;
+    }
+    head = &q;
+    break;
   }
-  if (s->queue.empty()) return 0;
+  if (!head) return 0;
   long long avail = s->d.hbm_budget - (held(s) - psba_batch_bytes(L.b));
-  const int cls = s->reqs[s->queue.front()].cls;
-  if (s->reqs[s->queue.front()].bytes > avail) {
+  const int cls = head->cls;
+  if (head->bytes > avail) {
     // give back arenas of the other idle lanes, then wait for running ones
     for (auto& o : s->lanes)
       if (&o != &L && !o.busy && psba_batch_bytes(o.b) > 0) PSBA_TRY(batch_release(o.b));
     avail = s->d.hbm_budget - (held(s) - psba_batch_bytes(L.b));
-    if (s->reqs[s->queue.front()].bytes > avail) return 0;
+    if (head->bytes > avail) return 0;
   }
   std::vector<long long> pick;
   long long bytes = 0;
   for (auto it = s->queue.begin(); it != s->queue.end() && (int)pick.size() < s->d.max_batch;) {
     Request& q = s->reqs[*it];
-    if (q.cls == cls && bytes + q.bytes <= avail) {
+    if (q.ready.load(std::memory_order_acquire) && q.build_rc == PSBA_OK && q.cls == cls &&
+        bytes + q.bytes <= avail) {
       pick.push_back(*it);
       bytes += q.bytes;
       it = s->queue.erase(it);
@@ -292,6 +352,7 @@ int psba_server_create(const psba_server_desc* d, psba_server** out) {
   s->opts.max_iter = s->d.max_iter;
   s->opts.rel_tol = s->d.rel_tol;
   s->lanes.resize(s->d.lanes);
+  s->workers.start((int)std::max(1u, std::min(8u, std::thread::hardware_concurrency() / 2)));
   for (auto& L : s->lanes) {
     psba_batch_desc bd{d->cnp, d->precision, d->device, 0};
     int rc = psba_batch_create(&bd, &L.b);
@@ -308,6 +369,7 @@ int psba_server_create(const psba_server_desc* d, psba_server** out) {
 
 int psba_server_destroy(psba_server* s) {
   if (!s) return PSBA_OK;
+  s->workers.join();
   for (auto& L : s->lanes) {
     if (L.b) {
       (void)hipSetDevice(L.b->device);
@@ -344,8 +406,9 @@ int psba_server_submit(psba_server* s, int n_cams, int n_pts, int n_obs, const d
     if (k > 0 && (iidx[k] < iidx[k - 1] || (iidx[k] == iidx[k - 1] && jidx[k] <= jidx[k - 1])))
       return sfail(PSBA_ERR_ARG, "observations must be point-major and camera-ascending (obs %d)", k);
   }
-  Request q;
-  q.id = s->next_id++;
+  const long long rid = s->next_id++;
+  Request& q = s->reqs[rid];
+  q.id = rid;
   q.t_submit = now_s();
   q.nc = n_cams;
   q.np = n_pts;
@@ -364,11 +427,20 @@ int psba_server_submit(psba_server* s, int n_cams, int n_pts, int n_obs, const d
     d.fix.assign(fixmask, fixmask + (size_t)C * n_cams);
   else
     d.fix.assign((size_t)C * n_cams, 0);
-  q.bytes = staged_bytes(C, s->d.precision, d);
   q.cls = size_class(C, n_cams);
-  *id = q.id;
-  s->queue.push_back(q.id);
-  s->reqs.emplace(q.id, std::move(q));
+  const int prec = s->d.precision;
+  Request* qp = &q;  // element references of the map stay valid
+  s->workers.push([qp, C, prec] {
+    auto idx = std::make_shared<LocalIndex>();
+    qp->build_rc = build_local_index(qp->data, C, *idx);
+    if (qp->build_rc == PSBA_OK) {
+      qp->bytes = psba_problem_bytes(C, prec, qp->data.nc, qp->data.np, qp->data.no, (long long)idx->items.size());
+      qp->data.index = std::move(idx);
+    }
+    qp->ready.store(1, std::memory_order_release);
+  });
+  *id = rid;
+  s->queue.push_back(rid);
   s->st.submitted++;
   return PSBA_OK;
 }

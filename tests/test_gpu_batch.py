@@ -158,3 +158,23 @@ def test_errors(P, gpu):
     with pytest.raises(P.PsbaError):
         b.result(1)
     b.close()
+
+
+def test_mixed_sizes(P, gpu):
+    """One-, two- and many-tile reduced systems in one batch: the batch then
+    runs its dense solve phase by phase (one launch per phase over all
+    problems), a problem alone with one or two tiles runs it in one workgroup;
+    both do the same tile operations in the same order, so the results agree
+    bit for bit.  The large problem also matches its single-problem solve."""
+    scenes = [P.Scene(4, 1004, problem=1, size_class=0), P.Scene(3, SEED, problem=5, n_cams=15),
+              P.Scene(4, 1004, problem=2, size_class=1, n_pts=2000),
+              P.Scene(4, 1004, problem=3, size_class=2, n_pts=2500)]
+    mix = solve_batch(P, scenes)
+    for sc, r in zip(scenes, mix):
+        same(solve_batch(P, [sc])[0], r)
+    big = scenes[-1]
+    c = P.Context.from_scene(big)
+    g = c.levmar(flags=P.FLAG_NO_HANDOFF, **OPTS)
+    c.close()
+    assert (mix[-1]["code"], mix[-1]["itno"]) == (g["code"], g["itno"])
+    assert abs(mix[-1]["final_err"] - g["final_err"]) <= 1e-9 * g["final_err"]
