@@ -59,6 +59,50 @@ def cpu_baseline(scene, kw):
                       "%.1f s, final cost %.10g)" % (r["itno"], len(r["trace"]), dt, r["final_err"])}
 
 
+def mixed_section(P, sc, a, t_plain, rms):
+    """Mixed precision (FP32 Jacobian / W / V / Schur products, FP64 factor and
+    cost) on the same scene, and both paths on the same geometry without gross
+    outliers, where every solve converges to one minimum and the final RMS
+    reprojection errors can be compared."""
+    kw = dict(max_iter=100, flags=P.FLAG_NO_HANDOFF, rel_tol=1e-8)
+
+    def solve_on(c):
+        c.restore()
+        c.itno = 0
+        return c.levmar(**kw)
+
+    def timed(c, steps, warmup):
+        c.snapshot()
+        for _ in range(warmup):
+            solve_on(c)
+        t0 = time.perf_counter()
+        rr = [solve_on(c) for _ in range(steps)]
+        return (time.perf_counter() - t0) / steps, rr[-1]
+
+    ms = max(1, a.steps // 2)
+    mctx = P.Context.from_scene(sc, precision=P.PREC_MIXED)
+    t_mix, r_mix = timed(mctx, ms, 1)
+    rms_mix = float(np.sqrt(r_mix["final_err"] / (2 * sc.n_obs)))
+    sc0 = P.Scene(2, a.seed, outlier_frac=0.0)
+    t_p0, r_p0 = timed(P.Context.from_scene(sc0), ms, 1)
+    t_m0, r_m0 = timed(P.Context.from_scene(sc0, precision=P.PREC_MIXED), ms, 1)
+    rms_p0 = float(np.sqrt(r_p0["final_err"] / (2 * sc0.n_obs)))
+    rms_m0 = float(np.sqrt(r_m0["final_err"] / (2 * sc0.n_obs)))
+    res = {
+        "solves_per_s": 1.0 / t_mix, "ms_per_solve": 1e3 * t_mix, "speedup_vs_plain": t_plain / t_mix,
+        "lm_iterations": r_mix["itno"], "final_rms_px": rms_mix,
+        "rel_rms_vs_plain": abs(rms_mix - rms) / rms,
+        "outlier_free": {"plain_ms": 1e3 * t_p0, "mixed_ms": 1e3 * t_m0, "speedup": t_p0 / t_m0,
+                         "plain_iterations": r_p0["itno"], "mixed_iterations": r_m0["itno"],
+                         "plain_rms_px": rms_p0, "mixed_rms_px": rms_m0,
+                         "rel_rms": abs(rms_m0 - rms_p0) / rms_p0},
+    }
+    log("mixed: %.3f ms/solve (%.2fx), rms %.6f (rel %.2e); outlier-free: plain %.3f ms, mixed %.3f ms (%.2fx), "
+        "rms rel %.2e" % (1e3 * t_mix, t_plain / t_mix, rms_mix, abs(rms_mix - rms) / rms, 1e3 * t_p0,
+                          1e3 * t_m0, t_p0 / t_m0, abs(rms_m0 - rms_p0) / rms_p0))
+    return res
+
+
 def batch_section(P, seed=1003, n=1024, reps=3):
     """1,024 independent windows (config 3) per launch: every LM stage is one
     launch over all problems, rounds are replayed HIP graphs, launch shapes
@@ -98,7 +142,7 @@ def batch_section(P, seed=1003, n=1024, reps=3):
     return out
 
 
-def stream_section(P, seed=1004, rates=(200, 1000, 2000), duration=2.0, serial_n=100):
+def stream_section(P, seed=1004, rates=(200, 1000, 2000, 4000), duration=2.0, serial_n=100):
     """Open-loop Poisson request stream (config 4 mix: 90% small, 9% medium,
     1% large) through the request loop, against one-at-a-time plain FP64
     solves of the same mix."""
@@ -177,7 +221,10 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--seed", type=int, default=1002)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--sections", default="mixed,batch,stream",
+                    help="comma-separated sections after the headline solves: mixed, batch, stream")
     a = ap.parse_args()
+    sections = set(x for x in a.sections.split(",") if x)
     if a.gpus != 1:
         print("bench.py measures one MI355X only: multi-GPU runs are not part of this build "
               "(run with --gpus 1)", file=sys.stderr)
@@ -198,14 +245,6 @@ def main():
 
     def solve():
         return solve_on(ctx)
-
-    def timed(c, steps, warmup):
-        c.snapshot()
-        for _ in range(warmup):
-            solve_on(c)
-        t0 = time.perf_counter()
-        rr = [solve_on(c) for _ in range(steps)]
-        return (time.perf_counter() - t0) / steps, rr[-1]
 
     for _ in range(a.warmup):
         solve()
@@ -259,33 +298,12 @@ def main():
                        "flops_per_launch": flops, "traffic_source": traffic_src}
     log("schur: %d launches, %.1f us avg, %.2f TFLOP/s (%.1f%% of FP64 peak)" % (
         schur_n, avg_s * 1e6, achieved, 100 * achieved / FP64_PEAK_TFLOPS))
-    # mixed precision (FP32 Jacobian / W / V / Schur products, FP64 factor and
-    # cost) on the same scene, and both paths on the same geometry without
-    # gross outliers, where every solve converges to one minimum and the final
-    # RMS reprojection errors can be compared
-    ms = max(1, a.steps // 2)
-    mctx = P.Context.from_scene(sc, precision=P.PREC_MIXED)
-    t_mix, r_mix = timed(mctx, ms, 1)
-    rms_mix = float(np.sqrt(r_mix["final_err"] / (2 * sc.n_obs)))
-    sc0 = P.Scene(2, a.seed, outlier_frac=0.0)
-    t_p0, r_p0 = timed(P.Context.from_scene(sc0), ms, 1)
-    t_m0, r_m0 = timed(P.Context.from_scene(sc0, precision=P.PREC_MIXED), ms, 1)
-    rms_p0 = float(np.sqrt(r_p0["final_err"] / (2 * sc0.n_obs)))
-    rms_m0 = float(np.sqrt(r_m0["final_err"] / (2 * sc0.n_obs)))
-    out["mixed"] = {
-        "solves_per_s": 1.0 / t_mix, "ms_per_solve": 1e3 * t_mix, "speedup_vs_plain": (dt / a.steps) / t_mix,
-        "lm_iterations": r_mix["itno"], "final_rms_px": rms_mix,
-        "rel_rms_vs_plain": abs(rms_mix - rms) / rms,
-        "outlier_free": {"plain_ms": 1e3 * t_p0, "mixed_ms": 1e3 * t_m0, "speedup": t_p0 / t_m0,
-                         "plain_iterations": r_p0["itno"], "mixed_iterations": r_m0["itno"],
-                         "plain_rms_px": rms_p0, "mixed_rms_px": rms_m0,
-                         "rel_rms": abs(rms_m0 - rms_p0) / rms_p0},
-    }
-    log("mixed: %.3f ms/solve (%.2fx), rms %.6f (rel %.2e); outlier-free: plain %.3f ms, mixed %.3f ms (%.2fx), "
-        "rms rel %.2e" % (1e3 * t_mix, (dt / a.steps) / t_mix, rms_mix, abs(rms_mix - rms) / rms, 1e3 * t_p0,
-                          1e3 * t_m0, t_p0 / t_m0, abs(rms_m0 - rms_p0) / rms_p0))
-    out["batch"] = batch_section(P)
-    out["stream"] = stream_section(P)
+    if "mixed" in sections:
+        out["mixed"] = mixed_section(P, sc, a, dt / a.steps, rms)
+    if "batch" in sections:
+        out["batch"] = batch_section(P)
+    if "stream" in sections:
+        out["stream"] = stream_section(P)
     if not a.no_cpu:
         out["cpu_baseline"] = cpu_baseline(sc, kw)
     print(json.dumps(out), flush=True)

@@ -230,7 +230,11 @@ Sizes problem_sizes(int C, int nc, int np, int no, long pair_items, long npairs)
   return z;
 }
 
+// Destroys the captured rounds.  A finished solve may still have rounds
+// queued behind its last check (they run as no-ops), so the stream is drained
+// first: an executable graph is never destroyed while a launch of it runs.
 int free_graph(psba_batch* b) {
+  if ((b->graph_exec[0] || b->graph_exec[1]) && b->stream) PSBA_HIP_TRY(hipStreamSynchronize(b->stream));
   for (int k = 0; k < 2; ++k) {
     if (b->graph_exec[k]) (void)hipGraphExecDestroy(b->graph_exec[k]);
     if (b->graph[k]) (void)hipGraphDestroy(b->graph[k]);
@@ -261,7 +265,7 @@ int enqueue_round(psba_batch* b, int parity) {
 
 int capture_round(psba_batch* b) {
   if (b->graph_exec[0] && b->graph_iter == b->max_iter && b->graph_tol == b->rel_tol) return PSBA_OK;
-  free_graph(b);
+  PSBA_TRY(free_graph(b));
   for (int k = 0; k < 2; ++k) {
     PSBA_HIP_TRY(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
     int rc = enqueue_round(b, k);
@@ -571,7 +575,7 @@ int psba_batch_commit(psba_batch* b) {
   size_t in_bytes = 0;
   psba_batch hv;  // views into the staging image
   const size_t need = carve(&hv, z, C, R, nullptr, &in_bytes);
-  free_graph(b);
+  PSBA_TRY(free_graph(b));
   PSBA_HIP_TRY(hipEventSynchronize(b->ev_staged));  // the previous upload has left the staging image
   if (need > b->arena_bytes) {
     PSBA_HIP_TRY(hipStreamSynchronize(b->stream));
@@ -868,7 +872,7 @@ int psba_batch_autotune(psba_batch* b, int* obs_block, int* pt_block) {
     }
   }
   b->pt_block = bp;
-  free_graph(b);
+  PSBA_TRY(free_graph(b));
   if (obs_block) *obs_block = bo;
   if (pt_block) *pt_block = bp;
   return PSBA_OK;
