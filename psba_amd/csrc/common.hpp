@@ -112,6 +112,18 @@ __device__ inline void grid_sum_finish(double (&v)[K], double* partials, unsigne
 
 inline unsigned ceil_div(long a, long b) { return (unsigned)((a + b - 1) / b); }
 
+// Workgroups of a launch go to the 8 XCDs round-robin (workgroup b to XCD
+// b % 8), each XCD with its own L2.  A grid of xcd_grid(n) workgroups with
+// item = xcd_item(b, n) gives every XCD one contiguous eighth of an n-item
+// work list (neighbouring items share data); -1 past the end.
+constexpr int kXcds = 8;
+inline unsigned xcd_grid(long n) { return (unsigned)(kXcds * ((n + kXcds - 1) / kXcds)); }
+__device__ inline int xcd_item(int b, int n) {
+  const int per = (n + kXcds - 1) / kXcds;
+  const int it = (b % kXcds) * per + b / kXcds;
+  return it < n ? it : -1;
+}
+
 // call FN<cnp>(args...) for the supported camera models, else return
 // PSBA_ERR_UNSUPPORTED from the enclosing function
 #define PSBA_DISPATCH_CNP(cnp, FN, ...)   \

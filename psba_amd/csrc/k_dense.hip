@@ -331,8 +331,8 @@ __device__ __forceinline__ double bcast(double v, int l) {
 template <int CTRL>
 __device__ __forceinline__ double quad_perm(double v) {
   const long long b = __double_as_longlong(v);
-  const int lo = __builtin_amdgcn_update_dpp(0, (int)b, CTRL, 0xF, 0xF, false);
-  const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), CTRL, 0xF, 0xF, false);
+  const int lo = __builtin_amdgcn_mov_dpp((int)b, CTRL, 0xF, 0xF, true);
+  const int hi = __builtin_amdgcn_mov_dpp((int)(b >> 32), CTRL, 0xF, 0xF, true);
   return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
 }
 __device__ __forceinline__ double quad_xor1(double v) { return quad_perm<0xB1>(v); }  // [1,0,3,2]
@@ -417,8 +417,8 @@ PSBA_OUTLINE void trsm_rows(double* out, const double* A, int ld, const double* 
 template <int N>
 __device__ __forceinline__ double row_bcast(double v) {
   const long long b = __double_as_longlong(v);
-  const int lo = __builtin_amdgcn_update_dpp(0, (int)b, 0x150 + N, 0xF, 0xF, false);
-  const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), 0x150 + N, 0xF, 0xF, false);
+  const int lo = __builtin_amdgcn_mov_dpp((int)b, 0x150 + N, 0xF, 0xF, true);
+  const int hi = __builtin_amdgcn_mov_dpp((int)(b >> 32), 0x150 + N, 0xF, 0xF, true);
   return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
 }
 

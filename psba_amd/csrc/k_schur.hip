@@ -10,6 +10,10 @@
 // summed in FP64.
 #include "context.hpp"
 
+#ifndef PSBA_SCHUR_XCD
+#define PSBA_SCHUR_XCD 1
+#endif
+
 namespace psba {
 
 template <int CNP, typename R>
@@ -17,13 +21,18 @@ __global__ __launch_bounds__(256) void k_schur_chunks(const int2* __restrict__ i
                                                       const int* __restrict__ se, const int* __restrict__ obs_pt,
                                                       const R* __restrict__ W, const double* __restrict__ Vinv,
                                                       R* __restrict__ part, const int* __restrict__ ownr,
-                                                      const unsigned char* __restrict__ run) {
+                                                      const unsigned char* __restrict__ run, int nchunk) {
   constexpr int C2 = CNP * CNP, REC = 3 * CNP;
   constexpr int G = (256 / C2) > 0 ? (256 / C2) : 1;
   __shared__ R sY[kPairChunk * REC];
   __shared__ R sW[kPairChunk * REC];
   __shared__ R sG[G * C2];
+#if PSBA_SCHUR_XCD
+  const int b = xcd_item(blockIdx.x, nchunk);  // one camera range per XCD
+  if (b < 0) return;
+#else
   const int b = blockIdx.x;
+#endif
   if (run && !run[ownr[b]]) return;
   const int beg = sb[b], cnt = se[b] - beg;
   // Y rows: task = (item, row r) -> Y[it][r][0..2] = W_a[r][:] V^-1
@@ -147,9 +156,9 @@ template <int CNP, typename R>
 static void run_schur_t(psba_ctx* c, double mu, const R* W, R* schpart, R* eapart) {
   if (c->n_schunk > 0) {
     probe_begin(c, PSBA_PROBE_SCHUR);
-    hipLaunchKernelGGL((k_schur_chunks<CNP, R>), dim3(c->n_schunk), dim3(256), 0, c->stream, c->pair_items,
-                       c->schunk_begin, c->schunk_end, c->obs_pt, W, c->Vinv, schpart, (const int*)nullptr,
-                       (const unsigned char*)nullptr);
+    hipLaunchKernelGGL((k_schur_chunks<CNP, R>), dim3(PSBA_SCHUR_XCD ? xcd_grid(c->n_schunk) : c->n_schunk), dim3(256),
+                       0, c->stream, c->pair_items, c->schunk_begin, c->schunk_end, c->obs_pt, W, c->Vinv, schpart,
+                       (const int*)nullptr, (const unsigned char*)nullptr, c->n_schunk);
     probe_end(c, PSBA_PROBE_SCHUR);
   }
   hipLaunchKernelGGL((k_schur_finish<CNP, R>), dim3(c->n_lblocks + (c->n_pad - c->n)),
@@ -247,8 +256,9 @@ static void run_b_schur_t(psba_batch* b) {
   const R* W = (const R*)b->W;
   R *schpart = (R*)b->schpart, *eapart = (R*)b->eapart;
   if (b->n_schunk > 0)
-    hipLaunchKernelGGL((k_schur_chunks<CNP, R>), dim3(b->n_schunk), dim3(256), 0, b->stream, b->items, b->schunk_b,
-                       b->schunk_e, b->obs_pt, W, b->Vinv, schpart, b->schunk_prob, b->active);
+    hipLaunchKernelGGL((k_schur_chunks<CNP, R>), dim3(PSBA_SCHUR_XCD ? xcd_grid(b->n_schunk) : b->n_schunk), dim3(256),
+                       0, b->stream, b->items, b->schunk_b, b->schunk_e, b->obs_pt, W, b->Vinv, schpart, b->schunk_prob,
+                       b->active, b->n_schunk);
   if (b->nlb > 0)
     hipLaunchKernelGGL((k_b_schur_finish<CNP, R>), dim3(b->nlb), dim3(CNP * CNP > 128 ? 256 : 128), 0, b->stream,
                        b->lb_prob, b->lb_k, b->lb_l, b->lb_pair, b->psp, schpart, b->U, b->fix, b->mu, b->cam_off,
