@@ -142,6 +142,36 @@ def batch_section(P, seed=1003, n=1024, reps=3):
     return out
 
 
+def index_section(P, sc, reps=5):
+    """Index construction at the headline scene: the host loops against the
+    device build (stable radix sorts and scans), both bit-identical; and
+    generate_idxs from the visibility mask, host and device."""
+    vm = np.zeros((sc.n_pts, sc.n_cams), np.int8)
+    vm[sc.iidx, sc.jidx] = 1
+    c = P.Context(sc.cnp, sc.n_cams, sc.n_pts, sc.n_obs)
+    c.set_index_device(sc.iidx, sc.jidx)  # warm-up (first launches)
+    P.generate_idxs_device(sc.n_cams, sc.n_pts, sc.n_obs, vm)
+
+    def med(fn):
+        t = []
+        for _ in range(reps):
+            t0 = time.perf_counter()
+            fn()
+            t.append(time.perf_counter() - t0)
+        return 1e3 * float(np.median(t))
+
+    out = {"set_index_host_ms": med(lambda: c.set_index(sc.iidx, sc.jidx)),
+           "set_index_device_ms": med(lambda: c.set_index_device(sc.iidx, sc.jidx)),
+           "generate_idxs_host_ms": med(lambda: P.generate_idxs(sc.n_cams, sc.n_pts, sc.n_obs, vm)),
+           "generate_idxs_device_ms": med(lambda: P.generate_idxs_device(sc.n_cams, sc.n_pts, sc.n_obs, vm)),
+           "pair_items": c.info()["n_pair_items"]}
+    c.close()
+    log("index: set_index host %.2f ms, device %.2f ms; generate_idxs host %.2f ms, device %.2f ms" % (
+        out["set_index_host_ms"], out["set_index_device_ms"], out["generate_idxs_host_ms"],
+        out["generate_idxs_device_ms"]))
+    return out
+
+
 def stream_section(P, seed=1004, rates=(200, 1000, 2000, 4000), duration=2.0, serial_n=100):
     """Open-loop Poisson request stream (config 4 mix: 90% small, 9% medium,
     1% large) through the request loop, against one-at-a-time plain FP64
@@ -221,8 +251,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--seed", type=int, default=1002)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--sections", default="mixed,batch,stream",
-                    help="comma-separated sections after the headline solves: mixed, batch, stream")
+    ap.add_argument("--sections", default="mixed,index,batch,stream",
+                    help="comma-separated sections after the headline solves: mixed, index, batch, stream")
     a = ap.parse_args()
     sections = set(x for x in a.sections.split(",") if x)
     if a.gpus != 1:
@@ -300,6 +330,8 @@ def main():
         schur_n, avg_s * 1e6, achieved, 100 * achieved / FP64_PEAK_TFLOPS))
     if "mixed" in sections:
         out["mixed"] = mixed_section(P, sc, a, dt / a.steps, rms)
+    if "index" in sections:
+        out["index"] = index_section(P, sc)
     if "batch" in sections:
         out["batch"] = batch_section(P)
     if "stream" in sections:

@@ -86,6 +86,9 @@ int psba_upload(psba_ctx* ctx, const double* K, const double* impts, const doubl
 /* replaces fill_idxBuffer, PSBA/cl_psba.h:123-126: the device index (point CSR,
  * camera CSR, camera-pair lists) is built from the observation order */
 int psba_set_index(psba_ctx* ctx, const int* iidx, const int* jidx);
+/* the same index built on the device (stable radix sorts and scans of the
+ * observation order); every index array is bit-identical to psba_set_index's */
+int psba_set_index_device(psba_ctx* ctx, const int* iidx, const int* jidx);
 /* addition (no reference counterpart): hold parameters fixed; mask has
  * cnp*n_cams bytes, 1 = fixed.  NULL clears the mask. */
 int psba_set_fixed(psba_ctx* ctx, const unsigned char* mask);
@@ -105,6 +108,12 @@ int psba_device_count(void);
  * restatement.  comm3DIdx may be NULL (it needs nCams*nCams*n3Dpts ints). */
 int psba_generate_idxs(int nCams, int n3Dpts, int n2Dprojs, const char* vmask, int* comm3DIdx,
                        int* comm3DIdxCnt, int* iidx, int* jidx, int* blk_idx);
+/* generate_idxs (PSBA/misc.cpp:178-218) on a device: compaction of the
+ * visibility mask by a scan; iidx, jidx, blk_idx and comm3DIdxCnt (may be
+ * NULL) bit-exact.  The dense comm3DIdx lists are not produced (the device
+ * index replaces them). */
+int psba_generate_idxs_device(int device, int nCams, int n3Dpts, int n2Dprojs, const char* vmask,
+                              int* comm3DIdxCnt, int* iidx, int* jidx, int* blk_idx);
 
 /* ---- drivers ---------------------------------------------------------------- */
 /* replaces levmar, PSBA/levmar.h:6-11.  *itno is the shared iteration counter
@@ -159,6 +168,25 @@ int psba_stage_load_S(psba_ctx* ctx, const double* S, const double* b);
 #define PSBA_BUF_L 13      /* n*n lower factor */
 long psba_buffer_size(const psba_ctx* ctx, int which);
 int psba_read_buffer(psba_ctx* ctx, int which, double* out, long n);
+
+/* device index read-back (ints) */
+#define PSBA_IDX_OBS_PT 0         /* n_obs */
+#define PSBA_IDX_OBS_CAM 1        /* n_obs */
+#define PSBA_IDX_PT_PTR 2         /* n_pts + 1 */
+#define PSBA_IDX_CAM_PTR 3        /* n_cams + 1 */
+#define PSBA_IDX_CAM_OBS 4        /* n_obs: observations of each camera, ascending */
+#define PSBA_IDX_CCHUNK_BEGIN 5   /* camera chunks */
+#define PSBA_IDX_CCHUNK_END 6
+#define PSBA_IDX_CAM_CCHUNK_PTR 7 /* n_cams + 1 */
+#define PSBA_IDX_PAIR_ITEMS 8     /* 2 per item: (obs in camera k, obs in camera l) */
+#define PSBA_IDX_PAIR_K 9
+#define PSBA_IDX_PAIR_L 10
+#define PSBA_IDX_SCHUNK_BEGIN 11  /* pair chunks */
+#define PSBA_IDX_SCHUNK_END 12
+#define PSBA_IDX_PAIR_SCHUNK_PTR 13
+#define PSBA_IDX_LBLOCK_PAIR 14   /* n_cams(n_cams+1)/2 */
+long psba_index_size(const psba_ctx* ctx, int which);
+int psba_read_index(psba_ctx* ctx, int which, int* out, long n);
 
 /* problem/index statistics */
 typedef struct {

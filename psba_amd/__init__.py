@@ -31,6 +31,8 @@ PREC_FP64 = 0
 PREC_MIXED = 1
 
 BUF = dict(EX=0, JA=1, JB=2, U=3, V=4, W=5, G=6, VINV=7, S=9, EA=10, DP=11, EB=12, L=13)
+INDEX = dict(OBS_PT=0, OBS_CAM=1, PT_PTR=2, CAM_PTR=3, CAM_OBS=4, CCHUNK_BEGIN=5, CCHUNK_END=6, CAM_CCHUNK_PTR=7,
+             PAIR_ITEMS=8, PAIR_K=9, PAIR_L=10, SCHUNK_BEGIN=11, SCHUNK_END=12, PAIR_SCHUNK_PTR=13, LBLOCK_PAIR=14)
 
 
 class PsbaError(RuntimeError):
@@ -184,6 +186,11 @@ def lib():
         "psba_server_result": (ct.c_int, [vp, ct.c_longlong, ip, ip, dp, dp, dp, dp, dp, dp]),
         "psba_server_get_stats": (ct.c_int, [vp, ct.POINTER(ServerStats)]),
         "psba_server_completed": (ct.c_int, [vp, ct.POINTER(ct.c_longlong), ct.c_int]),
+        "psba_set_index_device": (ct.c_int, [vp, ip, ip]),
+        "psba_generate_idxs_device": (ct.c_int, [ct.c_int, ct.c_int, ct.c_int, ct.c_int, ct.c_char_p, ip, ip, ip,
+                                                 ip]),
+        "psba_index_size": (ct.c_long, [vp, ct.c_int]),
+        "psba_read_index": (ct.c_int, [vp, ct.c_int, ip, ct.c_long]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -316,6 +323,21 @@ class Context:
         i = np.ascontiguousarray(iidx, dtype=np.int32)
         j = np.ascontiguousarray(jidx, dtype=np.int32)
         _check(lib().psba_set_index(self._h, _iptr(i), _iptr(j)), "psba_set_index")
+
+    def set_index_device(self, iidx, jidx):
+        """The same index built on the device (bit-identical to set_index)."""
+        i = np.ascontiguousarray(iidx, dtype=np.int32)
+        j = np.ascontiguousarray(jidx, dtype=np.int32)
+        _check(lib().psba_set_index_device(self._h, _iptr(i), _iptr(j)), "psba_set_index_device")
+
+    def read_index(self, name):
+        which = INDEX[name]
+        n = lib().psba_index_size(self._h, which)
+        if n < 0:
+            raise PsbaError("index not built")
+        out = np.empty(max(n, 1), np.int32)
+        _check(lib().psba_read_index(self._h, which, _iptr(out), out.size), "psba_read_index")
+        return out[:n]
 
     def set_fixed(self, mask):
         if mask is None:
@@ -453,6 +475,19 @@ def generate_idxs(n_cams, n_pts, n_obs, vmask, with_comm=False):
                                   _iptr(jidx), _iptr(blk))
     _check(rc, "generate_idxs")
     return iidx, jidx, blk, cnt, comm
+
+
+def generate_idxs_device(n_cams, n_pts, n_obs, vmask, device=0):
+    """generate_idxs on a device; returns (iidx, jidx, blk_idx, cnt)."""
+    vm = np.ascontiguousarray(vmask, dtype=np.int8).reshape(-1)
+    iidx = np.empty(n_obs, np.int32)
+    jidx = np.empty(n_obs, np.int32)
+    blk = np.empty(n_pts * n_cams, np.int32)
+    cnt = np.empty(n_cams * n_cams, np.int32)
+    rc = lib().psba_generate_idxs_device(device, n_cams, n_pts, n_obs, vm.ctypes.data_as(ct.c_char_p), _iptr(cnt),
+                                         _iptr(iidx), _iptr(jidx), _iptr(blk))
+    _check(rc, "generate_idxs_device")
+    return iidx, jidx, blk, cnt
 
 
 def problem_bytes(cnp, n_cams, n_pts, n_obs, pair_items, precision=PREC_FP64):

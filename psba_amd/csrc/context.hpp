@@ -150,6 +150,11 @@ int launch_factor_solve(psba_ctx* c);                     // S -> Lf, SC_FACTOR_
 int launch_modchol(psba_ctx* c, double* S_src);           // SC_LAMBDA (factor in S)
 int launch_copy_S(psba_ctx* c, double* dst, const double* src);
 
+// ---- index (ctx.cpp, k_index.hip) ----
+int check_obs_order(int nc, int np, int no, const int* iidx, const int* jidx);  // host validation
+// frees the previous index and allocates every index buffer and the chunk partials
+int alloc_index(psba_ctx* c, long nitems, int npairs, int n_cchunk, int n_schunk);
+
 // ---- launchers (k_tr.hip) ----
 int launch_cauchy(psba_ctx* c, double* out, const double* g, double gtg, double gtBg);
 int launch_tr_dots(psba_ctx* c, const double* PU, const double* PB, const double* g, int mode);  // SC_D0..D4

@@ -378,17 +378,66 @@ int psba_generate_idxs(int nCams, int n3Dpts, int n2Dprojs, const char* vmask, i
   return PSBA_OK;
 }
 
-int psba_set_index(psba_ctx* c, const int* iidx, const int* jidx) {
-  if (!c || !iidx || !jidx) return fail(PSBA_ERR_ARG, "null argument");
-  PSBA_HIP_TRY(hipSetDevice(c->device));
-  const int nc = c->nc, np = c->np, no = c->no, C = c->cnp;
-  // validate: point-major, camera ascending within a point
+}  // extern "C"
+
+namespace psba {
+
+int check_obs_order(int nc, int np, int no, const int* iidx, const int* jidx) {
   for (int k = 0; k < no; ++k) {
     if (iidx[k] < 0 || iidx[k] >= np || jidx[k] < 0 || jidx[k] >= nc)
       return fail(PSBA_ERR_ARG, "observation %d has index out of range", k);
     if (k > 0 && (iidx[k] < iidx[k - 1] || (iidx[k] == iidx[k - 1] && jidx[k] <= jidx[k - 1])))
       return fail(PSBA_ERR_ARG, "observations must be point-major and camera-ascending (obs %d)", k);
   }
+  return PSBA_OK;
+}
+
+int alloc_index(psba_ctx* c, long nitems, int npairs, int n_cchunk, int n_schunk) {
+  free_index(c);
+  const int nc = c->nc, np = c->np, no = c->no, C = c->cnp;
+  const int nlb = nc * (nc + 1) / 2;
+  c->n_cchunk = n_cchunk;
+  c->npairs = npairs;
+  c->n_items = nitems;
+  c->n_schunk = n_schunk;
+  c->n_lblocks = nlb;
+  const int NE = C * (C + 1) / 2 + C;
+  PSBA_TRY(dalloc(&c->obs_cam, no));
+  PSBA_TRY(dalloc(&c->obs_pt, no));
+  PSBA_TRY(dalloc(&c->pt_ptr, np + 1));
+  PSBA_TRY(dalloc(&c->cam_ptr, nc + 1));
+  PSBA_TRY(dalloc(&c->cam_obs, no));
+  PSBA_TRY(dalloc(&c->cchunk_begin, n_cchunk));
+  PSBA_TRY(dalloc(&c->cchunk_end, n_cchunk));
+  PSBA_TRY(dalloc(&c->cam_cchunk_ptr, nc + 1));
+  PSBA_TRY(dalloc(&c->pair_items, (size_t)nitems));
+  PSBA_TRY(dalloc(&c->pair_k, npairs));
+  PSBA_TRY(dalloc(&c->pair_l, npairs));
+  PSBA_TRY(dalloc(&c->schunk_begin, n_schunk));
+  PSBA_TRY(dalloc(&c->schunk_end, n_schunk));
+  PSBA_TRY(dalloc(&c->pair_schunk_ptr, npairs + 1));
+  PSBA_TRY(dalloc(&c->lblock_pair, nlb));
+  if (c->mixed()) {
+    PSBA_TRY(dalloc(&c->ucpart32, (size_t)n_cchunk * NE));
+    PSBA_TRY(dalloc(&c->eapart32, (size_t)n_cchunk * C));
+    PSBA_TRY(dalloc(&c->schpart32, (size_t)n_schunk * C * C));
+  } else {
+    PSBA_TRY(dalloc(&c->ucpart, (size_t)n_cchunk * NE));
+    PSBA_TRY(dalloc(&c->eapart, (size_t)n_cchunk * C));
+    PSBA_TRY(dalloc(&c->schpart, (size_t)n_schunk * C * C));
+  }
+  return PSBA_OK;
+}
+
+}  // namespace psba
+
+extern "C" {
+
+int psba_set_index(psba_ctx* c, const int* iidx, const int* jidx) {
+  if (!c || !iidx || !jidx) return fail(PSBA_ERR_ARG, "null argument");
+  PSBA_HIP_TRY(hipSetDevice(c->device));
+  const int nc = c->nc, np = c->np, no = c->no, C = c->cnp;
+  PSBA_TRY(check_obs_order(nc, np, no, iidx, jidx));
   free_index(c);
   std::vector<int> pt_ptr(np + 1, 0), cam_ptr(nc + 1, 0), cam_obs(no);
   for (int k = 0; k < no; ++k) {
@@ -457,37 +506,8 @@ int psba_set_index(psba_ctx* c, const int* iidx, const int* jidx) {
   for (int k = 0; k < nc; ++k)
     for (int l = 0; l <= k; ++l) lbp[k * (k + 1) / 2 + l] = pid[(size_t)k * nc + l];
 
-  c->n_cchunk = (int)cb.size();
-  c->npairs = npairs;
-  c->n_items = nitems;
-  c->n_schunk = (int)sb.size();
-  c->n_lblocks = nlb;
+  PSBA_TRY(alloc_index(c, nitems, npairs, (int)cb.size(), (int)sb.size()));
   c->sum_t2 = st2;
-  const int NE = C * (C + 1) / 2 + C;
-  PSBA_TRY(dalloc(&c->obs_cam, no));
-  PSBA_TRY(dalloc(&c->obs_pt, no));
-  PSBA_TRY(dalloc(&c->pt_ptr, np + 1));
-  PSBA_TRY(dalloc(&c->cam_ptr, nc + 1));
-  PSBA_TRY(dalloc(&c->cam_obs, no));
-  PSBA_TRY(dalloc(&c->cchunk_begin, cb.size()));
-  PSBA_TRY(dalloc(&c->cchunk_end, ce.size()));
-  PSBA_TRY(dalloc(&c->cam_cchunk_ptr, nc + 1));
-  PSBA_TRY(dalloc(&c->pair_items, (size_t)nitems));
-  PSBA_TRY(dalloc(&c->pair_k, npairs));
-  PSBA_TRY(dalloc(&c->pair_l, npairs));
-  PSBA_TRY(dalloc(&c->schunk_begin, sb.size()));
-  PSBA_TRY(dalloc(&c->schunk_end, se.size()));
-  PSBA_TRY(dalloc(&c->pair_schunk_ptr, npairs + 1));
-  PSBA_TRY(dalloc(&c->lblock_pair, nlb));
-  if (c->mixed()) {
-    PSBA_TRY(dalloc(&c->ucpart32, (size_t)c->n_cchunk * NE));
-    PSBA_TRY(dalloc(&c->eapart32, (size_t)c->n_cchunk * C));
-    PSBA_TRY(dalloc(&c->schpart32, (size_t)c->n_schunk * C * C));
-  } else {
-    PSBA_TRY(dalloc(&c->ucpart, (size_t)c->n_cchunk * NE));
-    PSBA_TRY(dalloc(&c->eapart, (size_t)c->n_cchunk * C));
-    PSBA_TRY(dalloc(&c->schpart, (size_t)c->n_schunk * C * C));
-  }
   auto up = [&](void* dst, const void* src, size_t bytes) -> int {
     if (bytes == 0) return PSBA_OK;
     PSBA_HIP_TRY(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, c->stream));
@@ -685,6 +705,46 @@ int psba_read_buffer(psba_ctx* c, int which, double* out, long n) {
     }
   }
   return fail(PSBA_ERR_ARG, "unknown buffer %d", which);
+}
+
+static const int* index_ptr(const psba_ctx* c, int which, long* n) {
+  switch (which) {
+    case PSBA_IDX_OBS_PT: *n = c->no; return c->obs_pt;
+    case PSBA_IDX_OBS_CAM: *n = c->no; return c->obs_cam;
+    case PSBA_IDX_PT_PTR: *n = c->np + 1; return c->pt_ptr;
+    case PSBA_IDX_CAM_PTR: *n = c->nc + 1; return c->cam_ptr;
+    case PSBA_IDX_CAM_OBS: *n = c->no; return c->cam_obs;
+    case PSBA_IDX_CCHUNK_BEGIN: *n = c->n_cchunk; return c->cchunk_begin;
+    case PSBA_IDX_CCHUNK_END: *n = c->n_cchunk; return c->cchunk_end;
+    case PSBA_IDX_CAM_CCHUNK_PTR: *n = c->nc + 1; return c->cam_cchunk_ptr;
+    case PSBA_IDX_PAIR_ITEMS: *n = 2 * c->n_items; return (const int*)c->pair_items;
+    case PSBA_IDX_PAIR_K: *n = c->npairs; return c->pair_k;
+    case PSBA_IDX_PAIR_L: *n = c->npairs; return c->pair_l;
+    case PSBA_IDX_SCHUNK_BEGIN: *n = c->n_schunk; return c->schunk_begin;
+    case PSBA_IDX_SCHUNK_END: *n = c->n_schunk; return c->schunk_end;
+    case PSBA_IDX_PAIR_SCHUNK_PTR: *n = c->npairs + 1; return c->pair_schunk_ptr;
+    case PSBA_IDX_LBLOCK_PAIR: *n = c->n_lblocks; return c->lblock_pair;
+    default: *n = -1; return nullptr;
+  }
+}
+
+long psba_index_size(const psba_ctx* c, int which) {
+  if (!c || !c->indexed) return -1;
+  long n = -1;
+  (void)index_ptr(c, which, &n);
+  return n;
+}
+
+int psba_read_index(psba_ctx* c, int which, int* out, long n) {
+  if (!c || !out) return fail(PSBA_ERR_ARG, "null argument");
+  if (!c->indexed) return fail(PSBA_ERR_STATE, "no index");
+  long size = -1;
+  const int* src = index_ptr(c, which, &size);
+  if (size < 0) return fail(PSBA_ERR_ARG, "unknown index array %d", which);
+  if (n < size) return fail(PSBA_ERR_ARG, "output holds %ld ints, %ld needed", n, size);
+  PSBA_HIP_TRY(hipSetDevice(c->device));
+  if (size > 0) PSBA_HIP_TRY(hipMemcpy(out, src, sizeof(int) * size, hipMemcpyDeviceToHost));
+  return PSBA_OK;
 }
 
 int psba_get_info(psba_ctx* c, psba_info* o) {
