@@ -92,6 +92,17 @@ int psba_set_index_device(psba_ctx* ctx, const int* iidx, const int* jidx);
 /* addition (no reference counterpart): hold parameters fixed; mask has
  * cnp*n_cams bytes, 1 = fixed.  NULL clears the mask. */
 int psba_set_fixed(psba_ctx* ctx, const unsigned char* mask);
+/* addition (no reference counterpart; the reference minimises the plain sum
+ * of squares): a robust loss on each observation's squared residual norm s,
+ *   PSBA_LOSS_HUBER   rho(s) = s (s <= d^2), 2 d sqrt(s) - d^2 beyond
+ *   PSBA_LOSS_CAUCHY  rho(s) = d^2 log(1 + s / d^2)
+ * with scale d (pixels).  Steps solve the reweighted problem (residual and
+ * Jacobian rows scaled by sqrt(rho'(s))); costs, the gain ratio and finalErr
+ * are sum rho(s).  PSBA_LOSS_NONE (the default) is the reference's cost. */
+#define PSBA_LOSS_NONE 0
+#define PSBA_LOSS_HUBER 1
+#define PSBA_LOSS_CAUCHY 2
+int psba_set_loss(psba_ctx* ctx, int kind, double scale);
 /* accessor for the optimised parameters (the reference leaves them in
  * cams_buffer / pts3D_buffer, PSBA/cl_psba.h:21-22) */
 int psba_read_params(psba_ctx* ctx, double* cams, double* pts);

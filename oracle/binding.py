@@ -12,6 +12,8 @@ import numpy as np
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "liboracle.so")
 
+LOSS_NONE, LOSS_HUBER, LOSS_CAUCHY = 0, 1, 2
+
 SOLVE_CHOLESKY = 0
 SOLVE_INVERSE = 1
 
@@ -82,6 +84,8 @@ def lib():
         "orc_hybrid": (ct.c_int, [vp, ct.POINTER(Opts), ip, dp, dp, ct.POINTER(TraceRec), ct.c_int, ip]),
         "orc_dense_factor_solve": (ct.c_int, [ct.c_int, ct.c_int, dp, dp, dp, dp]),
         "orc_dense_modchol": (ct.c_double, [ct.c_int, dp, dp, dp]),
+        "orc_set_loss": (ct.c_int, [vp, ct.c_int, ct.c_double]),
+        "orc_loss_eval": (None, [ct.c_int, ct.c_double, ct.c_double, dp, dp]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -195,6 +199,10 @@ class Oracle:
         p = np.ascontiguousarray(pts, np.float64)
         lib().orc_set_params(self._h, _d(c), _d(p))
 
+    def set_loss(self, kind, scale=1.0):
+        if lib().orc_set_loss(self._h, kind, scale) != 0:
+            raise ValueError("bad loss")
+
     def _run(self, fn, trace_cap, solve_mode, max_iter, max_pb_tries=0, init=False, flags=0, rel_tol=0.0):
         o = Opts(solve_mode, max_iter, max_pb_tries, flags, rel_tol)
         tr = (TraceRec * max(trace_cap, 1))()
@@ -267,3 +275,10 @@ class Oracle:
         out = np.empty(n)
         lib().orc_read_buffer(self._h, w, _d(out))
         return out
+
+
+def loss_eval(kind, scale, s):
+    """(rho(s), rho'(s)) of the oracle's robust loss."""
+    r, d = ct.c_double(), ct.c_double()
+    lib().orc_loss_eval(kind, scale, s, ct.byref(r), ct.byref(d))
+    return r.value, d.value

@@ -65,6 +65,10 @@ struct orc_ctx {
   double *ex, *JA, *JB, *U, *V, *W, *g, *uvdiag, *Vinv, *Y, *S, *Saux, *L, *ea, *eb,
       *dp, *xtmp, *Sinv;
   int L_valid;
+  /* robust loss on each observation's squared residual norm (an extension:
+   * the reference minimises the plain sum of squares) */
+  int loss;
+  double loss_scale;
 };
 
 static int g_threads = 1;
@@ -410,8 +414,42 @@ static double dotp(const double *a, const double *b, long n) {
 /* stages                                                                   */
 /* ------------------------------------------------------------------------ */
 
+/* Robust loss of one observation's squared residual norm s, with scale d:
+ *   Huber   rho(s) = s for s <= d^2, 2 d sqrt(s) - d^2 beyond; rho' = 1 or d / sqrt(s)
+ *   Cauchy  rho(s) = d^2 log(1 + s / d^2);                       rho' = 1 / (1 + s / d^2)
+ * Steps use the reweighted problem: residual and Jacobian rows of the
+ * observation scaled by sqrt(rho'(s)) (iteratively reweighted least squares);
+ * the cost that accepts or rejects a step is sum rho(s). */
+void orc_loss_eval(int kind, double d, double s, double *rho, double *drho) {
+  const double d2 = d * d;
+  if (kind == ORC_LOSS_HUBER) {
+    if (s <= d2) {
+      *rho = s;
+      *drho = 1.0;
+    } else {
+      const double r = sqrt(s);
+      *rho = 2.0 * d * r - d2;
+      *drho = d / r;
+    }
+  } else if (kind == ORC_LOSS_CAUCHY) {
+    *rho = d2 * log1p(s / d2);
+    *drho = 1.0 / (1.0 + s / d2);
+  } else {
+    *rho = s;
+    *drho = 1.0;
+  }
+}
+
+int orc_set_loss(orc_ctx *c, int kind, double scale) {
+  if (kind != ORC_LOSS_NONE && kind != ORC_LOSS_HUBER && kind != ORC_LOSS_CAUCHY) return -1;
+  if (kind != ORC_LOSS_NONE && !(scale > 0)) return -1;
+  c->loss = kind;
+  c->loss_scale = scale;
+  return 0;
+}
+
 /* residual e = m - pi(p) per observation (PSBA/CL_files/compute_exQT.cl:18-71)
- * and its squared norm (PSBA/misc.cpp:151-157) */
+ * and its squared norm (PSBA/misc.cpp:151-157), or the robust cost */
 double orc_stage_residual(orc_ctx *c, int use_new, double *ex) {
   const double *cams = use_new ? c->newcams : c->cams;
   const double *pts = use_new ? c->newpts : c->pts;
@@ -425,7 +463,15 @@ double orc_stage_residual(orc_ctx *c, int use_new, double *ex) {
     c->ex[2 * k + 1] = c->meas[2 * k + 1] - uv[1];
   }
   if (ex) memcpy(ex, c->ex, sizeof(double) * 2 * c->no);
-  return l2sq(2L * c->no, c->ex);
+  if (c->loss == ORC_LOSS_NONE) return l2sq(2L * c->no, c->ex);
+  double sum = 0;
+  for (int k = 0; k < c->no; k++) {
+    double rho, drho;
+    orc_loss_eval(c->loss, c->loss_scale, c->ex[2 * k] * c->ex[2 * k] + c->ex[2 * k + 1] * c->ex[2 * k + 1], &rho,
+                  &drho);
+    sum += rho;
+  }
+  return sum;
 }
 
 /* JA (2 x cnp) and JB (2 x 3) per observation at the current parameters,
@@ -444,6 +490,16 @@ void orc_stage_jacobian(orc_ctx *c) {
       if (c->fix[(long)C * j + col]) ja[col] = ja[C + col] = 0.0;
     c->ex[2 * k] = c->meas[2 * k] - uv[0];
     c->ex[2 * k + 1] = c->meas[2 * k + 1] - uv[1];
+    if (c->loss != ORC_LOSS_NONE) {
+      double rho, drho;
+      orc_loss_eval(c->loss, c->loss_scale, c->ex[2 * k] * c->ex[2 * k] + c->ex[2 * k + 1] * c->ex[2 * k + 1], &rho,
+                    &drho);
+      const double w = sqrt(drho);
+      for (int q = 0; q < 2 * C; q++) ja[q] *= w;
+      for (int q = 0; q < 6; q++) c->JB[6L * k + q] *= w;
+      c->ex[2 * k] *= w;
+      c->ex[2 * k + 1] *= w;
+    }
   }
 }
 

@@ -77,6 +77,14 @@ typedef struct {
 
 typedef struct orc_ctx orc_ctx;
 
+/* robust loss on each observation's squared residual norm (extension; the
+ * reference has none): ORC_LOSS_NONE reproduces the reference */
+#define ORC_LOSS_NONE 0
+#define ORC_LOSS_HUBER 1
+#define ORC_LOSS_CAUCHY 2
+int orc_set_loss(orc_ctx *c, int kind, double scale);
+void orc_loss_eval(int kind, double scale, double s, double *rho, double *drho);
+
 /* ---- model --------------------------------------------------------------- */
 int orc_cnp_supported(int cnp);
 /* projection of point M by camera (K5, q0, cam); optional Jacobians

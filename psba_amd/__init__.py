@@ -30,6 +30,8 @@ ITER_PASS = 7
 PREC_FP64 = 0
 PREC_MIXED = 1
 
+LOSS_NONE, LOSS_HUBER, LOSS_CAUCHY = 0, 1, 2
+
 BUF = dict(EX=0, JA=1, JB=2, U=3, V=4, W=5, G=6, VINV=7, S=9, EA=10, DP=11, EB=12, L=13)
 INDEX = dict(OBS_PT=0, OBS_CAM=1, PT_PTR=2, CAM_PTR=3, CAM_OBS=4, CCHUNK_BEGIN=5, CCHUNK_END=6, CAM_CCHUNK_PTR=7,
              PAIR_ITEMS=8, PAIR_K=9, PAIR_L=10, SCHUNK_BEGIN=11, SCHUNK_END=12, PAIR_SCHUNK_PTR=13, LBLOCK_PAIR=14)
@@ -187,6 +189,7 @@ def lib():
         "psba_server_get_stats": (ct.c_int, [vp, ct.POINTER(ServerStats)]),
         "psba_server_completed": (ct.c_int, [vp, ct.POINTER(ct.c_longlong), ct.c_int]),
         "psba_set_index_device": (ct.c_int, [vp, ip, ip]),
+        "psba_set_loss": (ct.c_int, [vp, ct.c_int, ct.c_double]),
         "psba_generate_idxs_device": (ct.c_int, [ct.c_int, ct.c_int, ct.c_int, ct.c_int, ct.c_char_p, ip, ip, ip,
                                                  ip]),
         "psba_index_size": (ct.c_long, [vp, ct.c_int]),
@@ -323,6 +326,10 @@ class Context:
         i = np.ascontiguousarray(iidx, dtype=np.int32)
         j = np.ascontiguousarray(jidx, dtype=np.int32)
         _check(lib().psba_set_index(self._h, _iptr(i), _iptr(j)), "psba_set_index")
+
+    def set_loss(self, kind, scale=1.0):
+        """Robust loss on each observation's squared residual norm (LOSS_*)."""
+        _check(lib().psba_set_loss(self._h, kind, scale), "psba_set_loss")
 
     def set_index_device(self, iidx, jidx):
         """The same index built on the device (bit-identical to set_index)."""

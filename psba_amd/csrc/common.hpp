@@ -112,6 +112,28 @@ __device__ inline void grid_sum_finish(double (&v)[K], double* partials, unsigne
 
 inline unsigned ceil_div(long a, long b) { return (unsigned)((a + b - 1) / b); }
 
+// Robust loss of an observation's squared residual norm s with scale d
+// (PSBA_LOSS_*): rho(s) and rho'(s).
+__device__ __forceinline__ void loss_eval(int kind, double d, double s, double& rho, double& drho) {
+  const double d2 = d * d;
+  if (kind == PSBA_LOSS_HUBER) {
+    if (s <= d2) {
+      rho = s;
+      drho = 1.0;
+    } else {
+      const double r = sqrt(s);
+      rho = 2.0 * d * r - d2;
+      drho = d / r;
+    }
+  } else if (kind == PSBA_LOSS_CAUCHY) {
+    rho = d2 * log1p(s / d2);
+    drho = 1.0 / (1.0 + s / d2);
+  } else {
+    rho = s;
+    drho = 1.0;
+  }
+}
+
 // Workgroups of a launch go to the 8 XCDs round-robin (workgroup b to XCD
 // b % 8), each XCD with its own L2.  A grid of xcd_grid(n) workgroups with
 // item = xcd_item(b, n) gives every XCD one contiguous eighth of an n-item

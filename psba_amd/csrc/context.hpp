@@ -56,6 +56,8 @@ struct psba_ctx {
   int cnp = 0, nc = 0, np = 0, no = 0;
   int n = 0, ntot = 0, n_pad = 0;
   int precision = 0, device = 0;
+  int loss = PSBA_LOSS_NONE;  // robust loss (PSBA_LOSS_*) and its scale
+  double loss_scale = 1.0;
   bool uploaded = false, indexed = false;
   bool s_loaded = false;  // a reduced system was loaded directly (psba_stage_load_S)
   hipStream_t stream = nullptr;

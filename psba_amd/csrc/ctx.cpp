@@ -294,6 +294,17 @@ int psba_upload(psba_ctx* c, const double* K, const double* impts, const double*
   return PSBA_OK;
 }
 
+int psba_set_loss(psba_ctx* c, int kind, double scale) {
+  if (!c) return fail(PSBA_ERR_ARG, "null context");
+  if (kind != PSBA_LOSS_NONE && kind != PSBA_LOSS_HUBER && kind != PSBA_LOSS_CAUCHY)
+    return fail(PSBA_ERR_ARG, "unknown loss %d", kind);
+  if (kind != PSBA_LOSS_NONE && !(scale > 0 && std::isfinite(scale)))
+    return fail(PSBA_ERR_ARG, "loss scale must be positive and finite");
+  c->loss = kind;
+  c->loss_scale = kind == PSBA_LOSS_NONE ? 1.0 : scale;
+  return PSBA_OK;
+}
+
 int psba_set_fixed(psba_ctx* c, const unsigned char* mask) {
   if (!c) return fail(PSBA_ERR_ARG, "null context");
   PSBA_HIP_TRY(hipSetDevice(c->device));

@@ -24,7 +24,8 @@ __global__ __launch_bounds__(256) void k_residual(const double* __restrict__ K, 
                                                   const double* __restrict__ dp, int n,
                                                   const double* __restrict__ meas, const int* __restrict__ obs_cam,
                                                   const int* __restrict__ obs_pt, int no, double* __restrict__ e_out,
-                                                  double* partials, unsigned* counter, double* out) {
+                                                  double* partials, unsigned* counter, double* out, int loss,
+                                                  double loss_d) {
   __shared__ double sh[4];
   double acc[1] = {0.0};
   int o = blockIdx.x * blockDim.x + threadIdx.x;
@@ -42,6 +43,10 @@ __global__ __launch_bounds__(256) void k_residual(const double* __restrict__ K, 
       e_out[2 * o + 1] = e1;
     }
     acc[0] = e0 * e0 + e1 * e1;
+    if (loss) {
+      double drho;
+      loss_eval(loss, loss_d, acc[0], acc[0], drho);
+    }
   }
   grid_sum_finish<1>(acc, partials, counter, out, sh);
 }
@@ -51,7 +56,8 @@ static void run_residual(psba_ctx* c, bool trial) {
   unsigned nb = ceil_div(c->no, 256);
   hipLaunchKernelGGL(k_residual<CNP>, dim3(nb), dim3(256), 0, c->stream, c->K, c->q0, c->cams, c->pts,
                      trial ? c->dp : nullptr, c->n, c->meas, c->obs_cam, c->obs_pt, c->no,
-                     trial ? nullptr : c->e, c->red_part, c->counters + CT_RESID, c->d_scal + SC_COST);
+                     trial ? nullptr : c->e, c->red_part, c->counters + CT_RESID, c->d_scal + SC_COST, c->loss,
+                     c->loss_scale);
 }
 
 int launch_residual(psba_ctx* c, bool trial) {
@@ -73,7 +79,8 @@ __global__ __launch_bounds__(256) void k_jacobian(const double* __restrict__ K, 
                                                   const int* __restrict__ obs_pt, const unsigned char* __restrict__ fix,
                                                   int no, double coeff, R* __restrict__ e, R* __restrict__ JA,
                                                   R* __restrict__ JB, R* __restrict__ W,
-                                                  const int* __restrict__ ownr, const unsigned char* __restrict__ run) {
+                                                  const int* __restrict__ ownr, const unsigned char* __restrict__ run,
+                                                  int loss, double loss_d) {
   int o = blockIdx.x * blockDim.x + threadIdx.x;
   if (o >= no) return;
   if (run && !run[ownr[o]]) return;  // batched problems: only those that need it
@@ -88,6 +95,18 @@ __global__ __launch_bounds__(256) void k_jacobian(const double* __restrict__ K, 
   for (int k = 0; k < CNP; ++k)
     if (fix[j * CNP + k]) ja[k] = ja[CNP + k] = 0.0;
   double e0 = meas[2 * o] - uv[0], e1 = meas[2 * o + 1] - uv[1];
+  if (loss) {
+    // reweighted rows: residual and Jacobian scaled by sqrt(rho'(s))
+    double rho, drho;
+    loss_eval(loss, loss_d, e0 * e0 + e1 * e1, rho, drho);
+    const double w = sqrt(drho);
+#pragma unroll
+    for (int k = 0; k < 2 * CNP; ++k) ja[k] *= w;
+#pragma unroll
+    for (int k = 0; k < 6; ++k) jb[k] *= w;
+    e0 *= w;
+    e1 *= w;
+  }
   e[2 * o] = (R)e0;
   e[2 * o + 1] = (R)e1;
   R* pa = JA + (size_t)o * 2 * CNP;
@@ -223,7 +242,7 @@ static void run_jac_normal_t(psba_ctx* c, double coeff, double coeff_g, R* e, R*
   probe_begin(c, PSBA_PROBE_JACOBIAN);
   hipLaunchKernelGGL((k_jacobian<CNP, R>), dim3(ceil_div(c->no, 256)), dim3(256), 0, c->stream, c->K, c->q0,
                      c->cams, c->pts, c->meas, c->obs_cam, c->obs_pt, c->fix, c->no, coeff, e, JA, JB, W,
-                     (const int*)nullptr, (const unsigned char*)nullptr);
+                     (const int*)nullptr, (const unsigned char*)nullptr, c->loss, c->loss_scale);
   probe_end(c, PSBA_PROBE_JACOBIAN);
   hipLaunchKernelGGL(k_point_normal<R>, dim3(ceil_div(c->np, 256)), dim3(256), 0, c->stream, c->pt_ptr, JB, e,
                      c->np, coeff, coeff_g, c->V, c->g + c->n, c->uvdiag + c->n, (const int*)nullptr,
@@ -543,7 +562,7 @@ static void run_b_jac_t(psba_batch* b) {
   const int bs = b->obs_block;
   hipLaunchKernelGGL((k_jacobian<CNP, R>), dim3(ceil_div(b->NO, bs)), dim3(bs), 0, b->stream, b->K, b->q0, b->cams,
                      b->pts, b->meas, b->obs_cam, b->obs_pt, b->fix, (int)b->NO, 1.0, e, JA, JB, W, b->obs_prob,
-                     b->need_jac);
+                     b->need_jac, PSBA_LOSS_NONE, 1.0);
   hipLaunchKernelGGL(k_point_normal<R>, dim3(ceil_div(b->NP, b->pt_block)), dim3(b->pt_block), 0, b->stream, b->pt_ptr, JB, e,
                      (int)b->NP, 1.0, 1.0, b->V, b->g + b->N, b->uvdiag + b->N, b->pt_prob, b->need_jac);
   if (b->n_cchunk > 0)
