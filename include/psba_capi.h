@@ -63,7 +63,7 @@ typedef struct {
 typedef struct {
   int max_iter;      /* shared iteration cap (reference: 50) */
   int max_pb_tries;  /* guard on the dogleg's factor-retry loop (0 = 64) */
-  int refine_steps;  /* mixed precision: FP64 refinement sweeps (0 = default) */
+  int refine_steps;  /* mixed precision: FP64 refinement sweeps of every step (0 = none) */
   int flags;         /* PSBA_FLAG_* (0 = reference behaviour) */
   double rel_tol;    /* > 0: LM also stops (ITER_DP_NO_CHANGE) once an accepted
                         step lowers the cost by less than rel_tol relative */
@@ -141,6 +141,29 @@ int psba_trust_region(psba_ctx* ctx, const psba_opts* opts, int* itno, double* f
 int psba_hybrid(psba_ctx* ctx, const psba_opts* opts, int* itno, double* initErr,
                 double* finalErr, psba_trace_rec* trace, int trace_cap, int* trace_len);
 
+/* The hybrid loop of PSBA/main.cpp:193-208 with the report that
+ * PSBA/main.cpp:209-225 prints: stop code, wall time, the reference's error
+ * measure sqrt(SSE) / n_obs (initial and final), total iterations and the time
+ * per stage (here device time, from HIP events on the context's stream).
+ * Added: the RMS reprojection error per observation sqrt(SSE / n_obs), the
+ * split of iterations between LM and the dogleg, and the hand-offs. */
+typedef struct {
+  double jacobian_normal, schur, factor, solve, backsub, residual, other;
+  int trials, jacobians;
+} psba_report_stages;
+typedef struct {
+  int code;            /* PSBA_ITER_* */
+  int itno;            /* the shared iteration counter after the loop */
+  int lm_iterations, tr_iterations, handoffs; /* iterations entered by each driver */
+  int lm_trials, tr_steps;
+  double init_sse, final_sse;
+  double init_error_ref, final_error_ref; /* sqrt(SSE) / n_obs (PSBA/main.cpp:214-215) */
+  double init_rms_px, final_rms_px;       /* sqrt(SSE / n_obs) */
+  double wall_ms;
+  psba_report_stages stages;               /* milliseconds per stage, counts */
+} psba_report;
+int psba_hybrid_report(psba_ctx* ctx, const psba_opts* opts, int* itno, psba_report* out);
+
 /* ---- stage entry points (PSBA/sba_func.h:10-138) ---------------------------- */
 /* residual and its squared norm at the current parameters (use_trial = 0) or
  * at p + dp (use_trial = 1); PSBA/sba_func.cpp:81-145 */
@@ -158,6 +181,12 @@ int psba_stage_backsub(psba_ctx* ctx);
 /* modified Cholesky of the current S, returns lambda = |sum E| / n;
  * PSBA/cl_cholmod.h (cholmod_blk, compute_cholmod_E) */
 int psba_stage_modchol(psba_ctx* ctx, double* lambda);
+/* mixed-precision contexts: FP64 iterative refinement of the current step dp
+ * at damping mu (after the backsub stage): per sweep, r = J^T (e - J dp) -
+ * mu dp with the Jacobian and residual recomputed in FP64, reduced and solved
+ * with the existing blocks and factor, dp += correction; g is replaced by the
+ * FP64 J^T e.  No reference counterpart (the reference is FP64 throughout). */
+int psba_stage_refine(psba_ctx* ctx, double mu, int sweeps);
 /* test hook: load an n x n matrix (n = cnp*n_cams, row-major, lower triangle
  * used) as S and, if b != NULL, b as e_a, so the dense stages can be checked
  * on arbitrary matrices */

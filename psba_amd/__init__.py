@@ -98,6 +98,20 @@ class SbaData(ct.Structure):
                 ("covimgpts", ct.POINTER(ct.c_double)), ("vmask", ct.POINTER(ct.c_char))]
 
 
+class ReportStages(ct.Structure):
+    _fields_ = [("jacobian_normal", ct.c_double), ("schur", ct.c_double), ("factor", ct.c_double),
+                ("solve", ct.c_double), ("backsub", ct.c_double), ("residual", ct.c_double),
+                ("other", ct.c_double), ("trials", ct.c_int), ("jacobians", ct.c_int)]
+
+
+class Report(ct.Structure):
+    _fields_ = [("code", ct.c_int), ("itno", ct.c_int), ("lm_iterations", ct.c_int), ("tr_iterations", ct.c_int),
+                ("handoffs", ct.c_int), ("lm_trials", ct.c_int), ("tr_steps", ct.c_int),
+                ("init_sse", ct.c_double), ("final_sse", ct.c_double), ("init_error_ref", ct.c_double),
+                ("final_error_ref", ct.c_double), ("init_rms_px", ct.c_double), ("final_rms_px", ct.c_double),
+                ("wall_ms", ct.c_double), ("stages", ReportStages)]
+
+
 class BatchDesc(ct.Structure):
     _fields_ = [("cnp", ct.c_int), ("precision", ct.c_int), ("device", ct.c_int), ("max_problems", ct.c_int)]
 
@@ -149,6 +163,7 @@ def lib():
         "psba_stage_factor_solve": (ct.c_int, [vp, ip]),
         "psba_stage_backsub": (ct.c_int, [vp]),
         "psba_stage_modchol": (ct.c_int, [vp, dp]),
+        "psba_stage_refine": (ct.c_int, [vp, ct.c_double, ct.c_int]),
         "psba_stage_load_S": (ct.c_int, [vp, dp, dp]),
         "psba_buffer_size": (ct.c_long, [vp, ct.c_int]),
         "psba_read_buffer": (ct.c_int, [vp, ct.c_int, dp, ct.c_long]),
@@ -190,6 +205,7 @@ def lib():
         "psba_server_completed": (ct.c_int, [vp, ct.POINTER(ct.c_longlong), ct.c_int]),
         "psba_set_index_device": (ct.c_int, [vp, ip, ip]),
         "psba_set_loss": (ct.c_int, [vp, ct.c_int, ct.c_double]),
+        "psba_hybrid_report": (ct.c_int, [vp, ct.POINTER(Opts), ip, ct.POINTER(Report)]),
         "psba_generate_idxs_device": (ct.c_int, [ct.c_int, ct.c_int, ct.c_int, ct.c_int, ct.c_char_p, ip, ip, ip,
                                                  ip]),
         "psba_index_size": (ct.c_long, [vp, ct.c_int]),
@@ -370,8 +386,8 @@ class Context:
     def restore(self):
         _check(lib().psba_restore_params(self._h), "psba_restore_params")
 
-    def _opts(self, max_iter=50, max_pb_tries=0, flags=0, rel_tol=0.0):
-        return Opts(max_iter, max_pb_tries, 0, flags, rel_tol)
+    def _opts(self, max_iter=50, max_pb_tries=0, flags=0, rel_tol=0.0, refine_steps=0):
+        return Opts(max_iter, max_pb_tries, refine_steps, flags, rel_tol)
 
     def _run(self, fn, trace_cap, opts, extra_init=False):
         tr = (TraceRec * max(trace_cap, 1))()
@@ -391,9 +407,10 @@ class Context:
             out["init_err"] = ie.value
         return out
 
-    def levmar(self, max_iter=50, trace_cap=4096, flags=0, rel_tol=0.0):
-        """PSBA/levmar.h:6-11 on the device; returns code, itno, final_err, trace."""
-        return self._run(lib().psba_levmar, trace_cap, self._opts(max_iter, 0, flags, rel_tol))
+    def levmar(self, max_iter=50, trace_cap=4096, flags=0, rel_tol=0.0, refine_steps=0):
+        """PSBA/levmar.h:6-11 on the device; returns code, itno, final_err, trace.
+        refine_steps: FP64 refinement sweeps per step (mixed contexts)."""
+        return self._run(lib().psba_levmar, trace_cap, self._opts(max_iter, 0, flags, rel_tol, refine_steps))
 
     def trust_region(self, max_iter=50, trace_cap=4096, max_pb_tries=0):
         """PSBA/trust_region.h:5-11 on the device."""
@@ -404,6 +421,18 @@ class Context:
         return self._run(lib().psba_hybrid, trace_cap, self._opts(max_iter), extra_init=True)
 
     # ---- stages ----
+    def hybrid_report(self, max_iter=50):
+        """The hybrid loop with the reference's report (PSBA/main.cpp:209-225)
+        plus the true RMS, LM/dogleg split and device time per stage."""
+        rep = Report()
+        it = ct.c_int(self.itno)
+        _check(lib().psba_hybrid_report(self._h, ct.byref(self._opts(max_iter)), ct.byref(it), ct.byref(rep)),
+               "psba_hybrid_report")
+        self.itno = it.value
+        out = {k: getattr(rep, k) for k, _ in Report._fields_ if k != "stages"}
+        out["stages"] = {k: getattr(rep.stages, k) for k, _ in ReportStages._fields_}
+        return out
+
     def residual(self, trial=False):
         c = ct.c_double()
         _check(lib().psba_stage_residual(self._h, int(trial), ct.byref(c)), "residual")
@@ -422,6 +451,9 @@ class Context:
 
     def backsub(self):
         _check(lib().psba_stage_backsub(self._h), "backsub")
+
+    def refine(self, mu, sweeps=1):
+        _check(lib().psba_stage_refine(self._h, mu, sweeps), "psba_stage_refine")
 
     def modchol(self):
         lam = ct.c_double()

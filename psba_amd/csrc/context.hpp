@@ -96,6 +96,11 @@ struct psba_ctx {
   double *S = nullptr, *ea = nullptr, *xsol = nullptr, *dp = nullptr, *eb = nullptr;
   double *vecA = nullptr, *vecB = nullptr, *vecC = nullptr, *vecD = nullptr;  // n_tot scratch
   double* mc_work = nullptr;   // modified Cholesky scratch (3*n_pad)
+  // mixed path's FP64 refinement: per-observation products, camera-chunk partials
+  double* ref_obs = nullptr;   // 2 (cnp + 3) n_obs
+  double* ref_part = nullptr;  // 2 cnp per camera chunk
+  long ref_part_cap = 0;
+  int refine_steps = 0;        // sweeps per trial (psba_opts.refine_steps)
   // fused factor + solve (k_chol_solve): factor, right-hand side copy, y = L^-1 b,
   // grid-barrier counter (only grows; bar_base is its value before a launch)
   double *Lf = nullptr, *bw = nullptr, *yv = nullptr;
@@ -146,11 +151,18 @@ int launch_axpby(psba_ctx* c, double* out, double alpha, const double* a, double
 
 // ---- launchers (k_schur.hip) ----
 int launch_schur(psba_ctx* c, double mu);                 // S (lower), ea
+int launch_ea_rhs(psba_ctx* c, const double* ga, const double* z, double* out);  // ga - sum W z
 
 // ---- launchers (k_dense.hip) ----
 int launch_factor_solve(psba_ctx* c);                     // S -> Lf, SC_FACTOR_OK; dp[0..n) = S^-1 ea
 int launch_modchol(psba_ctx* c, double* S_src);           // SC_LAMBDA (factor in S)
 int launch_copy_S(psba_ctx* c, double* dst, const double* src);
+int launch_tri_solve(psba_ctx* c, const double* b, double* out);  // S x = b with the factor in Lf
+
+// ---- mixed-path refinement (k_obs.hip) ----
+int launch_refine(psba_ctx* c, double mu, int sweeps);           // correction of dp -> vecB, g <- FP64 J^T e
+int refine_step(psba_ctx* c, double mu, int sweeps);             // guarded sweeps (drivers.cpp)
+int launch_backsub_rhs(psba_ctx* c, const double* gb, double* dp_out);
 
 // ---- index (ctx.cpp, k_index.hip) ----
 int check_obs_order(int nc, int np, int no, const int* iidx, const int* jidx);  // host validation

@@ -265,6 +265,8 @@ int psba_destroy(psba_ctx* c) {
   dfree(c->JA32);
   dfree(c->JB32);
   dfree(c->W32);
+  dfree(c->ref_obs);
+  dfree(c->ref_part);
   dfree(c->bar);
   dfree(c->chol_fail);
   if (c->h_scal) (void)hipHostFree(c->h_scal);
@@ -596,6 +598,18 @@ int psba_stage_factor_solve(psba_ctx* c, int* ok) {
 int psba_stage_backsub(psba_ctx* c) {
   PSBA_TRY(ready(c));
   PSBA_TRY(launch_backsub(c));
+  PSBA_HIP_TRY(hipStreamSynchronize(c->stream));
+  return PSBA_OK;
+}
+
+int psba_stage_refine(psba_ctx* c, double mu, int sweeps) {
+  PSBA_TRY(ready(c));
+  if (!c->mixed()) return fail(PSBA_ERR_UNSUPPORTED, "refinement applies to mixed-precision contexts");
+  if (sweeps < 0) return fail(PSBA_ERR_ARG, "sweeps < 0");
+  for (int k = 0; k < sweeps; ++k) {
+    PSBA_TRY(launch_refine(c, mu, 1));
+    PSBA_TRY(launch_axpby(c, c->dp, 1.0, c->dp, 1.0, c->vecB, c->ntot));
+  }
   PSBA_HIP_TRY(hipStreamSynchronize(c->stream));
   return PSBA_OK;
 }

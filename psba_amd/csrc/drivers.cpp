@@ -9,8 +9,10 @@
 //   hybrid  PSBA/main.cpp:193-208
 // The damping is applied on the fly to the undamped U/V blocks, so the
 // reference's update_UV / restore_UVdiag pair has no counterpart.
+#include <chrono>
 #include <cmath>
 #include <cstring>
+#include <vector>
 
 #include "context.hpp"
 
@@ -85,6 +87,10 @@ int trial_solve(psba_ctx* c, double mu) {
     StageTimer t(c, &c->timing.backsub);
     PSBA_TRY(launch_backsub(c));
   }
+  if (c->refine_steps > 0) {
+    StageTimer t(c, &c->timing.other);
+    PSBA_TRY(refine_step(c, mu, c->refine_steps));
+  }
   c->timing.trials++;
   return PSBA_OK;
 }
@@ -97,6 +103,38 @@ int jacobian(psba_ctx* c, double coeff, double coeff_g) {
 
 }  // namespace
 
+
+namespace psba {
+
+// FP64 iterative refinement of the mixed path's step dp (at damping mu):
+// corrections delta_k from the FP64 residual of the damped normal equations,
+// solved with the FP32-assembled blocks and their FP64 factor.  Refinement
+// contracts while that solve's relative error times the conditioning stays
+// below 1; at small damping it can exceed 1 and the corrections grow.  The
+// sweeps therefore stop at the first correction larger than half the previous
+// one, and if that happens at the second sweep the step returns to the
+// unrefined one (the first correction was not a contraction either).
+int refine_step(psba_ctx* c, double mu, int sweeps) {
+  PSBA_TRY(launch_axpby(c, c->vecD, 1.0, c->dp, 0.0, c->dp, c->ntot));  // the unrefined step
+  double prev = 0.0;
+  for (int k = 0; k < sweeps + 1; ++k) {
+    PSBA_TRY(launch_refine(c, mu, 1));
+    PSBA_TRY(launch_dot(c, c->vecB, c->vecB, SC_D5));
+    PSBA_TRY(fetch(c));
+    const double d = c->h_scal[SC_D5];
+    if (!std::isfinite(d) || (k > 0 && d > 0.25 * prev)) {
+      if (k == 1) PSBA_TRY(launch_axpby(c, c->dp, 1.0, c->vecD, 0.0, c->vecD, c->ntot));
+      return PSBA_OK;
+    }
+    if (k == sweeps) return PSBA_OK;  // the last pass only measured the correction
+    PSBA_TRY(launch_axpby(c, c->dp, 1.0, c->dp, 1.0, c->vecB, c->ntot));
+    prev = d;
+  }
+  return PSBA_OK;
+}
+
+}  // namespace psba
+
 extern "C" {
 
 int psba_levmar(psba_ctx* c, const psba_opts* o, int* itno, double* finalErr, psba_trace_rec* trace, int cap,
@@ -106,6 +144,7 @@ int psba_levmar(psba_ctx* c, const psba_opts* o, int* itno, double* finalErr, ps
   const int max_iter = opt_max_iter(o);
   const bool no_handoff = o && (o->flags & PSBA_FLAG_NO_HANDOFF);
   const double rel_tol = o ? o->rel_tol : 0.0;
+  c->refine_steps = (o && c->mixed() && o->refine_steps > 0) ? o->refine_steps : 0;
   double mu = 0.0, p_L2 = 0.0;
   int gooditer_cnt = 0, nu = 2;
   bool first = true, p_pending = false;
@@ -116,6 +155,8 @@ int psba_levmar(psba_ctx* c, const psba_opts* o, int* itno, double* finalErr, ps
   for (; *itno < max_iter && flag == PSBA_ITER_CONTINUE; (*itno)++) {
     PSBA_TRY(jacobian(c, 1.0, 1.0));
     if (first) {
+      // with refinement the mixed path takes mu0 from the FP64 diagonal too
+      if (c->refine_steps > 0) PSBA_TRY(launch_refine(c, 0.0, -1));
       PSBA_TRY(launch_maxdiag(c));
       PSBA_TRY(fetch(c));
       mu = kInitMu * c->h_scal[SC_MAXDIAG];
@@ -435,6 +476,60 @@ int psba_hybrid(psba_ctx* c, const psba_opts* o, int* itno, double* initErr, dou
     if (flag != PSBA_ITER_TURN_TO_LM) break;
   }
   return flag;
+}
+
+int psba_hybrid_report(psba_ctx* c, const psba_opts* o, int* itno, psba_report* out) {
+  PSBA_TRY(check_ready(c));
+  if (!itno || !out) return PSBA_ERR_ARG;
+  std::memset(out, 0, sizeof(*out));
+  const bool was_on = c->timing_on;
+  PSBA_TRY(psba_enable_timing(c, 1));
+  c->timing = Timing();
+  std::vector<psba_trace_rec> tr(1 << 16);
+  int len = 0;
+  double e0 = 0, e1 = 0;
+  const auto t0 = std::chrono::steady_clock::now();
+  const int code = psba_hybrid(c, o, itno, &e0, &e1, tr.data(), (int)tr.size(), &len);
+  const auto t1 = std::chrono::steady_clock::now();
+  PSBA_TRY(psba_enable_timing(c, was_on ? 1 : 0));
+  if (code < 0) return code;
+  out->code = code;
+  out->itno = *itno;
+  out->init_sse = e0;
+  out->final_sse = e1;
+  out->init_error_ref = std::sqrt(e0) / c->no;
+  out->final_error_ref = std::sqrt(e1) / c->no;
+  out->init_rms_px = std::sqrt(e0 / c->no);
+  out->final_rms_px = std::sqrt(e1 / c->no);
+  out->wall_ms = std::chrono::duration<double, std::milli>(t1 - t0).count();
+  const int n = std::min(len, (int)tr.size());
+  int prev_kind = -1, prev_it = -1;
+  for (int k = 0; k < n; ++k) {
+    const psba_trace_rec& r = tr[k];
+    if (r.kind == 0)
+      out->lm_trials++;
+    else
+      out->tr_steps++;
+    if (r.kind != prev_kind || r.itno != prev_it) {
+      if (r.kind == 0)
+        out->lm_iterations++;
+      else
+        out->tr_iterations++;
+    }
+    if (prev_kind >= 0 && r.kind != prev_kind) out->handoffs++;
+    prev_kind = r.kind;
+    prev_it = r.itno;
+  }
+  out->stages.jacobian_normal = c->timing.jac;
+  out->stages.schur = c->timing.schur;
+  out->stages.factor = c->timing.factor;
+  out->stages.solve = c->timing.solve;
+  out->stages.backsub = c->timing.backsub;
+  out->stages.residual = c->timing.resid;
+  out->stages.other = c->timing.other;
+  out->stages.trials = c->timing.trials;
+  out->stages.jacobians = c->timing.jacobians;
+  return code;
 }
 
 }  // extern "C"

@@ -802,6 +802,57 @@ __global__ __launch_bounds__(256) void k_chol_solve(double* __restrict__ A, int 
 
 __global__ void k_zero_int(int* p) { *p = 0; }
 
+// S x = b with the factor L already in Lf (one workgroup; used by the mixed
+// path's FP64 refinement): forward L y = b, then back L^T x = y, tile by tile.
+// b has n entries (zero-padded to T tiles here); y and x are n_pad scratch.
+
+// s = sum_c Lt[r][c] x[c] for row r = lane of a 64x64 tile, in a fixed order
+__device__ __forceinline__ double tile_matvec(CholSmem& sm, const double* Lt, int ld, const double* x) {
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  double s = 0.0;
+#pragma unroll
+  for (int c = 16 * w; c < 16 * w + 16; ++c) s += Lt[(size_t)lane * ld + c] * x[c];
+  sm.part[w][lane] = s;
+  __syncthreads();
+  const double t = ((sm.part[0][lane] + sm.part[1][lane]) + sm.part[2][lane]) + sm.part[3][lane];
+  __syncthreads();
+  return t;
+}
+
+__global__ __launch_bounds__(256) void k_tri_solve(const double* __restrict__ Lf, int ld, int T, int n,
+                                                   const double* __restrict__ b, double* __restrict__ y,
+                                                   double* __restrict__ x, double* __restrict__ out) {
+  __shared__ CholSmem sm;
+  const int tid = threadIdx.x, lane = tid & 63;
+  auto tile = [&](int i, int j) { return Lf + (size_t)i * NB * ld + (size_t)j * NB; };
+  for (int i = 0; i < T; ++i) {
+    const int q = i * NB + lane;
+    double br = (q < n) ? b[q] : 0.0;
+    for (int k = 0; k < i; ++k) br -= tile_matvec(sm, tile(i, k), ld, y + (size_t)k * NB);
+    load_lower(sm.P, tile(i, i), ld);
+    if (tid < NB) sm.v[tid] = br;
+    __syncthreads();
+    tile_forward(sm.P, sm.v, y + (size_t)i * NB, sm.w);
+    __syncthreads();
+  }
+  for (int i = T - 1; i >= 0; --i) {
+    double br = y[(size_t)i * NB + lane];
+    for (int k = i + 1; k < T; ++k) br -= tile_tmatvec(sm, tile(k, i), ld, x + (size_t)k * NB);
+    load_lower(sm.P, tile(i, i), ld);
+    __syncthreads();
+    tile_backward(sm.P, br, x + (size_t)i * NB, sm.v);
+    __syncthreads();
+    if (tid < NB && i * NB + tid < n) out[i * NB + tid] = sm.v[tid];
+  }
+}
+
+int launch_tri_solve(psba_ctx* c, const double* b, double* out) {
+  hipLaunchKernelGGL(k_tri_solve, dim3(1), dim3(256), 0, c->stream, c->Lf, c->n_pad, c->n_pad / NB, c->n, b, c->yv,
+                     c->xsol, out);
+  PSBA_LAUNCH_TRY();
+  return PSBA_OK;
+}
+
 int launch_factor_solve(psba_ctx* c) {
   const int T = c->n_pad / NB;
   if (c->chol_grid <= 0) {

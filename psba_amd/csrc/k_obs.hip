@@ -8,6 +8,8 @@
 // compute_Vinv.cl (damped V^-1), compute_eb.cl + compute_dpb.cl
 // (back-substitution), compute_newp.cl + update_p.cl (commit),
 // compute_Jmultiply.cl (J x for the dogleg's model decrease).
+#include <algorithm>
+
 #include "context.hpp"
 #include "model.hpp"
 
@@ -542,6 +544,237 @@ __global__ __launch_bounds__(256) void k_axpby(double* __restrict__ out, double 
 
 int launch_axpby(psba_ctx* c, double* out, double alpha, const double* a, double beta, const double* b, long n) {
   hipLaunchKernelGGL(k_axpby, dim3(ceil_div(n, 256)), dim3(256), 0, c->stream, out, alpha, a, beta, b, n);
+  PSBA_LAUNCH_TRY();
+  return PSBA_OK;
+}
+
+// ---------------------------------------------------------------------------
+// FP64 iterative refinement of the mixed path's step
+// ---------------------------------------------------------------------------
+// The mixed path assembles the reduced system from FP32 blocks, so its step dp
+// solves the damped normal equations (J^T J + mu I) dp = g only to FP32
+// accuracy.  A refinement sweep recomputes, per observation in FP64, the
+// Jacobian rows and residual (with the loss weights), forms
+//   r = J^T (e - J dp) - mu dp        (and the FP64 gradient g = J^T e),
+// reduces r with the existing factor and blocks (the same Schur steps as the
+// step itself) and adds the correction to dp.
+template <int CNP>
+__global__ __launch_bounds__(256) void k_refine_obs(const double* __restrict__ K, const double* __restrict__ q0,
+                                                    const double* __restrict__ cams, const double* __restrict__ pts,
+                                                    const double* __restrict__ meas, const int* __restrict__ obs_cam,
+                                                    const int* __restrict__ obs_pt,
+                                                    const unsigned char* __restrict__ fix, int no, int n,
+                                                    const double* __restrict__ dp, int loss, double loss_d,
+                                                    double* __restrict__ rA, double* __restrict__ rB,
+                                                    double* __restrict__ gA, double* __restrict__ gB, int diag) {
+  const int o = blockIdx.x * blockDim.x + threadIdx.x;
+  if (o >= no) return;
+  const int j = obs_cam[o], i = obs_pt[o];
+  double cam[CNP], M[3], uv[2], ja[2 * CNP], jb[6];
+#pragma unroll
+  for (int k = 0; k < CNP; ++k) cam[k] = cams[j * CNP + k];
+#pragma unroll
+  for (int k = 0; k < 3; ++k) M[k] = pts[3 * i + k];
+  project<double, CNP, true>(K + 5 * j, q0 + 4 * j, cam, M, uv, ja, jb);
+#pragma unroll
+  for (int k = 0; k < CNP; ++k)
+    if (fix[j * CNP + k]) ja[k] = ja[CNP + k] = 0.0;
+  double e0 = meas[2 * o] - uv[0], e1 = meas[2 * o + 1] - uv[1];
+  if (loss) {
+    double rho, drho;
+    loss_eval(loss, loss_d, e0 * e0 + e1 * e1, rho, drho);
+    const double w = sqrt(drho);
+#pragma unroll
+    for (int k = 0; k < 2 * CNP; ++k) ja[k] *= w;
+#pragma unroll
+    for (int k = 0; k < 6; ++k) jb[k] *= w;
+    e0 *= w;
+    e1 *= w;
+  }
+  if (diag) {
+    // diagonal of J^T J (for mu0) in place of r; g as usual
+#pragma unroll
+    for (int k = 0; k < CNP; ++k) {
+      rA[(size_t)o * CNP + k] = ja[k] * ja[k] + ja[CNP + k] * ja[CNP + k];
+      gA[(size_t)o * CNP + k] = ja[k] * e0 + ja[CNP + k] * e1;
+    }
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      rB[(size_t)o * 3 + k] = jb[k] * jb[k] + jb[3 + k] * jb[3 + k];
+      gB[(size_t)o * 3 + k] = jb[k] * e0 + jb[3 + k] * e1;
+    }
+    return;
+  }
+  double t0 = 0.0, t1 = 0.0;
+#pragma unroll
+  for (int k = 0; k < CNP; ++k) {
+    const double d = dp[j * CNP + k];
+    t0 += ja[k] * d;
+    t1 += ja[CNP + k] * d;
+  }
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    const double d = dp[n + 3 * i + k];
+    t0 += jb[k] * d;
+    t1 += jb[3 + k] * d;
+  }
+  const double u0 = e0 - t0, u1 = e1 - t1;
+#pragma unroll
+  for (int k = 0; k < CNP; ++k) {
+    rA[(size_t)o * CNP + k] = ja[k] * u0 + ja[CNP + k] * u1;
+    gA[(size_t)o * CNP + k] = ja[k] * e0 + ja[CNP + k] * e1;
+  }
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    rB[(size_t)o * 3 + k] = jb[k] * u0 + jb[3 + k] * u1;
+    gB[(size_t)o * 3 + k] = jb[k] * e0 + jb[3 + k] * e1;
+  }
+}
+
+// per camera chunk: sums of rA and gA over the chunk's observations (in order)
+template <int CNP>
+__global__ __launch_bounds__(64) void k_refine_cam_chunks(const int* __restrict__ cam_obs, const int* __restrict__ cb,
+                                                          const int* __restrict__ ce, const double* __restrict__ rA,
+                                                          const double* __restrict__ gA, double* __restrict__ part) {
+  const int b = blockIdx.x, t = threadIdx.x;
+  if (t >= 2 * CNP) return;
+  const double* src = t < CNP ? rA : gA;
+  const int k = t < CNP ? t : t - CNP;
+  double acc = 0.0;
+  for (int s = cb[b]; s < ce[b]; ++s) acc += src[(size_t)cam_obs[s] * CNP + k];
+  part[(size_t)b * 2 * CNP + t] = acc;
+}
+
+template <int CNP>
+__global__ __launch_bounds__(64) void k_refine_cam_finish(const int* __restrict__ ccp, const double* __restrict__ part,
+                                                          int nc, double mu, const double* __restrict__ dp,
+                                                          double* __restrict__ r, double* __restrict__ g) {
+  const int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= nc * CNP) return;
+  const int j = q / CNP, k = q - j * CNP;
+  double ar = 0.0, ag = 0.0;
+  for (int c = ccp[j]; c < ccp[j + 1]; ++c) {
+    ar += part[(size_t)c * 2 * CNP + k];
+    ag += part[(size_t)c * 2 * CNP + CNP + k];
+  }
+  r[q] = mu != 0.0 ? ar - mu * dp[q] : ar;
+  g[q] = ag;
+}
+
+__global__ __launch_bounds__(256) void k_refine_pt(const int* __restrict__ pt_ptr, const double* __restrict__ rB,
+                                                   const double* __restrict__ gB, int np, int n, double mu,
+                                                   const double* __restrict__ dp, const double* __restrict__ Vinv,
+                                                   double* __restrict__ r, double* __restrict__ g,
+                                                   double* __restrict__ z) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= np) return;
+  double ar[3] = {0, 0, 0}, ag[3] = {0, 0, 0};
+  for (int o = pt_ptr[i]; o < pt_ptr[i + 1]; ++o)
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      ar[k] += rB[(size_t)o * 3 + k];
+      ag[k] += gB[(size_t)o * 3 + k];
+    }
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    if (mu != 0.0) ar[k] -= mu * dp[n + 3 * i + k];
+    r[n + 3 * i + k] = ar[k];
+    g[n + 3 * i + k] = ag[k];
+  }
+  if (!z) return;
+  // z = V^-1 r_b for the reduced right-hand side
+  const double* w = Vinv + (size_t)i * 6;
+  z[3 * i + 0] = w[0] * ar[0] + w[1] * ar[1] + w[2] * ar[2];
+  z[3 * i + 1] = w[1] * ar[0] + w[3] * ar[1] + w[4] * ar[2];
+  z[3 * i + 2] = w[2] * ar[0] + w[4] * ar[1] + w[5] * ar[2];
+}
+
+template <int CNP>
+static void refine_pass(psba_ctx* c, double mu, int diag, double* r, double* z) {
+  const int C = CNP;
+  double* rA = c->ref_obs;
+  double* gA = rA + (size_t)C * c->no;
+  double* rB = gA + (size_t)C * c->no;
+  double* gB = rB + 3 * (size_t)c->no;
+  hipLaunchKernelGGL(k_refine_obs<CNP>, dim3(ceil_div(c->no, 256)), dim3(256), 0, c->stream, c->K, c->q0, c->cams,
+                     c->pts, c->meas, c->obs_cam, c->obs_pt, c->fix, c->no, c->n, c->dp, c->loss, c->loss_scale, rA, rB,
+                     gA, gB, diag);
+  if (c->n_cchunk > 0)
+    hipLaunchKernelGGL(k_refine_cam_chunks<CNP>, dim3(c->n_cchunk), dim3(64), 0, c->stream, c->cam_obs,
+                       c->cchunk_begin, c->cchunk_end, rA, gA, c->ref_part);
+  hipLaunchKernelGGL(k_refine_cam_finish<CNP>, dim3(ceil_div((long)c->nc * CNP, 64)), dim3(64), 0, c->stream,
+                     c->cam_cchunk_ptr, c->ref_part, c->nc, mu, c->dp, r, c->g);
+  hipLaunchKernelGGL(k_refine_pt, dim3(ceil_div(c->np, 256)), dim3(256), 0, c->stream, c->pt_ptr, rB, gB, c->np, c->n,
+                     mu, c->dp, c->Vinv, r, c->g, z);
+}
+
+template <int CNP>
+static int run_refine(psba_ctx* c, double mu, int sweeps) {
+  if (sweeps < 0) {
+    // FP64 diagonal of J^T J into uvdiag (and the FP64 gradient into g)
+    refine_pass<CNP>(c, 0.0, 1, c->uvdiag, nullptr);
+    return PSBA_OK;
+  }
+  // one correction: delta (vecB) solving the reduced system for
+  // r = J^T (e - J dp) - mu dp; dp is not changed here
+  refine_pass<CNP>(c, mu, 0, c->vecA, c->vecC);
+  PSBA_TRY(launch_ea_rhs(c, c->vecA, c->vecC, c->mc_work));
+  PSBA_TRY(launch_tri_solve(c, c->mc_work, c->vecB));
+  PSBA_TRY(launch_backsub_rhs(c, c->vecA + c->n, c->vecB));
+  return PSBA_OK;
+}
+
+// sweeps > 0: the correction of dp into vecB (and the FP64 gradient into g);
+// sweeps < 0: FP64 diag(J^T J) into uvdiag (for mu0) and the FP64 gradient
+// into g, at the current parameters
+int launch_refine(psba_ctx* c, double mu, int sweeps) {
+  if (sweeps == 0) return PSBA_OK;
+  if (!c->ref_obs) {
+    if (hipMalloc((void**)&c->ref_obs, sizeof(double) * 2 * (c->cnp + 3) * (size_t)c->no) != hipSuccess) {
+      c->ref_obs = nullptr;
+      set_last_error(__FILE__, __LINE__, "hipMalloc of the refinement buffers failed");
+      return PSBA_ERR_NOMEM;
+    }
+  }
+  if (c->ref_part_cap < (long)c->n_cchunk) {
+    if (c->ref_part) (void)hipFree(c->ref_part);
+    c->ref_part = nullptr;
+    c->ref_part_cap = 0;
+    if (hipMalloc((void**)&c->ref_part, sizeof(double) * 2 * c->cnp * (size_t)std::max(c->n_cchunk, 1)) != hipSuccess) {
+      c->ref_part = nullptr;
+      set_last_error(__FILE__, __LINE__, "hipMalloc of the refinement buffers failed");
+      return PSBA_ERR_NOMEM;
+    }
+    c->ref_part_cap = c->n_cchunk;
+  }
+  int rc = PSBA_OK;
+  switch (c->cnp) {
+    case 6: rc = run_refine<6>(c, mu, sweeps); break;
+    case 9: rc = run_refine<9>(c, mu, sweeps); break;
+    case 11: rc = run_refine<11>(c, mu, sweeps); break;
+    case 16: rc = run_refine<16>(c, mu, sweeps); break;
+    default: return PSBA_ERR_UNSUPPORTED;
+  }
+  if (rc != PSBA_OK) return rc;
+  PSBA_LAUNCH_TRY();
+  return PSBA_OK;
+}
+
+// dp_out[n..) = V^-1 (gb - W^T dp_out[0..n)) with a caller-given right-hand side
+template <int CNP>
+static void run_backsub_rhs(psba_ctx* c, const double* gb, double* dp_out) {
+  if (c->mixed())
+    hipLaunchKernelGGL((k_backsub<CNP, float>), dim3(ceil_div(c->np, 256)), dim3(256), 0, c->stream, c->pt_ptr,
+                       c->obs_cam, c->W32, c->Vinv, gb, c->np, c->n, dp_out, c->eb, (const int*)nullptr,
+                       (const unsigned char*)nullptr);
+  else
+    hipLaunchKernelGGL((k_backsub<CNP, double>), dim3(ceil_div(c->np, 256)), dim3(256), 0, c->stream, c->pt_ptr,
+                       c->obs_cam, c->W, c->Vinv, gb, c->np, c->n, dp_out, c->eb, (const int*)nullptr,
+                       (const unsigned char*)nullptr);
+}
+
+int launch_backsub_rhs(psba_ctx* c, const double* gb, double* dp_out) {
+  PSBA_DISPATCH_CNP(c->cnp, run_backsub_rhs, c, gb, dp_out);
   PSBA_LAUNCH_TRY();
   return PSBA_OK;
 }

@@ -180,6 +180,34 @@ static void run_schur(psba_ctx* c, double mu) {
     run_schur_t<CNP, double>(c, mu, c->W, c->schpart, c->eapart);
 }
 
+// e_a = ga - sum_i W_ij z_i with caller-given ga (n) and z (3 n_p) -> out (n)
+template <int CNP>
+static void run_ea_rhs(psba_ctx* c, const double* ga, const double* z, double* out) {
+  if (c->mixed()) {
+    if (c->n_cchunk > 0)
+      hipLaunchKernelGGL((k_ea_chunks<CNP, float>), dim3(c->n_cchunk), dim3(256), 0, c->stream, c->cam_obs,
+                         c->cchunk_begin, c->cchunk_end, c->obs_pt, c->W32, z, c->eapart32, (const int*)nullptr,
+                         (const unsigned char*)nullptr);
+    hipLaunchKernelGGL((k_ea_finish<CNP, float>), dim3(ceil_div((long)c->nc * CNP, 64)), dim3(64), 0, c->stream,
+                       c->cam_cchunk_ptr, c->eapart32, ga, c->nc, out, (const int*)nullptr,
+                       (const unsigned char*)nullptr);
+  } else {
+    if (c->n_cchunk > 0)
+      hipLaunchKernelGGL((k_ea_chunks<CNP, double>), dim3(c->n_cchunk), dim3(256), 0, c->stream, c->cam_obs,
+                         c->cchunk_begin, c->cchunk_end, c->obs_pt, c->W, z, c->eapart, (const int*)nullptr,
+                         (const unsigned char*)nullptr);
+    hipLaunchKernelGGL((k_ea_finish<CNP, double>), dim3(ceil_div((long)c->nc * CNP, 64)), dim3(64), 0, c->stream,
+                       c->cam_cchunk_ptr, c->eapart, ga, c->nc, out, (const int*)nullptr,
+                       (const unsigned char*)nullptr);
+  }
+}
+
+int launch_ea_rhs(psba_ctx* c, const double* ga, const double* z, double* out) {
+  PSBA_DISPATCH_CNP(c->cnp, run_ea_rhs, c, ga, z, out);
+  PSBA_LAUNCH_TRY();
+  return PSBA_OK;
+}
+
 int launch_schur(psba_ctx* c, double mu) {
   switch (c->cnp) {
     case 6: run_schur<6>(c, mu); break;
