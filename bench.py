@@ -88,6 +88,19 @@ def mixed_section(P, sc, a, t_plain, rms):
     t_m0, r_m0 = timed(P.Context.from_scene(sc0, precision=P.PREC_MIXED), ms, 1)
     rms_p0 = float(np.sqrt(r_p0["final_err"] / (2 * sc0.n_obs)))
     rms_m0 = float(np.sqrt(r_m0["final_err"] / (2 * sc0.n_obs)))
+    # mixed with one FP64 refinement sweep per step (follows the FP64 trials)
+    kw_r = dict(kw, refine_steps=1)
+    cref = P.Context.from_scene(sc0, precision=P.PREC_MIXED)
+    cref.snapshot()
+    cref.levmar(**kw_r)
+    t0 = time.perf_counter()
+    for _ in range(ms):
+        cref.restore()
+        cref.itno = 0
+        r_r0 = cref.levmar(**kw_r)
+    t_r0 = (time.perf_counter() - t0) / ms
+    rms_r0 = float(np.sqrt(r_r0["final_err"] / (2 * sc0.n_obs)))
+    step_err = step_errors(P, sc)
     res = {
         "solves_per_s": 1.0 / t_mix, "ms_per_solve": 1e3 * t_mix, "speedup_vs_plain": t_plain / t_mix,
         "lm_iterations": r_mix["itno"], "final_rms_px": rms_mix,
@@ -95,12 +108,45 @@ def mixed_section(P, sc, a, t_plain, rms):
         "outlier_free": {"plain_ms": 1e3 * t_p0, "mixed_ms": 1e3 * t_m0, "speedup": t_p0 / t_m0,
                          "plain_iterations": r_p0["itno"], "mixed_iterations": r_m0["itno"],
                          "plain_rms_px": rms_p0, "mixed_rms_px": rms_m0,
-                         "rel_rms": abs(rms_m0 - rms_p0) / rms_p0},
+                         "rel_rms": abs(rms_m0 - rms_p0) / rms_p0,
+                         "mixed_refined_ms": 1e3 * t_r0, "mixed_refined_iterations": r_r0["itno"],
+                         "mixed_refined_rel_rms": abs(rms_r0 - rms_p0) / rms_p0},
+        "first_step_rel_error_vs_fp64": step_err,
     }
     log("mixed: %.3f ms/solve (%.2fx), rms %.6f (rel %.2e); outlier-free: plain %.3f ms, mixed %.3f ms (%.2fx), "
         "rms rel %.2e" % (1e3 * t_mix, t_plain / t_mix, rms_mix, abs(rms_mix - rms) / rms, 1e3 * t_p0,
                           1e3 * t_m0, t_p0 / t_m0, abs(rms_m0 - rms_p0) / rms_p0))
+    log("mixed refined: %.3f ms/solve, %d iterations, rms rel %.2e; first-step error vs FP64 %s" % (
+        1e3 * t_r0, r_r0["itno"], abs(rms_r0 - rms_p0) / rms_p0,
+        ", ".join("%s %.2e" % kv for kv in step_err.items())))
     return res
+
+
+def step_errors(P, sc):
+    """Relative error of the mixed path's first step (at mu0) against the FP64
+    step, unrefined and after one and two FP64 refinement sweeps."""
+    c64 = P.Context.from_scene(sc)
+    c64.jacobian_normal(1.0, 1.0)
+    C = sc.cnp
+    diag = np.concatenate([c64.buffer("U").reshape(-1, C, C).diagonal(axis1=1, axis2=2).ravel(),
+                           c64.buffer("V").reshape(-1, 3, 3).diagonal(axis1=1, axis2=2).ravel()])
+    mu = 1e-3 * float(diag.max())
+    c64.schur(mu)
+    c64.factor_solve()
+    c64.backsub()
+    ref = c64.buffer("DP")
+    cm = P.Context.from_scene(sc, precision=P.PREC_MIXED)
+    cm.jacobian_normal(1.0, 1.0)
+    cm.schur(mu)
+    cm.factor_solve()
+    cm.backsub()
+    out = {}
+    den = float(np.max(np.abs(ref)))
+    for k in range(3):
+        out["sweeps_%d" % k] = float(np.max(np.abs(cm.buffer("DP") - ref)) / den)
+        if k < 2:
+            cm.refine(mu, 1)
+    return out
 
 
 def batch_section(P, seed=1003, n=1024, reps=3):
