@@ -76,29 +76,29 @@ __device__ inline void block_sum(double (&v)[K], double* sh) {
   __syncthreads();
 }
 
-// Two-level deterministic reduction: each block writes its K partials; the
-// last block to finish (ticket counter) sums all partials in block order and
-// writes `out[0..K)`, then resets the counter so the kernel can be relaunched
-// (also under graph replay).
+// Two-level deterministic reduction: each block writes its K partials
+// (block order); k_sum_partials, one block queued behind the kernel on the same
+// stream, adds them in block order and writes out[0..K).  No inter-block
+// atomics or device-scope fences: on a multi-XCD device each such fence forces
+// an L2 write-back, which made the per-block ticket scheme cost tens of
+// microseconds per launch.
 template <int K>
-__device__ inline void grid_sum_finish(double (&v)[K], double* partials, unsigned* counter,
-                                       double* out, double* sh) {
+__device__ inline void grid_sum_partials(double (&v)[K], double* partials, double* sh) {
   block_sum<K>(v, sh);
-  __shared__ bool last;
   if (threadIdx.x == 0) {
 #pragma unroll
     for (int k = 0; k < K; ++k) partials[(size_t)blockIdx.x * K + k] = v[k];
-    __threadfence();
-    unsigned ticket = atomicAdd(counter, 1u);
-    last = (ticket == gridDim.x - 1);
   }
-  __syncthreads();
-  if (!last) return;
-  __threadfence();
+}
+
+template <int K>
+__global__ __launch_bounds__(256) void k_sum_partials(const double* __restrict__ partials, int nb,
+                                                      double* __restrict__ out) {
+  __shared__ double sh[K * 4];
   double acc[K];
 #pragma unroll
   for (int k = 0; k < K; ++k) acc[k] = 0;
-  for (unsigned b = threadIdx.x; b < gridDim.x; b += blockDim.x) {
+  for (int b = threadIdx.x; b < nb; b += blockDim.x) {
 #pragma unroll
     for (int k = 0; k < K; ++k) acc[k] += partials[(size_t)b * K + k];
   }
@@ -106,8 +106,12 @@ __device__ inline void grid_sum_finish(double (&v)[K], double* partials, unsigne
   if (threadIdx.x == 0) {
 #pragma unroll
     for (int k = 0; k < K; ++k) out[k] = acc[k];
-    *counter = 0u;
   }
+}
+
+template <int K>
+inline void launch_sum_partials(hipStream_t st, const double* partials, int nb, double* out) {
+  hipLaunchKernelGGL(k_sum_partials<K>, dim3(1), dim3(256), 0, st, partials, nb, out);
 }
 
 inline unsigned ceil_div(long a, long b) { return (unsigned)((a + b - 1) / b); }

@@ -43,7 +43,6 @@ enum Scalar {
   SC_COUNT = 32
 };
 
-enum Counter { CT_RESID = 0, CT_STEP, CT_COMMIT, CT_DOT, CT_JMUL, CT_MAXD, CT_COUNT = 16 };
 
 struct Timing {
   double jac = 0, schur = 0, factor = 0, solve = 0, backsub = 0, resid = 0, other = 0;
@@ -113,7 +112,6 @@ struct psba_ctx {
   double* snap = nullptr;      // parameter snapshot (psba_snapshot_params)
   double* red_part = nullptr;  // reduction partials
   long red_part_cap = 0;
-  unsigned* counters = nullptr;
   double* d_scal = nullptr;
   double* h_scal = nullptr;  // pinned
 

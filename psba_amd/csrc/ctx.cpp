@@ -217,7 +217,6 @@ int psba_create(const psba_desc* d, psba_ctx** out) {
   chk(dalloc(&c->chol_fail, (size_t)1));
   c->red_part_cap = 8 * (long)std::max(ceil_div(c->no, 256), ceil_div(c->ntot, 256)) + 64;
   chk(dalloc(&c->red_part, (size_t)c->red_part_cap));
-  chk(dalloc(&c->counters, (size_t)CT_COUNT));
   chk(dalloc(&c->d_scal, (size_t)SC_COUNT));
   if (rc == PSBA_OK && hipHostMalloc((void**)&c->h_scal, sizeof(double) * SC_COUNT, hipHostMallocDefault) != hipSuccess)
     rc = fail(PSBA_ERR_NOMEM, "pinned host allocation failed");
@@ -230,7 +229,6 @@ int psba_create(const psba_desc* d, psba_ctx** out) {
     z(hipMemsetAsync(c->S, 0, sizeof(double) * (size_t)c->n_pad * c->n_pad, c->stream));
     z(hipMemsetAsync(c->ea, 0, sizeof(double) * (size_t)c->n_pad, c->stream));
     z(hipMemsetAsync(c->dp, 0, sizeof(double) * (size_t)c->ntot, c->stream));
-    z(hipMemsetAsync(c->counters, 0, sizeof(unsigned) * CT_COUNT, c->stream));
     z(hipMemsetAsync(c->d_scal, 0, sizeof(double) * SC_COUNT, c->stream));
     z(hipMemsetAsync(c->bar, 0, sizeof(unsigned long long), c->stream));
     z(hipMemsetAsync(c->Lf, 0, sizeof(double) * (size_t)c->n_pad * c->n_pad, c->stream));
@@ -260,7 +258,6 @@ int psba_destroy(psba_ctx* c) {
   for (double* p : bufs)
     if (p) (void)hipFree(p);
   dfree(c->fix);
-  dfree(c->counters);
   dfree(c->e32);
   dfree(c->JA32);
   dfree(c->JB32);

@@ -26,8 +26,7 @@ __global__ __launch_bounds__(256) void k_residual(const double* __restrict__ K, 
                                                   const double* __restrict__ dp, int n,
                                                   const double* __restrict__ meas, const int* __restrict__ obs_cam,
                                                   const int* __restrict__ obs_pt, int no, double* __restrict__ e_out,
-                                                  double* partials, unsigned* counter, double* out, int loss,
-                                                  double loss_d) {
+                                                  double* partials, int loss, double loss_d) {
   __shared__ double sh[4];
   double acc[1] = {0.0};
   int o = blockIdx.x * blockDim.x + threadIdx.x;
@@ -50,7 +49,7 @@ __global__ __launch_bounds__(256) void k_residual(const double* __restrict__ K, 
       loss_eval(loss, loss_d, acc[0], acc[0], drho);
     }
   }
-  grid_sum_finish<1>(acc, partials, counter, out, sh);
+  grid_sum_partials<1>(acc, partials, sh);
 }
 
 template <int CNP>
@@ -58,8 +57,8 @@ static void run_residual(psba_ctx* c, bool trial) {
   unsigned nb = ceil_div(c->no, 256);
   hipLaunchKernelGGL(k_residual<CNP>, dim3(nb), dim3(256), 0, c->stream, c->K, c->q0, c->cams, c->pts,
                      trial ? c->dp : nullptr, c->n, c->meas, c->obs_cam, c->obs_pt, c->no,
-                     trial ? nullptr : c->e, c->red_part, c->counters + CT_RESID, c->d_scal + SC_COST, c->loss,
-                     c->loss_scale);
+                     trial ? nullptr : c->e, c->red_part, c->loss, c->loss_scale);
+  launch_sum_partials<1>(c->stream, c->red_part, (int)nb, c->d_scal + SC_COST);
 }
 
 int launch_residual(psba_ctx* c, bool trial) {
@@ -141,6 +140,9 @@ __global__ __launch_bounds__(256) void k_point_normal(const int* __restrict__ pt
   if (i >= np) return;
   if (run && !run[ownr[i]]) return;
   R v[6] = {0, 0, 0, 0, 0, 0}, g[3] = {0, 0, 0};
+  // unrolled so that several observations' loads are in flight at once (a
+  // long track otherwise waits one load latency per observation)
+#pragma unroll 4
   for (int o = pt_ptr[i]; o < pt_ptr[i + 1]; ++o) {
     const R* b = JB + (size_t)o * 6;
     R b0 = b[0], b1 = b[1], b2 = b[2], b3 = b[3], b4 = b[4], b5 = b[5];
@@ -364,47 +366,60 @@ int launch_vinv(psba_ctx* c, double mu) {
 // ---------------------------------------------------------------------------
 // back-substitution: eb_i = g_b,i - sum_j W_ij^T dpa_j ; dpb_i = V_i^-1 eb_i
 // ---------------------------------------------------------------------------
+// Three lanes per point, one per component of eb_i: each lane keeps the single
+// running sum over the point's observations (ascending camera order) and
+// their CNP products, the order of PSBA/CL_files/compute_eb.cl:20-36, so the
+// result does not depend on the launch shape.  A wavefront holds 21 points
+// (lane 63 idles); the three eb values meet through lane shuffles for
+// dpb_i = V_i^-1 eb_i (V_i^-1 packed as its 6 unique entries).
+constexpr int kBacksubPtsPerWave = 21;
+
 template <int CNP, typename R>
 __global__ __launch_bounds__(256) void k_backsub(const int* __restrict__ pt_ptr, const int* __restrict__ obs_cam,
                                                  const R* __restrict__ W, const double* __restrict__ Vinv,
                                                  const double* __restrict__ gb, int np, int n,
                                                  double* __restrict__ dp, double* __restrict__ eb,
                                                  const int* __restrict__ ownr, const unsigned char* __restrict__ run) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= np) return;
-  if (run && !run[ownr[i]]) return;
-  R s[3] = {0, 0, 0};
-  for (int o = pt_ptr[i]; o < pt_ptr[i + 1]; ++o) {
-    const R* w = W + (size_t)o * 3 * CNP;
-    const double* da = dp + obs_cam[o] * CNP;
+  const int lane = threadIdx.x & 63;
+  const int grp = lane / 3;
+  const int cc = lane - 3 * grp;
+  const int i = (blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6)) * kBacksubPtsPerWave + grp;
+  const bool ok = grp < kBacksubPtsPerWave && i < np && (!run || run[ownr[i]]);
+  R s = 0;
+  if (ok) {
+    for (int o = pt_ptr[i]; o < pt_ptr[i + 1]; ++o) {
+      const R* w = W + (size_t)o * 3 * CNP + cc;
+      const double* da = dp + obs_cam[o] * CNP;
 #pragma unroll
-    for (int k = 0; k < CNP; ++k) {
-      R d = (R)da[k];
-      s[0] += w[3 * k] * d;
-      s[1] += w[3 * k + 1] * d;
-      s[2] += w[3 * k + 2] * d;
+      for (int k = 0; k < CNP; ++k) s += w[3 * k] * (R)da[k];
     }
   }
-  double e0 = gb[3 * i] - (double)s[0], e1 = gb[3 * i + 1] - (double)s[1], e2 = gb[3 * i + 2] - (double)s[2];
-  eb[3 * i] = e0;
-  eb[3 * i + 1] = e1;
-  eb[3 * i + 2] = e2;
-  const double* w = Vinv + (size_t)i * 6;
-  dp[n + 3 * i + 0] = w[0] * e0 + w[1] * e1 + w[2] * e2;
-  dp[n + 3 * i + 1] = w[1] * e0 + w[3] * e1 + w[4] * e2;
-  dp[n + 3 * i + 2] = w[2] * e0 + w[4] * e1 + w[5] * e2;
+  const double e = ok ? gb[3 * (size_t)i + cc] - (double)s : 0.0;
+  const int base = 3 * grp;
+  const double e0 = __shfl(e, base), e1 = __shfl(e, base + 1), e2 = __shfl(e, base + 2);
+  if (!ok) return;
+  eb[3 * (size_t)i + cc] = e;
+  // row cc of the symmetric 3x3 from its packed entries (0 1 2 / 1 3 4 / 2 4 5)
+  const double* v = Vinv + (size_t)i * 6;
+  const double v0 = v[cc], v1 = v[cc == 0 ? 1 : cc + 2], v2 = v[cc == 0 ? 2 : cc + 3];
+  dp[n + 3 * (size_t)i + cc] = v0 * e0 + v1 * e1 + v2 * e2;
+}
+
+template <int CNP, typename R>
+static void backsub_launch(hipStream_t st, const int* pt_ptr, const int* obs_cam, const R* W, const double* Vinv,
+                           const double* gb, int np, int n, double* dp, double* eb, const int* pt_ownr,
+                           const unsigned char* run, int block) {
+  const int per_block = (block / 64) * kBacksubPtsPerWave;
+  hipLaunchKernelGGL((k_backsub<CNP, R>), dim3(ceil_div(np, per_block)), dim3(block), 0, st, pt_ptr, obs_cam, W, Vinv,
+                     gb, np, n, dp, eb, pt_ownr, run);
 }
 
 template <int CNP>
 static void run_backsub(psba_ctx* c) {
   if (c->mixed())
-    hipLaunchKernelGGL((k_backsub<CNP, float>), dim3(ceil_div(c->np, 256)), dim3(256), 0, c->stream, c->pt_ptr,
-                       c->obs_cam, c->W32, c->Vinv, c->g + c->n, c->np, c->n, c->dp, c->eb, (const int*)nullptr,
-                       (const unsigned char*)nullptr);
+    backsub_launch<CNP, float>(c->stream, c->pt_ptr, c->obs_cam, c->W32, c->Vinv, c->g + c->n, c->np, c->n, c->dp, c->eb, (const int*)nullptr, (const unsigned char*)nullptr, 256);
   else
-    hipLaunchKernelGGL((k_backsub<CNP, double>), dim3(ceil_div(c->np, 256)), dim3(256), 0, c->stream, c->pt_ptr,
-                       c->obs_cam, c->W, c->Vinv, c->g + c->n, c->np, c->n, c->dp, c->eb, (const int*)nullptr,
-                       (const unsigned char*)nullptr);
+    backsub_launch<CNP, double>(c->stream, c->pt_ptr, c->obs_cam, c->W, c->Vinv, c->g + c->n, c->np, c->n, c->dp, c->eb, (const int*)nullptr, (const unsigned char*)nullptr, 256);
 }
 
 int launch_backsub(psba_ctx* c) {
@@ -417,8 +432,7 @@ int launch_backsub(psba_ctx* c) {
 // step statistics, commit, dot products
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(256) void k_step_stats(const double* __restrict__ dp, const double* __restrict__ g,
-                                                    long ntot, double mu, double* partials, unsigned* counter,
-                                                    double* out) {
+                                                    long ntot, double mu, double* partials) {
   __shared__ double sh[8];
   double acc[2] = {0.0, 0.0};
   long q = (long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -427,19 +441,19 @@ __global__ __launch_bounds__(256) void k_step_stats(const double* __restrict__ d
     acc[0] = d * d;
     acc[1] = d * (mu * d + g[q]);
   }
-  grid_sum_finish<2>(acc, partials, counter, out, sh);
+  grid_sum_partials<2>(acc, partials, sh);
 }
 
 int launch_step_stats(psba_ctx* c, double mu) {
-  hipLaunchKernelGGL(k_step_stats, dim3(ceil_div(c->ntot, 256)), dim3(256), 0, c->stream, c->dp, c->g,
-                     (long)c->ntot, mu, c->red_part, c->counters + CT_STEP, c->d_scal + SC_DP2);
+  const unsigned nb = ceil_div(c->ntot, 256);
+  hipLaunchKernelGGL(k_step_stats, dim3(nb), dim3(256), 0, c->stream, c->dp, c->g, (long)c->ntot, mu, c->red_part);
+  launch_sum_partials<2>(c->stream, c->red_part, (int)nb, c->d_scal + SC_DP2);
   PSBA_LAUNCH_TRY();
   return PSBA_OK;
 }
 
 __global__ __launch_bounds__(256) void k_commit(double* __restrict__ cams, double* __restrict__ pts,
-                                                const double* __restrict__ dp, int n, long ntot, double* partials,
-                                                unsigned* counter, double* out) {
+                                                const double* __restrict__ dp, int n, long ntot, double* partials) {
   __shared__ double sh[4];
   double acc[1] = {0.0};
   long q = (long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -454,12 +468,14 @@ __global__ __launch_bounds__(256) void k_commit(double* __restrict__ cams, doubl
     }
     acc[0] = v * v;
   }
-  grid_sum_finish<1>(acc, partials, counter, out, sh);
+  grid_sum_partials<1>(acc, partials, sh);
 }
 
 int launch_commit(psba_ctx* c) {
-  hipLaunchKernelGGL(k_commit, dim3(ceil_div(c->ntot, 256)), dim3(256), 0, c->stream, c->cams, c->pts, c->dp, c->n,
-                     (long)c->ntot, c->red_part, c->counters + CT_COMMIT, c->d_scal + SC_P2);
+  const unsigned nb = ceil_div(c->ntot, 256);
+  hipLaunchKernelGGL(k_commit, dim3(nb), dim3(256), 0, c->stream, c->cams, c->pts, c->dp, c->n, (long)c->ntot,
+                     c->red_part);
+  launch_sum_partials<1>(c->stream, c->red_part, (int)nb, c->d_scal + SC_P2);
   PSBA_LAUNCH_TRY();
   return PSBA_OK;
 }
@@ -471,8 +487,7 @@ template <int CNP>
 __global__ __launch_bounds__(256) void k_jmul_dots(const double* __restrict__ JA, const double* __restrict__ JB,
                                                    const int* __restrict__ obs_cam, const int* __restrict__ obs_pt,
                                                    int no, int n, const double* __restrict__ x1,
-                                                   const double* __restrict__ x2, double* partials,
-                                                   unsigned* counter, double* out) {
+                                                   const double* __restrict__ x2, double* partials) {
   __shared__ double sh[12];
   double acc[3] = {0.0, 0.0, 0.0};
   int o = blockIdx.x * blockDim.x + threadIdx.x;
@@ -501,14 +516,15 @@ __global__ __launch_bounds__(256) void k_jmul_dots(const double* __restrict__ JA
     acc[1] = y1[0] * y2[0] + y1[1] * y2[1];
     acc[2] = y2[0] * y2[0] + y2[1] * y2[1];
   }
-  grid_sum_finish<3>(acc, partials, counter, out, sh);
+  grid_sum_partials<3>(acc, partials, sh);
 }
 
 template <int CNP>
 static void run_jmul(psba_ctx* c, const double* x1, const double* x2) {
-  hipLaunchKernelGGL(k_jmul_dots<CNP>, dim3(ceil_div(c->no, 256)), dim3(256), 0, c->stream, c->JA, c->JB,
-                     c->obs_cam, c->obs_pt, c->no, c->n, x1, x2, c->red_part, c->counters + CT_JMUL,
-                     c->d_scal + SC_D0);
+  const unsigned nb = ceil_div(c->no, 256);
+  hipLaunchKernelGGL(k_jmul_dots<CNP>, dim3(nb), dim3(256), 0, c->stream, c->JA, c->JB, c->obs_cam, c->obs_pt, c->no,
+                     c->n, x1, x2, c->red_part);
+  launch_sum_partials<3>(c->stream, c->red_part, (int)nb, c->d_scal + SC_D0);
 }
 
 int launch_jmul_dots(psba_ctx* c, const double* x1, const double* x2) {
@@ -518,17 +534,18 @@ int launch_jmul_dots(psba_ctx* c, const double* x1, const double* x2) {
 }
 
 __global__ __launch_bounds__(256) void k_dot(const double* __restrict__ a, const double* __restrict__ b, long n,
-                                             double* partials, unsigned* counter, double* out) {
+                                             double* partials) {
   __shared__ double sh[4];
   double acc[1] = {0.0};
   long q = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (q < n) acc[0] = a[q] * b[q];
-  grid_sum_finish<1>(acc, partials, counter, out, sh);
+  grid_sum_partials<1>(acc, partials, sh);
 }
 
 int launch_dot(psba_ctx* c, const double* a, const double* b, int slot) {
-  hipLaunchKernelGGL(k_dot, dim3(ceil_div(c->ntot, 256)), dim3(256), 0, c->stream, a, b, (long)c->ntot,
-                     c->red_part, c->counters + CT_DOT, c->d_scal + slot);
+  const unsigned nb = ceil_div(c->ntot, 256);
+  hipLaunchKernelGGL(k_dot, dim3(nb), dim3(256), 0, c->stream, a, b, (long)c->ntot, c->red_part);
+  launch_sum_partials<1>(c->stream, c->red_part, (int)nb, c->d_scal + slot);
   PSBA_LAUNCH_TRY();
   return PSBA_OK;
 }
@@ -764,13 +781,9 @@ int launch_refine(psba_ctx* c, double mu, int sweeps) {
 template <int CNP>
 static void run_backsub_rhs(psba_ctx* c, const double* gb, double* dp_out) {
   if (c->mixed())
-    hipLaunchKernelGGL((k_backsub<CNP, float>), dim3(ceil_div(c->np, 256)), dim3(256), 0, c->stream, c->pt_ptr,
-                       c->obs_cam, c->W32, c->Vinv, gb, c->np, c->n, dp_out, c->eb, (const int*)nullptr,
-                       (const unsigned char*)nullptr);
+    backsub_launch<CNP, float>(c->stream, c->pt_ptr, c->obs_cam, c->W32, c->Vinv, gb, c->np, c->n, dp_out, c->eb, (const int*)nullptr, (const unsigned char*)nullptr, 256);
   else
-    hipLaunchKernelGGL((k_backsub<CNP, double>), dim3(ceil_div(c->np, 256)), dim3(256), 0, c->stream, c->pt_ptr,
-                       c->obs_cam, c->W, c->Vinv, gb, c->np, c->n, dp_out, c->eb, (const int*)nullptr,
-                       (const unsigned char*)nullptr);
+    backsub_launch<CNP, double>(c->stream, c->pt_ptr, c->obs_cam, c->W, c->Vinv, gb, c->np, c->n, dp_out, c->eb, (const int*)nullptr, (const unsigned char*)nullptr, 256);
 }
 
 int launch_backsub_rhs(psba_ctx* c, const double* gb, double* dp_out) {
@@ -829,13 +842,9 @@ int launch_b_vinv(psba_batch* b) {
 template <int CNP>
 static void run_b_backsub(psba_batch* b) {
   if (b->mixed())
-    hipLaunchKernelGGL((k_backsub<CNP, float>), dim3(ceil_div(b->NP, b->pt_block)), dim3(b->pt_block), 0, b->stream, b->pt_ptr,
-                       b->obs_cam, (const float*)b->W, b->Vinv, b->g + b->N, (int)b->NP, (int)b->N, b->dp, b->eb,
-                       b->pt_prob, b->active);
+    backsub_launch<CNP, float>(b->stream, b->pt_ptr, b->obs_cam, (const float*)b->W, b->Vinv, b->g + b->N, (int)b->NP, (int)b->N, b->dp, b->eb, b->pt_prob, b->active, b->pt_block);
   else
-    hipLaunchKernelGGL((k_backsub<CNP, double>), dim3(ceil_div(b->NP, b->pt_block)), dim3(b->pt_block), 0, b->stream, b->pt_ptr,
-                       b->obs_cam, (const double*)b->W, b->Vinv, b->g + b->N, (int)b->NP, (int)b->N, b->dp, b->eb,
-                       b->pt_prob, b->active);
+    backsub_launch<CNP, double>(b->stream, b->pt_ptr, b->obs_cam, (const double*)b->W, b->Vinv, b->g + b->N, (int)b->NP, (int)b->N, b->dp, b->eb, b->pt_prob, b->active, b->pt_block);
 }
 
 int launch_b_backsub(psba_batch* b) {

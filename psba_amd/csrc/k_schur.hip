@@ -16,6 +16,17 @@
 
 namespace psba {
 
+// One workgroup per chunk (one camera pair, at most kPairChunk shared points).
+//   1. the chunk's item indices (W_a, W_b observation rows and the point) go to
+//      LDS in one round of loads;
+//   2. four threads per item load W_a, W_b and V^-1 with independent
+//      addresses (no dependent global round trips) and stage Y = W_a V^-1 and
+//      W_b in LDS;
+//   3. thread (g, blk) accumulates a 3x3 block of the CNP x CNP product over
+//      items g, g + G, ... in registers (18 LDS values per 27 multiply-adds);
+//   4. the G group partials are summed in group order into part[b].
+// The order of every sum is fixed by the chunk, so results do not depend on
+// scheduling.
 template <int CNP, typename R>
 __global__ __launch_bounds__(256) void k_schur_chunks(const int2* __restrict__ items, const int* __restrict__ sb,
                                                       const int* __restrict__ se, const int* __restrict__ obs_pt,
@@ -23,10 +34,14 @@ __global__ __launch_bounds__(256) void k_schur_chunks(const int2* __restrict__ i
                                                       R* __restrict__ part, const int* __restrict__ ownr,
                                                       const unsigned char* __restrict__ run, int nchunk) {
   constexpr int C2 = CNP * CNP, REC = 3 * CNP;
-  constexpr int G = (256 / C2) > 0 ? (256 / C2) : 1;
-  __shared__ R sY[kPairChunk * REC];
-  __shared__ R sW[kPairChunk * REC];
-  __shared__ R sG[G * C2];
+  constexpr int NBK = (CNP + 2) / 3;  // 3-wide row / column blocks
+  constexpr int NB2 = NBK * NBK;
+  constexpr int G = 256 / NB2;        // item groups
+  static_assert(G * C2 <= 2 * kPairChunk * REC, "group partials must fit the staging buffer");
+  __shared__ R sBuf[2 * kPairChunk * REC];
+  __shared__ int sIdx[3 * kPairChunk];
+  R* const sY = sBuf;
+  R* const sW = sBuf + kPairChunk * REC;
 #if PSBA_SCHUR_XCD
   const int b = xcd_item(blockIdx.x, nchunk);  // one camera range per XCD
   if (b < 0) return;
@@ -35,40 +50,77 @@ __global__ __launch_bounds__(256) void k_schur_chunks(const int2* __restrict__ i
 #endif
   if (run && !run[ownr[b]]) return;
   const int beg = sb[b], cnt = se[b] - beg;
-  // Y rows: task = (item, row r) -> Y[it][r][0..2] = W_a[r][:] V^-1
-  for (int task = threadIdx.x; task < cnt * CNP; task += blockDim.x) {
-    int it = task / CNP, r = task - it * CNP;
-    int2 pr = items[beg + it];
-    const R* w = W + (size_t)pr.x * REC + 3 * r;
-    const double* vi = Vinv + (size_t)obs_pt[pr.x] * 6;
-    R w0 = w[0], w1 = w[1], w2 = w[2];
-    R v00 = (R)vi[0], v01 = (R)vi[1], v02 = (R)vi[2], v11 = (R)vi[3], v12 = (R)vi[4], v22 = (R)vi[5];
-    R* y = sY + it * REC + 3 * r;
-    y[0] = w0 * v00 + w1 * v01 + w2 * v02;
-    y[1] = w0 * v01 + w1 * v11 + w2 * v12;
-    y[2] = w0 * v02 + w1 * v12 + w2 * v22;
-  }
-  for (int f = threadIdx.x; f < cnt * REC; f += blockDim.x) {
-    int it = f / REC, el = f - it * REC;
-    sW[f] = W[(size_t)items[beg + it].y * REC + el];
+  const int t = threadIdx.x;
+  if (t < cnt) {
+    const int2 pr = items[beg + t];
+    sIdx[t] = pr.x;
+    sIdx[kPairChunk + t] = pr.y;
+    sIdx[2 * kPairChunk + t] = obs_pt[pr.x];
   }
   __syncthreads();
-  const int grp = threadIdx.x / C2, ent = threadIdx.x - grp * C2;
-  if (grp < G) {
-    const int r = ent / CNP, cc = ent - r * CNP;
-    R acc = 0;
-    for (int it = grp; it < cnt; it += G) {
-      const R* y = sY + it * REC + 3 * r;
-      const R* w = sW + it * REC + 3 * cc;
-      acc += y[0] * w[0] + y[1] * w[1] + y[2] * w[2];
+  {
+    const int it = t >> 2, q = t & 3;
+    if (it < cnt) {
+      const R* wa = W + (size_t)sIdx[it] * REC;
+      const R* wb = W + (size_t)sIdx[kPairChunk + it] * REC;
+      const double* vi = Vinv + (size_t)sIdx[2 * kPairChunk + it] * 6;
+      const R v00 = (R)vi[0], v01 = (R)vi[1], v02 = (R)vi[2], v11 = (R)vi[3], v12 = (R)vi[4], v22 = (R)vi[5];
+#pragma unroll
+      for (int r = q; r < CNP; r += 4) {
+        const R w0 = wa[3 * r], w1 = wa[3 * r + 1], w2 = wa[3 * r + 2];
+        R* y = sY + it * REC + 3 * r;
+        y[0] = w0 * v00 + w1 * v01 + w2 * v02;
+        y[1] = w0 * v01 + w1 * v11 + w2 * v12;
+        y[2] = w0 * v02 + w1 * v12 + w2 * v22;
+      }
+#pragma unroll
+      for (int el = q; el < REC; el += 4) sW[it * REC + el] = wb[el];
     }
-    sG[grp * C2 + ent] = acc;
   }
   __syncthreads();
-  for (int t = threadIdx.x; t < C2; t += blockDim.x) {
+  const int grp = t / NB2, blk = t - grp * NB2;
+  const int rb = blk / NBK, cb = blk - rb * NBK;
+  R acc[3][3];
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+#pragma unroll
+    for (int j = 0; j < 3; ++j) acc[i][j] = 0;
+  if (grp < G) {
+    for (int it = grp; it < cnt; it += G) {
+      const R* y = sY + it * REC;
+      const R* w = sW + it * REC;
+      R yv[3][3], wv[3][3];
+#pragma unroll
+      for (int i = 0; i < 3; ++i) {
+        const int r = 3 * rb + i, cc = 3 * cb + i;
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+          yv[i][c] = r < CNP ? y[3 * r + c] : (R)0;
+          wv[i][c] = cc < CNP ? w[3 * cc + c] : (R)0;
+        }
+      }
+#pragma unroll
+      for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j) acc[i][j] += yv[i][0] * wv[j][0] + yv[i][1] * wv[j][1] + yv[i][2] * wv[j][2];
+    }
+  }
+  __syncthreads();  // the staging buffer now takes the group partials
+  R* const sG = sBuf;
+  if (grp < G) {
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+      for (int j = 0; j < 3; ++j) {
+        const int r = 3 * rb + i, cc = 3 * cb + j;
+        if (r < CNP && cc < CNP) sG[grp * C2 + r * CNP + cc] = acc[i][j];
+      }
+  }
+  __syncthreads();
+  for (int e = t; e < C2; e += blockDim.x) {
     R s = 0;
-    for (int q = 0; q < G; ++q) s += sG[q * C2 + t];
-    part[(size_t)b * C2 + t] = s;
+    for (int g = 0; g < G; ++g) s += sG[g * C2 + e];
+    part[(size_t)b * C2 + e] = s;
   }
 }
 

@@ -15,8 +15,8 @@ __global__ __launch_bounds__(256) void k_cauchy(double* __restrict__ out, const 
 // mode 0: (P_U.g, P_B.g, P_U.P_U, P_U.P_B, P_B.P_B)            trust_region.cpp:530-534
 // mode 1: sum (B-U)^2, sum (B-U)(2U-B), sum (2U-B)^2, -, -      trust_region.cpp:577-582
 __global__ __launch_bounds__(256) void k_tr_dots(const double* __restrict__ PU, const double* __restrict__ PB,
-                                                 const double* __restrict__ g, long n, int mode, double* partials,
-                                                 unsigned* counter, double* out) {
+                                                 const double* __restrict__ g, long n, int mode,
+                                                 double* partials) {
   __shared__ double sh[20];
   double acc[5] = {0, 0, 0, 0, 0};
   long q = (long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -36,7 +36,7 @@ __global__ __launch_bounds__(256) void k_tr_dots(const double* __restrict__ PU, 
       acc[2] = B * B;
     }
   }
-  grid_sum_finish<5>(acc, partials, counter, out, sh);
+  grid_sum_partials<5>(acc, partials, sh);
 }
 
 // op 0: s1 a + s2 b        (2-D subspace minimiser)
@@ -61,8 +61,9 @@ int launch_cauchy(psba_ctx* c, double* out, const double* g, double gtg, double 
 }
 
 int launch_tr_dots(psba_ctx* c, const double* PU, const double* PB, const double* g, int mode) {
-  hipLaunchKernelGGL(k_tr_dots, dim3(ceil_div(c->ntot, 256)), dim3(256), 0, c->stream, PU, PB, g, (long)c->ntot,
-                     mode, c->red_part, c->counters + CT_DOT, c->d_scal + SC_D0);
+  const unsigned nb = ceil_div(c->ntot, 256);
+  hipLaunchKernelGGL(k_tr_dots, dim3(nb), dim3(256), 0, c->stream, PU, PB, g, (long)c->ntot, mode, c->red_part);
+  launch_sum_partials<5>(c->stream, c->red_part, (int)nb, c->d_scal + SC_D0);
   PSBA_LAUNCH_TRY();
   return PSBA_OK;
 }
