@@ -307,6 +307,12 @@ typedef double dvec4 __attribute__((ext_vector_type(4)));
 // together into the kernel they exceed the register file and spill.
 #define PSBA_OUTLINE __device__ __attribute__((noinline))
 
+// LDS pointers for the out-of-line routines: with a generic pointer parameter
+// every LDS access would carry a generic-to-local conversion (a null test and
+// a select per address); callers cast their __shared__ addresses, which folds.
+typedef __attribute__((address_space(3))) double lds_dbl;
+#define LDS(p) ((lds_dbl*)(p))
+
 struct CholSmem {
   double P[NB * TP];  // L_kk of the current phase (lower, zeros above)
   double X[NB * TP];  // L_ik
@@ -318,6 +324,8 @@ struct CholSmem {
   double part[4][NB];
   int bad;
 };
+typedef __attribute__((address_space(3))) CholSmem lds_smem;
+
 
 // value of v on lane l (l a compile-time constant in the unrolled loops)
 __device__ __forceinline__ double bcast(double v, int l) {
@@ -338,11 +346,18 @@ __device__ __forceinline__ double quad_perm(double v) {
 __device__ __forceinline__ double quad_xor1(double v) { return quad_perm<0xB1>(v); }  // [1,0,3,2]
 __device__ __forceinline__ double quad_xor2(double v) { return quad_perm<0x4E>(v); }  // [2,3,0,1]
 
-__device__ __forceinline__ void grid_barrier(unsigned long long* bar, unsigned long long target, int* fail) {
+// Grid barrier in two halves: between grid_arrive and grid_wait a workgroup
+// may read data that was published before the previous barrier (prefetch).
+__device__ __forceinline__ void grid_arrive(unsigned long long* bar) {
   __syncthreads();
   if (threadIdx.x == 0) {
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
     __hip_atomic_fetch_add(bar, 1ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  }
+}
+
+__device__ __forceinline__ void grid_wait(unsigned long long* bar, unsigned long long target, int* fail) {
+  if (threadIdx.x == 0) {
     unsigned spins = 0;
     while (__hip_atomic_load(bar, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target) {
       __builtin_amdgcn_s_sleep(2);
@@ -356,11 +371,50 @@ __device__ __forceinline__ void grid_barrier(unsigned long long* bar, unsigned l
   __syncthreads();
 }
 
+__device__ __forceinline__ void grid_barrier(unsigned long long* bar, unsigned long long target, int* fail) {
+  grid_arrive(bar);
+  grid_wait(bar, target, fail);
+}
+
+// Tile transfers for the 256-thread kernels of this file: thread t moves the
+// elements t + 256 q (q = 0..15) of a 64x64 tile.  All 16 loads are issued
+// before the first LDS store, so a transfer waits for one memory latency.
+__device__ __forceinline__ void tile_regs(double (&v)[16], const double* src, int ld) {
+#pragma unroll
+  for (int q = 0; q < 16; ++q) {
+    const int f = (int)threadIdx.x + 256 * q;
+    v[q] = src[(size_t)(f >> 6) * ld + (f & 63)];
+  }
+}
+
+// v -> LDS tile (stride TP); with `lower`, zeros above the diagonal
+__device__ __forceinline__ void regs_to_lds(double* dst, const double (&v)[16], bool lower) {
+#pragma unroll
+  for (int q = 0; q < 16; ++q) {
+    const int f = (int)threadIdx.x + 256 * q, r = f >> 6, c = f & 63;
+    dst[r * TP + c] = (!lower || c <= r) ? v[q] : 0.0;
+  }
+}
+
 // lower triangle of a global tile into LDS (zeros above the diagonal)
 __device__ __forceinline__ void load_lower(double* dst, const double* src, int ld) {
-  for (int f = threadIdx.x; f < NB * NB; f += blockDim.x) {
+  double v[16];
+  tile_regs(v, src, ld);
+  regs_to_lds(dst, v, true);
+}
+
+// a full global tile into LDS (stride TP)
+__device__ __forceinline__ void load_full(double* dst, const double* src, int ld) {
+  double v[16];
+  tile_regs(v, src, ld);
+  regs_to_lds(dst, v, false);
+}
+
+// store_tile by waves 1-3 only (wave 0 is free for a concurrent tile solve)
+__device__ __forceinline__ void store_tile_w123(const double* src, double* dst, int ld) {
+  for (int f = (int)threadIdx.x - 64; f >= 0 && f < NB * NB; f += (int)blockDim.x - 64) {
     const int r = f >> 6, c = f & 63;
-    dst[r * TP + c] = (c <= r) ? src[(size_t)r * ld + c] : 0.0;
+    dst[(size_t)r * ld + c] = src[r * TP + c];
   }
 }
 
@@ -371,43 +425,52 @@ __device__ __forceinline__ void store_tile(const double* src, double* dst, int l
   }
 }
 
-// X = A L^-T (A a global 64x64 tile, L = sm.P): rows are independent; wave w
-// owns rows 16w..16w+15, the four lanes of a quad share a row and hold its
-// columns c = 4q + g (g = lane % 4).  Per column: partial sums over the solved
-// columns, a quad butterfly (the same total on all four lanes), and the owning
-// lane divides.  Branch-free; no workgroup barriers.
-PSBA_OUTLINE void trsm_rows(double* out, const double* A, int ld, const double* L, const double* rd) {
+// value of v on lane G0 of each quad (DPP quad_perm [G0, G0, G0, G0])
+template <int G0>
+__device__ __forceinline__ double quad_bcast(double v) {
+  return quad_perm<G0 * 0x55>(v);
+}
+
+// X = A L^-T (A a global 64x64 tile, L = sm.P lower with zeros above): rows
+// are independent; wave w owns rows 16w..16w+15, the four lanes of a quad
+// share a row and hold its columns c = 4q + g (g = lane % 4).  Branch-free;
+// no workgroup barriers.
+PSBA_OUTLINE void trsm_rows(lds_dbl* out, const double* A, int ld, const lds_dbl* L, const lds_dbl* rd) {
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   const int r = 16 * w + (lane >> 2), g = lane & 3;
-  double x[16], rdv[16], Lc[16], Ln[16];
+  double x[16], rdv[16], Lc[16];
 #pragma unroll
   for (int q = 0; q < 16; ++q) {
     x[q] = A[(size_t)r * ld + 4 * q + g];
     rdv[q] = rd[4 * q + g];
-    Lc[q] = 0.0;
-    Ln[q] = 0.0;
+    Lc[q] = L[(4 * q + g) * TP];
   }
+  // Column c: the owner lane (g == c % 4) scales its accumulated value, the
+  // quad shares it, and every later column of the row takes its term at once
+  // (right-looking), so a column's dependent chain is a multiply, a quad
+  // broadcast and one fused multiply-add.  Column c accumulates
+  // a_c - x_0 L[c][0] - x_1 L[c][1] - ... in ascending order, times 1/L[c][c].
 #pragma unroll
   for (int c = 0; c < NB; ++c) {
-    const int qf = c >> 2, cg = c & 3;
-    // next row of L, loaded while this column is reduced
+    const int q0 = c >> 2, g0 = c & 3;
+    const double xo = x[q0] * rdv[q0];
+    double xc;
+    if (g0 == 0) xc = quad_bcast<0>(xo);
+    else if (g0 == 1) xc = quad_bcast<1>(xo);
+    else if (g0 == 2) xc = quad_bcast<2>(xo);
+    else xc = quad_bcast<3>(xo);
+    const double fm = fma(-xc, Lc[q0], x[q0]);
+    x[q0] = (g == g0) ? xc : ((g > g0) ? fm : x[q0]);
+#pragma unroll
+    for (int q = q0 + 1; q < 16; ++q) x[q] = fma(-xc, Lc[q], x[q]);
     if (c + 1 < NB) {
 #pragma unroll
-      for (int q = 0; q <= ((c + 1) >> 2); ++q) Ln[q] = L[(c + 1) * TP + 4 * q + g];
+      for (int q = (c + 1) >> 2; q < 16; ++q) Lc[q] = L[(4 * q + g) * TP + c + 1];
     }
-    double s = 0.0;
+    // pin this column's updates here: left alone, the compiler defers the
+    // off-chain updates to their last use and runs out of registers
 #pragma unroll
-    for (int q = 0; q < qf; ++q) s = fma(x[q], Lc[q], s);
-    if (cg != 0) {
-      const double t = fma(x[qf], Lc[qf], s);
-      s = (g < cg) ? t : s;
-    }
-    s += quad_xor1(s);
-    s += quad_xor2(s);
-    const double nx = (x[qf] - s) * rdv[qf];
-    x[qf] = (g == cg) ? nx : x[qf];
-#pragma unroll
-    for (int q = 0; q < 16; ++q) Lc[q] = Ln[q];
+    for (int q = q0; q < 16; ++q) __asm__ volatile("" : "+v"(x[q]));
   }
 #pragma unroll
   for (int q = 0; q < 16; ++q) out[r * TP + 4 * q + g] = x[q];
@@ -446,7 +509,7 @@ struct Potrf16Row<C, 16> {
 
 template <int C>
 struct Potrf16Step {
-  static __device__ __forceinline__ void run(double (&a)[16], int lane, int* bad) {
+  static __device__ __forceinline__ void run(double (&a)[16], int lane, __attribute__((address_space(3))) int* bad) {
     const double piv = row_bcast<C>(a[C]);
     if (lane == 0 && !(piv > 0.0)) *bad = 1;
     const double ur = a[C] * fast_rcp(piv);
@@ -456,12 +519,12 @@ struct Potrf16Step {
 };
 template <>
 struct Potrf16Step<16> {
-  static __device__ __forceinline__ void run(double (&)[16], int, int*) {}
+  static __device__ __forceinline__ void run(double (&)[16], int, __attribute__((address_space(3))) int*) {}
 };
 
 template <int Q>
 struct Potrf16Scale {
-  static __device__ __forceinline__ void run(const double (&a)[16], double* B, int r, int lane, double d, double rdl) {
+  static __device__ __forceinline__ void run(const double (&a)[16], lds_dbl* B, int r, int lane, double d, double rdl) {
     const double rq = row_bcast<Q>(rdl);
     const double v = (Q < r) ? a[Q] * rq : ((Q == r) ? d : 0.0);
     if (lane < 16) B[r * TP + Q] = v;
@@ -470,7 +533,7 @@ struct Potrf16Scale {
 };
 template <>
 struct Potrf16Scale<16> {
-  static __device__ __forceinline__ void run(const double (&)[16], double*, int, int, double, double) {}
+  static __device__ __forceinline__ void run(const double (&)[16], lds_dbl*, int, int, double, double) {}
 };
 
 // 16x16 Cholesky of B (LDS, stride TP) by one wave, lane = row (the four
@@ -478,7 +541,7 @@ struct Potrf16Scale<16> {
 // the pivot column broadcast by DPP row_newbcast (one reciprocal per pivot on
 // the dependent chain), then L = U sqrt(D) with one sqrt per lane.
 // rd16 = 1 / diag(L).
-__device__ __forceinline__ void potrf16(double* B, double* rd16, int* bad) {
+__device__ __forceinline__ void potrf16(lds_dbl* B, lds_dbl* rd16, __attribute__((address_space(3))) int* bad) {
   const int lane = threadIdx.x & 63;
   const int r = lane & 15;
   double a[16];
@@ -493,10 +556,11 @@ __device__ __forceinline__ void potrf16(double* B, double* rd16, int* bad) {
   if (lane < 16) rd16[r] = rdl;
 }
 
-// Cholesky of the 64x64 tile in sm.Y (lower part), in place, zeros above.
+// Cholesky of the 64x64 LDS tile Yt (lower part), in place, zeros above; with
+// write_rd also 1 / diag(L) into sm.rd.
 // Right-looking over 16-column blocks: the diagonal block (one wave), the
 // panel below (one lane per row), the trailing lower blocks (MFMA, K = 16).
-PSBA_OUTLINE int tile_potrf(CholSmem& sm) {
+PSBA_OUTLINE int tile_potrf(lds_smem& sm, lds_dbl* Yt, bool write_rd) {
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   const int li = lane & 15, lk = lane >> 4;
   if (threadIdx.x == 0) sm.bad = 0;
@@ -504,38 +568,35 @@ PSBA_OUTLINE int tile_potrf(CholSmem& sm) {
 #pragma unroll 1
   for (int kb = 0; kb < 4; ++kb) {
     const int b0 = 16 * kb;
-    if (w == 0) potrf16(sm.Y + b0 * TP + b0, sm.rd16, &sm.bad);
+    if (w == 0) potrf16(Yt + b0 * TP + b0, sm.rd16, &sm.bad);
     __syncthreads();
+    if (write_rd && threadIdx.x < 16) sm.rd[b0 + threadIdx.x] = sm.rd16[threadIdx.x];
     const int R = 48 - b0;
     if ((int)threadIdx.x < R) {
       const int row = b0 + 16 + threadIdx.x;
-      double x[16], Lc[16], Ln[16], rdv[16];
+      // right-looking: column c is scaled, then every later column of the row
+      // takes its term (a_m - x_0 L[m][0] - x_1 L[m][1] - ..., ascending)
+      double x[16], Lc[16], rdv[16];
 #pragma unroll
       for (int c = 0; c < 16; ++c) {
-        x[c] = sm.Y[row * TP + b0 + c];
+        x[c] = Yt[row * TP + b0 + c];
         rdv[c] = sm.rd16[c];
-        Lc[c] = 0.0;
-        Ln[c] = 0.0;
+        Lc[c] = Yt[(b0 + c) * TP + b0];
       }
 #pragma unroll
       for (int c = 0; c < 16; ++c) {
+        x[c] = x[c] * rdv[c];
+#pragma unroll
+        for (int m = c + 1; m < 16; ++m) x[m] = fma(-x[c], Lc[m], x[m]);
         if (c + 1 < 16) {
 #pragma unroll
-          for (int m = 0; m <= c; ++m) Ln[m] = sm.Y[(b0 + c + 1) * TP + b0 + m];
+          for (int m = c + 2; m < 16; ++m) Lc[m] = Yt[(b0 + m) * TP + b0 + c + 1];
         }
-        double s0 = 0.0, s1 = 0.0;
 #pragma unroll
-        for (int m = 0; m + 1 < c; m += 2) {
-          s0 = fma(x[m], Lc[m], s0);
-          s1 = fma(x[m + 1], Lc[m + 1], s1);
-        }
-        if (c & 1) s0 = fma(x[c - 1], Lc[c - 1], s0);
-        x[c] = (x[c] - (s0 + s1)) * rdv[c];
-#pragma unroll
-        for (int m = 0; m < 16; ++m) Lc[m] = Ln[m];
+        for (int m = c; m < 16; ++m) __asm__ volatile("" : "+v"(x[m]));
       }
 #pragma unroll
-      for (int c = 0; c < 16; ++c) sm.Y[row * TP + b0 + c] = x[c];
+      for (int c = 0; c < 16; ++c) Yt[row * TP + b0 + c] = x[c];
     }
     __syncthreads();
     const int m = 3 - kb, nblk = m * (m + 1) / 2;
@@ -545,21 +606,21 @@ PSBA_OUTLINE int tile_potrf(CholSmem& sm) {
       const int ib = kb + 1 + jb + (bi - cum), jc = kb + 1 + jb;
       dvec4 acc;
 #pragma unroll
-      for (int rr = 0; rr < 4; ++rr) acc[rr] = sm.Y[(16 * ib + 4 * rr + lk) * TP + 16 * jc + li];
+      for (int rr = 0; rr < 4; ++rr) acc[rr] = Yt[(16 * ib + 4 * rr + lk) * TP + 16 * jc + li];
 #pragma unroll
       for (int k0 = 0; k0 < 16; k0 += 4) {
-        const double av = -sm.Y[(16 * ib + li) * TP + b0 + k0 + lk];
-        const double bv = sm.Y[(16 * jc + li) * TP + b0 + k0 + lk];
+        const double av = -Yt[(16 * ib + li) * TP + b0 + k0 + lk];
+        const double bv = Yt[(16 * jc + li) * TP + b0 + k0 + lk];
         acc = __builtin_amdgcn_mfma_f64_16x16x4f64(av, bv, acc, 0, 0, 0);
       }
 #pragma unroll
-      for (int rr = 0; rr < 4; ++rr) sm.Y[(16 * ib + 4 * rr + lk) * TP + 16 * jc + li] = acc[rr];
+      for (int rr = 0; rr < 4; ++rr) Yt[(16 * ib + 4 * rr + lk) * TP + 16 * jc + li] = acc[rr];
     }
     __syncthreads();
   }
   for (int f = threadIdx.x; f < NB * NB; f += blockDim.x) {
     const int rr = f >> 6, c = f & 63;
-    if (c > rr) sm.Y[rr * TP + c] = 0.0;
+    if (c > rr) Yt[rr * TP + c] = 0.0;
   }
   __syncthreads();
   return sm.bad;
@@ -614,7 +675,7 @@ __device__ __forceinline__ void acc_update(dvec4 (&acc)[2][2], const double* X, 
 // L is loaded first, the solved value is broadcast by readlane, and every
 // lane updates unconditionally (a lane past its own row no longer matters,
 // entries above the diagonal are zero).  y -> yg (global) and ys (LDS).
-PSBA_OUTLINE void tile_forward(const double* L, const double* b, double* yg, double* ys) {
+PSBA_OUTLINE void tile_forward(const lds_dbl* L, const lds_dbl* b, double* yg, lds_dbl* ys) {
   if (threadIdx.x >= 64) return;
   const int lane = threadIdx.x;
   double Lr[NB];
@@ -633,7 +694,7 @@ PSBA_OUTLINE void tile_forward(const double* L, const double* b, double* yg, dou
 }
 
 // x = L^-T r (L lower tile in LDS), wave 0, same scheme with the lane's column
-PSBA_OUTLINE void tile_backward(const double* L, double br, double* xg, double* xsm) {
+PSBA_OUTLINE void tile_backward(const lds_dbl* L, double br, double* xg, lds_dbl* xsm) {
   if (threadIdx.x >= 64) return;
   const int lane = threadIdx.x;
   double Lc[NB];
@@ -676,22 +737,23 @@ __global__ __launch_bounds__(256) void k_chol_solve(double* __restrict__ A, int 
   const int tid = threadIdx.x;
   unsigned long long nbar = 0;
   auto tile = [&](double* base, int i, int j) { return base + (size_t)i * NB * ld + (size_t)j * NB; };
-  // workgroup 0 carries the diagonal chain: it keeps L_kk and y_k in LDS when
-  // the diagonal task was its only task of the phase
+  // workgroup 0 carries the diagonal chain: when the diagonal task is its last
+  // task of the phase it factors straight into sm.P (with 1/diag in sm.rd) and
+  // keeps y_k in sm.w, ready for the next phase
   bool carry = false;
 
   // ---- phase -1: factor tile (0,0) and y_0; copy the right-hand side
   if (blockIdx.x == 0) {
     for (int f = tid; f < NB * NB; f += blockDim.x) {
       const int r = f >> 6, c = f & 63;
-      sm.Y[r * TP + c] = A[(size_t)r * ld + c];
+      sm.P[r * TP + c] = A[(size_t)r * ld + c];
     }
     if (tid < NB) sm.part[0][tid] = ea[tid];
     __syncthreads();
-    const int bad = tile_potrf(sm);
+    const int bad = tile_potrf(*(lds_smem*)&sm, LDS(sm.P), true);
     if (bad && tid == 0) __hip_atomic_fetch_or(fail, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    store_tile(sm.Y, Lf, ld);
-    tile_forward(sm.Y, sm.part[0], yv, sm.w);
+    store_tile_w123(sm.P, Lf, ld);
+    tile_forward(LDS(sm.P), LDS(sm.part[0]), yv, LDS(sm.w));
     carry = true;
   }
   for (int q = NB + (int)blockIdx.x * (int)blockDim.x + tid; q < T * NB; q += G * (int)blockDim.x) bw[q] = ea[q];
@@ -702,18 +764,15 @@ __global__ __launch_bounds__(256) void k_chol_solve(double* __restrict__ A, int 
     const int m = T - 1 - k, ntasks = m * (m + 1) / 2;
     if ((int)blockIdx.x < ntasks) {
       if (carry) {
-        for (int f = tid; f < NB * NB; f += blockDim.x) {
-          const int r = f >> 6, c = f & 63;
-          sm.P[r * TP + c] = sm.Y[r * TP + c];
-        }
         if (tid < NB) sm.v[tid] = sm.w[tid];
+        __syncthreads();
       } else {
         load_lower(sm.P, tile(Lf, k, k), ld);
         if (tid < NB) sm.v[tid] = yv[(size_t)k * NB + tid];
+        __syncthreads();
+        if (tid < NB) sm.rd[tid] = 1.0 / sm.P[tid * TP + tid];
+        __syncthreads();
       }
-      __syncthreads();
-      if (tid < NB) sm.rd[tid] = 1.0 / sm.P[tid * TP + tid];
-      __syncthreads();
     }
     carry = false;
     for (int t = blockIdx.x; t < ntasks; t += G) {
@@ -721,37 +780,41 @@ __global__ __launch_bounds__(256) void k_chol_solve(double* __restrict__ A, int 
       int a = 0, cum = 0;
       while (cum + (m - a) <= t) { cum += m - a; ++a; }
       const int j = k + 1 + a, i = j + (t - cum);
+      // b_i is read now so that its load overlaps the tile loads
+      const int br_row = tid >> 2, br_p = tid & 3;
+      const double b_old = (j == k + 1 && br_p == 0) ? bw[(size_t)i * NB + br_row] : 0.0;
       dvec4 acc[2][2];
       acc_load(acc, tile(A, i, j), ld);
-      trsm_rows(sm.X, tile(A, i, k), ld, sm.P, sm.rd);
-      if (i != j) trsm_rows(sm.Y, tile(A, j, k), ld, sm.P, sm.rd);
+      trsm_rows(LDS(sm.X), tile(A, i, k), ld, LDS(sm.P), LDS(sm.rd));
+      if (i != j) trsm_rows(LDS(sm.Y), tile(A, j, k), ld, LDS(sm.P), LDS(sm.rd));
       __syncthreads();
       acc_update(acc, sm.X, (i != j) ? sm.Y : sm.X, i == j);
       if (j == k + 1) {
         // store L_ik; b_i -= L_ik y_k (4 threads per row, fixed order)
         store_tile(sm.X, tile(Lf, i, k), ld);
-        const int r = tid >> 2, p = tid & 3;
         double s = 0.0;
 #pragma unroll
-        for (int c = 0; c < 16; ++c) s += sm.X[r * TP + 16 * p + c] * sm.v[16 * p + c];
+        for (int c = 0; c < 16; ++c) s += sm.X[br_row * TP + 16 * br_p + c] * sm.v[16 * br_p + c];
         s += __shfl_xor(s, 1, 64);
         s += __shfl_xor(s, 2, 64);
-        if (p == 0) {
-          const double bi = bw[(size_t)i * NB + r] - s;
-          bw[(size_t)i * NB + r] = bi;
-          if (i == j) sm.part[0][r] = bi;
+        if (br_p == 0) {
+          const double bi = b_old - s;
+          bw[(size_t)i * NB + br_row] = bi;
+          if (i == j) sm.part[0][br_row] = bi;
         }
       }
       if (i == j && j == k + 1) {
-        // the next diagonal tile: factor it and form y_{k+1}
-        __syncthreads();
-        acc_store(acc, sm.Y, TP);
-        __syncthreads();
-        const int bad = tile_potrf(sm);
-        if (bad && tid == 0) __hip_atomic_fetch_or(fail, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        store_tile(sm.Y, tile(Lf, i, i), ld);
-        tile_forward(sm.Y, sm.part[0], yv + (size_t)i * NB, sm.w);
+        // the next diagonal tile: factor it and form y_{k+1}; as the
+        // workgroup's last task of the phase, into sm.P for the next phase
         carry = (t + G >= ntasks);
+        double* Dt = carry ? sm.P : sm.Y;
+        __syncthreads();
+        acc_store(acc, Dt, TP);
+        __syncthreads();
+        const int bad = tile_potrf(*(lds_smem*)&sm, LDS(Dt), carry);
+        if (bad && tid == 0) __hip_atomic_fetch_or(fail, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        store_tile_w123(Dt, tile(Lf, i, i), ld);
+        tile_forward(LDS(Dt), LDS(sm.part[0]), yv + (size_t)i * NB, LDS(sm.w));
       } else {
         acc_store(acc, tile(A, i, j), ld);
       }
@@ -760,39 +823,75 @@ __global__ __launch_bounds__(256) void k_chol_solve(double* __restrict__ A, int 
     grid_barrier(bar, bar_base + (unsigned long long)G * (++nbar), fail);
   }
 
-  // ---- back substitution L^T x = y, right-looking over block columns.  The
-  // residual r lives in yv (y is not needed afterwards).  Step k: workgroup 0
-  // applies x_{k+1} to r_k and solves x_k; the other workgroups apply x_{k+1}
-  // to r_i, i < k.
+  // ---- back substitution L^T x = y: r_i = y_i - sum_{j > i} L_ji^T x_j with
+  // the terms subtracted in descending j, then x_i = L_ii^-T r_i.  Workgroup 0
+  // solves the chain and applies x_{i+2} and x_{i+1} to r_i itself; the other
+  // workgroups ("helpers") apply x_j, j >= i+3, each to the rows it owns, in
+  // step j-1, so that r_i is complete in yv two steps before workgroup 0
+  // needs it.  Everything a step reads that does not come from the step just
+  // finished (factor tiles, the completed r_i) is loaded into LDS while the
+  // workgroup waits at the barrier: L_kk, L_{k+1,k}, L_{k+2,k} and r_k by
+  // workgroup 0, the tile of its first row by each helper, which also keeps
+  // that row's r in registers.
+  const int first = (G > 1) ? 1 : 0, stride = (G > 1) ? G - 1 : 1;
+  const int i0 = (int)blockIdx.x - first;
+  const bool helper_pf = G > 1 && blockIdx.x >= 1;
+  double r_own = 0.0;  // helper: r of row i0 (lane tid < NB)
+  if (helper_pf && i0 + 3 <= T - 1 && tid < NB) r_own = yv[(size_t)i0 * NB + tid];
+  double r_next = 0.0;  // workgroup 0: r_k of the coming step (lane tid & 63)
+  if (blockIdx.x == 0) {
+    load_lower(sm.P, tile(Lf, T - 1, T - 1), ld);
+    r_next = yv[(size_t)(T - 1) * NB + (tid & 63)];
+  }
+  __syncthreads();
   for (int k = T - 1; k >= 0; --k) {
     if (blockIdx.x == 0) {
-      load_lower(sm.P, tile(Lf, k, k), ld);
-      double br = yv[(size_t)k * NB + (tid & 63)];
-      if (k + 1 < T) br -= tile_tmatvec(sm, tile(Lf, k + 1, k), ld, sm.w);
+      // sm.w = x_{k+1}, sm.rd = x_{k+2} (LDS vectors from the last two steps)
+      double br = r_next;
+      if (k + 2 < T) br -= tile_tmatvec(sm, sm.Y, TP, sm.rd);
+      if (k + 1 < T) br -= tile_tmatvec(sm, sm.X, TP, sm.w);
       __syncthreads();
-      tile_backward(sm.P, br, xs + (size_t)k * NB, sm.v);
+      tile_backward(LDS(sm.P), br, xs + (size_t)k * NB, LDS(sm.v));
       __syncthreads();
       if (tid < NB) {
+        sm.rd[tid] = sm.w[tid];
         sm.w[tid] = sm.v[tid];
         const int q = k * NB + tid;
         if (q < n) dp[q] = sm.v[tid];
       }
       __syncthreads();
     }
-    if (k + 1 < T) {
-      const int first = (G > 1) ? 1 : 0, stride = (G > 1) ? G - 1 : 1;
-      if ((int)blockIdx.x >= first) {
-        if ((int)blockIdx.x - first < k) {
-          if (tid < NB) sm.v[tid] = xs[(size_t)(k + 1) * NB + tid];
-          __syncthreads();
-        }
-        for (int i = (int)blockIdx.x - first; i < k; i += stride) {
+    // helpers: x_{k+1} to the rows i <= k-2 they own (i = i0, i0 + stride, ..)
+    if (k + 1 < T && (int)blockIdx.x >= first && i0 <= k - 2) {
+      if (tid < NB) sm.v[tid] = xs[(size_t)(k + 1) * NB + tid];
+      __syncthreads();
+      for (int i = i0; i <= k - 2; i += stride) {
+        if (helper_pf && i == i0) {
+          const double t = tile_tmatvec(sm, sm.X, TP, sm.v);
+          r_own -= t;
+          if (i0 == k - 2 && tid < NB) yv[(size_t)i0 * NB + tid] = r_own;
+        } else {
           const double t = tile_tmatvec(sm, tile(Lf, k + 1, i), ld, sm.v);
           if (tid < NB) yv[(size_t)i * NB + tid] -= t;
         }
       }
     }
-    if (k > 0) grid_barrier(bar, bar_base + (unsigned long long)G * (++nbar), fail);
+    if (k > 0) {
+      grid_arrive(bar);
+      if (blockIdx.x == 0) {
+        double v0[16], v1[16], v2[16];
+        tile_regs(v0, tile(Lf, k - 1, k - 1), ld);
+        tile_regs(v1, tile(Lf, k, k - 1), ld);
+        if (k + 1 < T) tile_regs(v2, tile(Lf, k + 1, k - 1), ld);
+        r_next = yv[(size_t)(k - 1) * NB + (tid & 63)];
+        regs_to_lds(sm.P, v0, true);
+        regs_to_lds(sm.X, v1, false);
+        if (k + 1 < T) regs_to_lds(sm.Y, v2, false);
+      } else if (helper_pf && i0 <= k - 3) {
+        load_full(sm.X, tile(Lf, k, i0), ld);
+      }
+      grid_wait(bar, bar_base + (unsigned long long)G * (++nbar), fail);
+    }
   }
   if (blockIdx.x == 0 && tid == 0) {
     const int f = __hip_atomic_load(fail, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -832,7 +931,7 @@ __global__ __launch_bounds__(256) void k_tri_solve(const double* __restrict__ Lf
     load_lower(sm.P, tile(i, i), ld);
     if (tid < NB) sm.v[tid] = br;
     __syncthreads();
-    tile_forward(sm.P, sm.v, y + (size_t)i * NB, sm.w);
+    tile_forward(LDS(sm.P), LDS(sm.v), y + (size_t)i * NB, LDS(sm.w));
     __syncthreads();
   }
   for (int i = T - 1; i >= 0; --i) {
@@ -840,7 +939,7 @@ __global__ __launch_bounds__(256) void k_tri_solve(const double* __restrict__ Lf
     for (int k = i + 1; k < T; ++k) br -= tile_tmatvec(sm, tile(k, i), ld, x + (size_t)k * NB);
     load_lower(sm.P, tile(i, i), ld);
     __syncthreads();
-    tile_backward(sm.P, br, x + (size_t)i * NB, sm.v);
+    tile_backward(LDS(sm.P), br, x + (size_t)i * NB, LDS(sm.v));
     __syncthreads();
     if (tid < NB && i * NB + tid < n) out[i * NB + tid] = sm.v[tid];
   }
@@ -920,9 +1019,9 @@ __global__ __launch_bounds__(256) void k_b_chol_solve(double* __restrict__ S, do
   }
   if (tid < NB) sm.part[0][tid] = (tid < n) ? b[tid] : 0.0;
   __syncthreads();
-  bad |= tile_potrf(sm);
+  bad |= tile_potrf(*(lds_smem*)&sm, LDS(sm.Y), false);
   store_tile(sm.Y, L, ld);
-  tile_forward(sm.Y, sm.part[0], yvp, sm.w);
+  tile_forward(LDS(sm.Y), LDS(sm.part[0]), yvp, LDS(sm.w));
   __syncthreads();
 
   for (int k = 0; k + 1 < T; ++k) {
@@ -943,8 +1042,8 @@ __global__ __launch_bounds__(256) void k_b_chol_solve(double* __restrict__ S, do
       const int j = k + 1 + a, i = j + (t - cum);
       dvec4 acc[2][2];
       acc_load(acc, tile(A, i, j), ld);
-      trsm_rows(sm.X, tile(A, i, k), ld, sm.P, sm.rd);
-      if (i != j) trsm_rows(sm.Y, tile(A, j, k), ld, sm.P, sm.rd);
+      trsm_rows(LDS(sm.X), tile(A, i, k), ld, LDS(sm.P), LDS(sm.rd));
+      if (i != j) trsm_rows(LDS(sm.Y), tile(A, j, k), ld, LDS(sm.P), LDS(sm.rd));
       __syncthreads();
       acc_update(acc, sm.X, (i != j) ? sm.Y : sm.X, i == j);
       if (j == k + 1) {
@@ -965,9 +1064,9 @@ __global__ __launch_bounds__(256) void k_b_chol_solve(double* __restrict__ S, do
         __syncthreads();
         acc_store(acc, sm.Y, TP);
         __syncthreads();
-        bad |= tile_potrf(sm);
+        bad |= tile_potrf(*(lds_smem*)&sm, LDS(sm.Y), false);
         store_tile(sm.Y, tile(L, i, i), ld);
-        tile_forward(sm.Y, sm.part[0], yvp + (size_t)i * NB, sm.w);
+        tile_forward(LDS(sm.Y), LDS(sm.part[0]), yvp + (size_t)i * NB, LDS(sm.w));
       } else {
         acc_store(acc, tile(A, i, j), ld);
       }
@@ -982,7 +1081,7 @@ __global__ __launch_bounds__(256) void k_b_chol_solve(double* __restrict__ S, do
     double br = yvp[(size_t)k * NB + (tid & 63)];
     if (k + 1 < T) br -= tile_tmatvec(sm, tile(L, k + 1, k), ld, sm.w);
     __syncthreads();
-    tile_backward(sm.P, br, xsp + (size_t)k * NB, sm.v);
+    tile_backward(LDS(sm.P), br, xsp + (size_t)k * NB, LDS(sm.v));
     __syncthreads();
     if (tid < NB) {
       const int q = k * NB + tid;
@@ -1046,14 +1145,14 @@ __global__ __launch_bounds__(256) void k_b_chol_init(double* __restrict__ S, dou
   }
   if (tid < NB) sm.part[0][tid] = (tid < n) ? b[tid] : 0.0;
   __syncthreads();
-  const int bad = tile_potrf(sm);
+  const int bad = tile_potrf(*(lds_smem*)&sm, LDS(sm.Y), false);
   store_tile(sm.Y, L, ld);
-  tile_forward(sm.Y, sm.part[0], yv + vec_off[p], sm.w);
+  tile_forward(LDS(sm.Y), LDS(sm.part[0]), yv + vec_off[p], LDS(sm.w));
   __syncthreads();
   if (T == 1) {
     const double br = sm.w[tid & 63];
     __syncthreads();
-    tile_backward(sm.Y, br, xs + vec_off[p], sm.v);
+    tile_backward(LDS(sm.Y), br, xs + vec_off[p], LDS(sm.v));
     __syncthreads();
     if (tid < n) dp[(size_t)cam_off[p] * cnp + tid] = sm.v[tid];
   }
@@ -1088,8 +1187,8 @@ __global__ __launch_bounds__(256) void k_b_chol_phase(int k, const int* __restri
   const int j = k + 1 + a, i = j + (t - cum);
   dvec4 acc[2][2];
   acc_load(acc, tile(A, i, j), ld);
-  trsm_rows(sm.X, tile(A, i, k), ld, sm.P, sm.rd);
-  if (i != j) trsm_rows(sm.Y, tile(A, j, k), ld, sm.P, sm.rd);
+  trsm_rows(LDS(sm.X), tile(A, i, k), ld, LDS(sm.P), LDS(sm.rd));
+  if (i != j) trsm_rows(LDS(sm.Y), tile(A, j, k), ld, LDS(sm.P), LDS(sm.rd));
   __syncthreads();
   acc_update(acc, sm.X, (i != j) ? sm.Y : sm.X, i == j);
   if (j == k + 1) {
@@ -1110,9 +1209,9 @@ __global__ __launch_bounds__(256) void k_b_chol_phase(int k, const int* __restri
     __syncthreads();
     acc_store(acc, sm.Y, TP);
     __syncthreads();
-    const int bad = tile_potrf(sm);
+    const int bad = tile_potrf(*(lds_smem*)&sm, LDS(sm.Y), false);
     store_tile(sm.Y, tile(L, i, i), ld);
-    tile_forward(sm.Y, sm.part[0], yvp + (size_t)i * NB, sm.w);
+    tile_forward(LDS(sm.Y), LDS(sm.part[0]), yvp + (size_t)i * NB, LDS(sm.w));
     if (bad && tid == 0) fok[p] = 0.0;
   } else {
     acc_store(acc, tile(A, i, j), ld);
@@ -1145,7 +1244,7 @@ __global__ __launch_bounds__(256) void k_b_chol_back(int s, const int* __restric
     double br = yvp[(size_t)k * NB + (tid & 63)];
     if (s > 0) br -= tile_tmatvec(sm, tile(L, k + 1, k), ld, sm.w);
     __syncthreads();
-    tile_backward(sm.P, br, xsp + (size_t)k * NB, sm.v);
+    tile_backward(LDS(sm.P), br, xsp + (size_t)k * NB, LDS(sm.v));
     __syncthreads();
     if (tid < NB) {
       const int q = k * NB + tid;

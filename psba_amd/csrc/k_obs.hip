@@ -181,9 +181,14 @@ __global__ __launch_bounds__(256) void k_cam_chunks(const int* __restrict__ cam_
   const int b = blockIdx.x;
   if (run && !run[ownr[b]]) return;
   const int beg = cb[b], cnt = ce[b] - beg;
+  // the chunk's observation indices first (one round of loads), then the
+  // Jacobian rows and residuals element by element (coalesced, independent)
+  __shared__ int sO[CH];
+  for (int s = threadIdx.x; s < cnt; s += blockDim.x) sO[s] = cam_obs[beg + s];
+  __syncthreads();
+#pragma unroll 4
   for (int f = threadIdx.x; f < cnt * REC; f += blockDim.x) {
-    int s = f / REC, el = f - s * REC;
-    int o = cam_obs[beg + s];
+    const int s = f / REC, el = f - s * REC, o = sO[s];
     sJ[f] = el < 2 * CNP ? JA[(size_t)o * 2 * CNP + el] : e[2 * o + (el - 2 * CNP)];
   }
   __syncthreads();
@@ -387,11 +392,20 @@ __global__ __launch_bounds__(256) void k_backsub(const int* __restrict__ pt_ptr,
   const bool ok = grp < kBacksubPtsPerWave && i < np && (!run || run[ownr[i]]);
   R s = 0;
   if (ok) {
-    for (int o = pt_ptr[i]; o < pt_ptr[i + 1]; ++o) {
+    // camera indices are fetched two observations ahead, so each step waits
+    // for one round of loads (W and dpa) instead of two dependent ones
+    const int beg = pt_ptr[i], end = pt_ptr[i + 1];
+    int jc = beg < end ? obs_cam[beg] : 0;
+    int jn = beg + 1 < end ? obs_cam[beg + 1] : 0;
+#pragma unroll 2
+    for (int o = beg; o < end; ++o) {
+      const int jnn = o + 2 < end ? obs_cam[o + 2] : 0;
       const R* w = W + (size_t)o * 3 * CNP + cc;
-      const double* da = dp + obs_cam[o] * CNP;
+      const double* da = dp + (size_t)jc * CNP;
 #pragma unroll
       for (int k = 0; k < CNP; ++k) s += w[3 * k] * (R)da[k];
+      jc = jn;
+      jn = jnn;
     }
   }
   const double e = ok ? gb[3 * (size_t)i + cc] - (double)s : 0.0;

@@ -176,11 +176,20 @@ __global__ __launch_bounds__(256) void k_ea_chunks(const int* __restrict__ cam_o
   const int b = blockIdx.x;
   if (run && !run[ownr[b]]) return;
   const int beg = cb[b], cnt = ce[b] - beg;
+  // the chunk's observation and point indices first (the dependent loads once
+  // per observation), then the products with independent, coalesced loads
+  __shared__ int sO[CH], sI[CH];
+  for (int s = threadIdx.x; s < cnt; s += blockDim.x) {
+    const int o = cam_obs[beg + s];
+    sO[s] = o;
+    sI[s] = obs_pt[o];
+  }
+  __syncthreads();
+#pragma unroll 4
   for (int task = threadIdx.x; task < cnt * CNP; task += blockDim.x) {
-    int s = task / CNP, r = task - s * CNP;
-    int o = cam_obs[beg + s];
-    const R* w = W + (size_t)o * 3 * CNP + 3 * r;
-    const double* zi = z + (size_t)obs_pt[o] * 3;
+    const int s = task / CNP, r = task - s * CNP;
+    const R* w = W + (size_t)sO[s] * 3 * CNP + 3 * r;
+    const double* zi = z + (size_t)sI[s] * 3;
     sP[task] = w[0] * (R)zi[0] + w[1] * (R)zi[1] + w[2] * (R)zi[2];
   }
   __syncthreads();

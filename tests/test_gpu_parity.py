@@ -214,6 +214,28 @@ def test_dense_factor_solve_matches_lapack(P, gpu, n_cams):
         assert rel(np.tril(L), np.linalg.cholesky(S)) < 1e-11
 
 
+@pytest.mark.parametrize("n_cams", [7, 40])
+def test_dense_factor_reads_the_lower_triangle_only(P, gpu, n_cams):
+    """The Schur assembly writes only the lower block positions of S, so the
+    factorisation must not read above the diagonal: garbage there changes
+    nothing (the same factor and solution, bit for bit)."""
+    rng = np.random.default_rng(100 + n_cams)
+    n = 6 * n_cams
+    S = spd(n, rng, cond=1e8)
+    b = rng.standard_normal(n)
+    junk = np.tril(S) + np.triu(rng.standard_normal((n, n)) * 1e3, 1)
+    with P.Context(6, n_cams, 4, 8) as c:
+        c.load_S(S, b)
+        ok1, x1 = c.factor_only()
+        L1 = np.tril(c.buffer("L").reshape(n, n))
+        c.load_S(junk, b)
+        ok2, x2 = c.factor_only()
+        L2 = np.tril(c.buffer("L").reshape(n, n))
+    assert ok1 and ok2
+    assert np.array_equal(x1, x2) and np.array_equal(L1, L2)
+    assert rel(x1, np.linalg.solve(S, b)) < 1e-7
+
+
 def test_dense_factor_reports_indefinite(P, gpu):
     rng = np.random.default_rng(9)
     n = 6 * 20
