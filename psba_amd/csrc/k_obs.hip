@@ -181,15 +181,17 @@ __global__ __launch_bounds__(256) void k_cam_chunks(const int* __restrict__ cam_
   const int b = blockIdx.x;
   if (run && !run[ownr[b]]) return;
   const int beg = cb[b], cnt = ce[b] - beg;
-  // the chunk's observation indices first (one round of loads), then the
-  // Jacobian rows and residuals element by element (coalesced, independent)
-  __shared__ int sO[CH];
-  for (int s = threadIdx.x; s < cnt; s += blockDim.x) sO[s] = cam_obs[beg + s];
-  __syncthreads();
-#pragma unroll 4
-  for (int f = threadIdx.x; f < cnt * REC; f += blockDim.x) {
-    const int s = f / REC, el = f - s * REC, o = sO[s];
-    sJ[f] = el < 2 * CNP ? JA[(size_t)o * 2 * CNP + el] : e[2 * o + (el - 2 * CNP)];
+  // one thread per observation stages its Jacobian rows and residual (one
+  // round of dependent loads for the whole chunk, all of a record's loads in
+  // flight together)
+  for (int s = threadIdx.x; s < cnt; s += blockDim.x) {
+    const int o = cam_obs[beg + s];
+    const R* ja = JA + (size_t)o * 2 * CNP;
+    R* dst = sJ + s * REC;
+#pragma unroll
+    for (int el = 0; el < 2 * CNP; ++el) dst[el] = ja[el];
+    dst[2 * CNP] = e[2 * o];
+    dst[2 * CNP + 1] = e[2 * o + 1];
   }
   __syncthreads();
   for (int t = threadIdx.x; t < NE; t += blockDim.x) {
@@ -378,13 +380,21 @@ int launch_vinv(psba_ctx* c, double mu) {
 // (lane 63 idles); the three eb values meet through lane shuffles for
 // dpb_i = V_i^-1 eb_i (V_i^-1 packed as its 6 unique entries).
 constexpr int kBacksubPtsPerWave = 21;
+constexpr int kBacksubLdsDpa = 4096;  // camera parameters that go through LDS (32 KB)
 
-template <int CNP, typename R>
+// With LDS_DPA the camera step dp[0..n) is first copied into LDS (dynamic
+// shared memory of n doubles), so that a long track waits for the W rows only.
+template <int CNP, typename R, bool LDS_DPA>
 __global__ __launch_bounds__(256) void k_backsub(const int* __restrict__ pt_ptr, const int* __restrict__ obs_cam,
                                                  const R* __restrict__ W, const double* __restrict__ Vinv,
                                                  const double* __restrict__ gb, int np, int n,
                                                  double* __restrict__ dp, double* __restrict__ eb,
                                                  const int* __restrict__ ownr, const unsigned char* __restrict__ run) {
+  extern __shared__ double sDa[];
+  if constexpr (LDS_DPA) {
+    for (int q = threadIdx.x; q < n; q += blockDim.x) sDa[q] = dp[q];
+    __syncthreads();
+  }
   const int lane = threadIdx.x & 63;
   const int grp = lane / 3;
   const int cc = lane - 3 * grp;
@@ -401,9 +411,15 @@ __global__ __launch_bounds__(256) void k_backsub(const int* __restrict__ pt_ptr,
     for (int o = beg; o < end; ++o) {
       const int jnn = o + 2 < end ? obs_cam[o + 2] : 0;
       const R* w = W + (size_t)o * 3 * CNP + cc;
-      const double* da = dp + (size_t)jc * CNP;
+      if constexpr (LDS_DPA) {
+        const double* da = sDa + jc * CNP;
 #pragma unroll
-      for (int k = 0; k < CNP; ++k) s += w[3 * k] * (R)da[k];
+        for (int k = 0; k < CNP; ++k) s += w[3 * k] * (R)da[k];
+      } else {
+        const double* da = dp + (size_t)jc * CNP;
+#pragma unroll
+        for (int k = 0; k < CNP; ++k) s += w[3 * k] * (R)da[k];
+      }
       jc = jn;
       jn = jnn;
     }
@@ -424,8 +440,12 @@ static void backsub_launch(hipStream_t st, const int* pt_ptr, const int* obs_cam
                            const double* gb, int np, int n, double* dp, double* eb, const int* pt_ownr,
                            const unsigned char* run, int block) {
   const int per_block = (block / 64) * kBacksubPtsPerWave;
-  hipLaunchKernelGGL((k_backsub<CNP, R>), dim3(ceil_div(np, per_block)), dim3(block), 0, st, pt_ptr, obs_cam, W, Vinv,
-                     gb, np, n, dp, eb, pt_ownr, run);
+  if (n <= kBacksubLdsDpa)
+    hipLaunchKernelGGL((k_backsub<CNP, R, true>), dim3(ceil_div(np, per_block)), dim3(block),
+                       sizeof(double) * (size_t)n, st, pt_ptr, obs_cam, W, Vinv, gb, np, n, dp, eb, pt_ownr, run);
+  else
+    hipLaunchKernelGGL((k_backsub<CNP, R, false>), dim3(ceil_div(np, per_block)), dim3(block), 0, st, pt_ptr,
+                       obs_cam, W, Vinv, gb, np, n, dp, eb, pt_ownr, run);
 }
 
 template <int CNP>

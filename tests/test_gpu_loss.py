@@ -8,7 +8,10 @@ and the robust cost with outliers has many nearby minima, so, as with the
 plain cost on outlier scenes, two correct FP64 evaluations drift apart after
 a few tens of iterations (first relative difference above 1e-9 at iterations
 20-157 on these scenes, measured).  Parity is therefore asserted over the
-first 15 iterations."""
+first 15 iterations: every trial's decision and cost within 1e-9, the
+parameters reached within 1e-8 (the points whose observations the loss
+down-weights are the least determined and carry the largest rounding drift:
+1.0e-9 after 15 Cauchy iterations on the 54-camera scene, measured)."""
 import numpy as np
 import pytest
 
@@ -47,7 +50,7 @@ def test_robust_levmar_trial_by_trial(P, O, gpu, kind, name, config, kw):
     compare_traces(rg["trace"], ro["trace"])
     gc, gp = g.read_params()
     oc, op = o.read_params()
-    assert rel(gc, oc) < 1e-9 and rel(gp, op) < 1e-9
+    assert rel(gc, oc) < 1e-8 and rel(gp, op) < 1e-8
 
 
 def test_robust_full_size(P, O, gpu):
