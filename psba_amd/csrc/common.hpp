@@ -116,6 +116,17 @@ inline void launch_sum_partials(hipStream_t st, const double* partials, int nb, 
 
 inline unsigned ceil_div(long a, long b) { return (unsigned)((a + b - 1) / b); }
 
+// Launch order of the Schur pair chunks: the chunks of camera pairs in the
+// same block of 8 x 8 cameras and the same range of 4096 points run together,
+// so the W rows they gather come back from L2 instead of HBM (with the XCD
+// mapping each XCD walks its own share of this order).  Sorting is stable, so
+// ties keep the chunk order.
+constexpr int kSchurTileCams = 8, kSchurPointRangeLog2 = 12;
+__host__ __device__ inline unsigned long long schunk_launch_key(int k, int l, int pt) {
+  const unsigned long long K = (unsigned)k / kSchurTileCams, L = (unsigned)l / kSchurTileCams;
+  return ((K * (K + 1) / 2 + L) << 32) | ((unsigned)pt >> kSchurPointRangeLog2);
+}
+
 // Robust loss of an observation's squared residual norm s with scale d
 // (PSBA_LOSS_*): rho(s) and rho'(s).
 __device__ __forceinline__ void loss_eval(int kind, double d, double s, double& rho, double& drho) {

@@ -80,6 +80,7 @@ struct psba_ctx {
   int *schunk_begin = nullptr, *schunk_end = nullptr, *pair_schunk_ptr = nullptr;
   int n_lblocks = 0;            // nc*(nc+1)/2 lower block positions
   int* lblock_pair = nullptr;   // pair index of each lower block position or -1
+  int* schunk_order = nullptr;  // launch order of the pair chunks (schunk_launch_key)
   double sum_t2 = 0;            // sum over points of track length squared
 
   bool mixed() const { return precision == PSBA_PREC_MIXED; }

@@ -66,6 +66,7 @@ static void free_index(psba_ctx* c) {
   dfree(c->schunk_begin);
   dfree(c->schunk_end);
   dfree(c->pair_schunk_ptr);
+  dfree(c->schunk_order);
   dfree(c->lblock_pair);
   dfree(c->ucpart);
   dfree(c->eapart);
@@ -426,6 +427,7 @@ int alloc_index(psba_ctx* c, long nitems, int npairs, int n_cchunk, int n_schunk
   PSBA_TRY(dalloc(&c->schunk_begin, n_schunk));
   PSBA_TRY(dalloc(&c->schunk_end, n_schunk));
   PSBA_TRY(dalloc(&c->pair_schunk_ptr, npairs + 1));
+  PSBA_TRY(dalloc(&c->schunk_order, n_schunk));
   PSBA_TRY(dalloc(&c->lblock_pair, nlb));
   if (c->mixed()) {
     PSBA_TRY(dalloc(&c->ucpart32, (size_t)n_cchunk * NE));
@@ -511,6 +513,17 @@ int psba_set_index(psba_ctx* c, const int* iidx, const int* jidx) {
     }
     psp[q + 1] = (int)sb.size();
   }
+  // launch order: by camera-block tile, then point range, then chunk (stable)
+  std::vector<int> sord(sb.size());
+  {
+    std::vector<unsigned long long> key(sb.size());
+    for (size_t q = 0; q < sb.size(); ++q) {
+      const int2 it = items[(size_t)sb[q]];
+      key[q] = schunk_launch_key(jidx[it.x], jidx[it.y], iidx[it.x]);
+      sord[q] = (int)q;
+    }
+    std::stable_sort(sord.begin(), sord.end(), [&](int x, int y) { return key[x] < key[y]; });
+  }
   const int nlb = nc * (nc + 1) / 2;
   std::vector<int> lbp(nlb);
   for (int k = 0; k < nc; ++k)
@@ -538,6 +551,7 @@ int psba_set_index(psba_ctx* c, const int* iidx, const int* jidx) {
   PSBA_TRY(up(c->schunk_end, se.data(), sizeof(int) * se.size()));
   PSBA_TRY(up(c->pair_schunk_ptr, psp.data(), sizeof(int) * (npairs + 1)));
   PSBA_TRY(up(c->lblock_pair, lbp.data(), sizeof(int) * nlb));
+  PSBA_TRY(up(c->schunk_order, sord.data(), sizeof(int) * sord.size()));
   PSBA_HIP_TRY(hipStreamSynchronize(c->stream));
   c->indexed = true;
   return PSBA_OK;
@@ -746,6 +760,7 @@ static const int* index_ptr(const psba_ctx* c, int which, long* n) {
     case PSBA_IDX_SCHUNK_END: *n = c->n_schunk; return c->schunk_end;
     case PSBA_IDX_PAIR_SCHUNK_PTR: *n = c->npairs + 1; return c->pair_schunk_ptr;
     case PSBA_IDX_LBLOCK_PAIR: *n = c->n_lblocks; return c->lblock_pair;
+    case PSBA_IDX_SCHUNK_ORDER: *n = c->n_schunk; return c->schunk_order;
     default: *n = -1; return nullptr;
   }
 }

@@ -96,6 +96,17 @@ __global__ void k_item_counts(const int* __restrict__ pt_ptr, int np, long long*
   if (t) atomicAdd((unsigned long long*)t2, (unsigned long long)(t * t));
 }
 
+// launch key of each pair chunk (its first item's cameras and point)
+__global__ void k_schunk_keys(const int2* __restrict__ items, const int* __restrict__ sbeg, int n,
+                              const int* __restrict__ obs_cam, const int* __restrict__ obs_pt,
+                              unsigned long long* __restrict__ key, int* __restrict__ val) {
+  const int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= n) return;
+  const int2 it = items[sbeg[q]];
+  key[q] = schunk_launch_key(obs_cam[it.x], obs_cam[it.y], obs_pt[it.x]);
+  val[q] = q;
+}
+
 // chunk ranges of each segment: [seg_ptr[s], seg_ptr[s+1]) cut every CH
 __global__ void k_fill_chunks(const int* __restrict__ seg_ptr, int nseg, const int* __restrict__ chunk_ptr, int CH,
                               int* __restrict__ cb, int* __restrict__ ce) {
@@ -337,6 +348,21 @@ int psba_set_index_device(psba_ctx* c, const int* iidx, const int* jidx) {
   IX_TRY(hipMemcpyAsync(c->pair_schunk_ptr + npairs, &n_schunk, sizeof(int), hipMemcpyHostToDevice, st));
   hipLaunchKernelGGL(k_fill_chunks, grid(npairs), dim3(256), 0, st, pptr.p, npairs, psp.p, kPairChunk,
                      c->schunk_begin, c->schunk_end);
+  // launch order of the pair chunks: a stable sort by schunk_launch_key
+  if (n_schunk > 0) {
+    DevBuf<unsigned long long> okey, okey_s;
+    DevBuf<int> oval;
+    PSBA_TRY(okey.alloc(n_schunk));
+    PSBA_TRY(okey_s.alloc(n_schunk));
+    PSBA_TRY(oval.alloc(n_schunk));
+    hipLaunchKernelGGL(k_schunk_keys, grid(n_schunk), dim3(256), 0, st, c->pair_items, c->schunk_begin, n_schunk,
+                       c->obs_cam, c->obs_pt, okey.p, oval.p);
+    IX_TRY(rocprim::radix_sort_pairs(nullptr, tb, okey.p, okey_s.p, oval.p, c->schunk_order, (size_t)n_schunk, 0,
+                                     64, st));
+    PSBA_TRY(tmp.ensure(tb));
+    IX_TRY(rocprim::radix_sort_pairs(tmp.p, tb, okey.p, okey_s.p, oval.p, c->schunk_order, (size_t)n_schunk, 0, 64,
+                                     st));
+  }
   IX_TRY(hipGetLastError());
   IX_TRY(hipStreamSynchronize(st));  // the temporaries are freed on return
   c->indexed = true;

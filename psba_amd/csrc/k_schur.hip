@@ -32,7 +32,8 @@ __global__ __launch_bounds__(256) void k_schur_chunks(const int2* __restrict__ i
                                                       const int* __restrict__ se, const int* __restrict__ obs_pt,
                                                       const R* __restrict__ W, const double* __restrict__ Vinv,
                                                       R* __restrict__ part, const int* __restrict__ ownr,
-                                                      const unsigned char* __restrict__ run, int nchunk) {
+                                                      const unsigned char* __restrict__ run, int nchunk,
+                                                      const int* __restrict__ order) {
   constexpr int C2 = CNP * CNP, REC = 3 * CNP;
   constexpr int NBK = (CNP + 2) / 3;  // 3-wide row / column blocks
   constexpr int NB2 = NBK * NBK;
@@ -43,11 +44,12 @@ __global__ __launch_bounds__(256) void k_schur_chunks(const int2* __restrict__ i
   R* const sY = sBuf;
   R* const sW = sBuf + kPairChunk * REC;
 #if PSBA_SCHUR_XCD
-  const int b = xcd_item(blockIdx.x, nchunk);  // one camera range per XCD
+  int b = xcd_item(blockIdx.x, nchunk);  // one share of the launch order per XCD
   if (b < 0) return;
 #else
-  const int b = blockIdx.x;
+  int b = blockIdx.x;
 #endif
+  if (order) b = order[b];
   if (run && !run[ownr[b]]) return;
   const int beg = sb[b], cnt = se[b] - beg;
   const int t = threadIdx.x;
@@ -219,7 +221,7 @@ static void run_schur_t(psba_ctx* c, double mu, const R* W, R* schpart, R* eapar
     probe_begin(c, PSBA_PROBE_SCHUR);
     hipLaunchKernelGGL((k_schur_chunks<CNP, R>), dim3(PSBA_SCHUR_XCD ? xcd_grid(c->n_schunk) : c->n_schunk), dim3(256),
                        0, c->stream, c->pair_items, c->schunk_begin, c->schunk_end, c->obs_pt, W, c->Vinv, schpart,
-                       (const int*)nullptr, (const unsigned char*)nullptr, c->n_schunk);
+                       (const int*)nullptr, (const unsigned char*)nullptr, c->n_schunk, (const int*)c->schunk_order);
     probe_end(c, PSBA_PROBE_SCHUR);
   }
   hipLaunchKernelGGL((k_schur_finish<CNP, R>), dim3(c->n_lblocks + (c->n_pad - c->n)),
@@ -347,7 +349,7 @@ static void run_b_schur_t(psba_batch* b) {
   if (b->n_schunk > 0)
     hipLaunchKernelGGL((k_schur_chunks<CNP, R>), dim3(PSBA_SCHUR_XCD ? xcd_grid(b->n_schunk) : b->n_schunk), dim3(256),
                        0, b->stream, b->items, b->schunk_b, b->schunk_e, b->obs_pt, W, b->Vinv, schpart, b->schunk_prob,
-                       b->active, b->n_schunk);
+                       b->active, b->n_schunk, (const int*)nullptr);
   if (b->nlb > 0)
     hipLaunchKernelGGL((k_b_schur_finish<CNP, R>), dim3(b->nlb), dim3(CNP * CNP > 128 ? 256 : 128), 0, b->stream,
                        b->lb_prob, b->lb_k, b->lb_l, b->lb_pair, b->psp, schpart, b->U, b->fix, b->mu, b->cam_off,

@@ -42,7 +42,8 @@ def same_index(a, b):
 
 
 P_INDEX = ["OBS_PT", "OBS_CAM", "PT_PTR", "CAM_PTR", "CAM_OBS", "CCHUNK_BEGIN", "CCHUNK_END", "CAM_CCHUNK_PTR",
-           "PAIR_ITEMS", "PAIR_K", "PAIR_L", "SCHUNK_BEGIN", "SCHUNK_END", "PAIR_SCHUNK_PTR", "LBLOCK_PAIR"]
+           "PAIR_ITEMS", "PAIR_K", "PAIR_L", "SCHUNK_BEGIN", "SCHUNK_END", "PAIR_SCHUNK_PTR", "LBLOCK_PAIR",
+           "SCHUNK_ORDER"]
 
 
 @pytest.mark.parametrize("config,kw", SCENES, ids=IDS)
