@@ -230,6 +230,7 @@ __global__ __launch_bounds__(256) void k_cam_finish(const int* __restrict__ ccp,
   if (run && !run[ownr[j]]) return;
   for (int t = threadIdx.x; t < NE; t += blockDim.x) {
     double acc = 0.0;
+#pragma unroll 8
     for (int q = ccp[j]; q < ccp[j + 1]; ++q) acc += (double)part[(size_t)q * NE + t];
     if (t < NU) {
       int r = 0, rem = t;

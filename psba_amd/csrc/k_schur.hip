@@ -152,6 +152,7 @@ __global__ __launch_bounds__(256) void k_schur_finish(const int* __restrict__ lb
     const int r = t / CNP, cc = t - r * CNP;
     double acc = 0.0;
     if (q >= 0)
+#pragma unroll 8
       for (int h = psp[q]; h < psp[q + 1]; ++h) acc += (double)part[(size_t)h * C2 + t];
     double val;
     if (k == l) {
@@ -211,6 +212,7 @@ __global__ __launch_bounds__(64) void k_ea_finish(const int* __restrict__ ccp, c
   int j = t / CNP, r = t - j * CNP;
   if (run && !run[ownr[j]]) return;
   double acc = 0.0;
+#pragma unroll 8
   for (int q = ccp[j]; q < ccp[j + 1]; ++q) acc += (double)part[(size_t)q * CNP + r];
   ea[t] = ga[t] - acc;
 }
@@ -314,6 +316,7 @@ __global__ __launch_bounds__(256) void k_b_schur_finish(const int* __restrict__ 
     const int r = t / CNP, cc = t - r * CNP;
     double acc = 0.0;
     if (q >= 0)
+#pragma unroll 8
       for (int h = psp[q]; h < psp[q + 1]; ++h) acc += (double)part[(size_t)h * C2 + t];
     double val;
     if (k == l) {
