@@ -214,7 +214,7 @@ int psba_create(const psba_desc* d, psba_ctx** out) {
   chk(dalloc(&c->Lf, (size_t)c->n_pad * c->n_pad));
   chk(dalloc(&c->bw, (size_t)c->n_pad));
   chk(dalloc(&c->yv, (size_t)c->n_pad));
-  chk(dalloc(&c->bar, (size_t)1));
+  chk(dalloc(&c->bar, (size_t)2 + (size_t)(c->n_pad / kNB)));  // barrier counter, back-substitution flags
   chk(dalloc(&c->chol_fail, (size_t)1));
   c->red_part_cap = 8 * (long)std::max(ceil_div(c->no, 256), ceil_div(c->ntot, 256)) + 64;
   chk(dalloc(&c->red_part, (size_t)c->red_part_cap));
@@ -231,7 +231,7 @@ int psba_create(const psba_desc* d, psba_ctx** out) {
     z(hipMemsetAsync(c->ea, 0, sizeof(double) * (size_t)c->n_pad, c->stream));
     z(hipMemsetAsync(c->dp, 0, sizeof(double) * (size_t)c->ntot, c->stream));
     z(hipMemsetAsync(c->d_scal, 0, sizeof(double) * SC_COUNT, c->stream));
-    z(hipMemsetAsync(c->bar, 0, sizeof(unsigned long long), c->stream));
+    z(hipMemsetAsync(c->bar, 0, sizeof(unsigned long long) * (2 + (size_t)(c->n_pad / kNB)), c->stream));
     z(hipMemsetAsync(c->Lf, 0, sizeof(double) * (size_t)c->n_pad * c->n_pad, c->stream));
     if (c->n_pad > c->n)
       hipLaunchKernelGGL(k_pad_identity, dim3(1), dim3(64), 0, c->stream, c->S, c->n_pad, c->n, c->n_pad);

@@ -376,6 +376,33 @@ __device__ __forceinline__ void grid_barrier(unsigned long long* bar, unsigned l
   grid_wait(bar, target, fail);
 }
 
+// Point-to-point flags (back substitution).  A flag only grows, across
+// launches too: a launch's values start above its epoch (its barrier base),
+// which exceeds every value an earlier launch wrote, so nothing is reset.
+// Publish: the calling wave's earlier stores become visible first (thread 0
+// only, after the caller's __syncthreads when other waves stored).
+__device__ __forceinline__ void flag_publish(unsigned long long* f, unsigned long long v) {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+  __hip_atomic_store(f, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Wait until *f >= v (thread 0 polls, bounded like grid_wait), then the whole
+// workgroup may read what was published before.
+__device__ __forceinline__ void flag_wait(unsigned long long* f, unsigned long long v, int* fail) {
+  if (threadIdx.x == 0) {
+    unsigned spins = 0;
+    while (__hip_atomic_load(f, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < v) {
+      __builtin_amdgcn_s_sleep(2);
+      if (++spins > (1u << 24)) {
+        __hip_atomic_fetch_or(fail, 2, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        break;
+      }
+    }
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+  }
+  __syncthreads();
+}
+
 // Tile transfers for the 256-thread kernels of this file: thread t moves the
 // elements t + 256 q (q = 0..15) of a 64x64 tile.  All 16 loads are issued
 // before the first LDS store, so a transfer waits for one memory latency.
@@ -826,27 +853,41 @@ __global__ __launch_bounds__(256) void k_chol_solve(double* __restrict__ A, int 
   // ---- back substitution L^T x = y: r_i = y_i - sum_{j > i} L_ji^T x_j with
   // the terms subtracted in descending j, then x_i = L_ii^-T r_i.  Workgroup 0
   // solves the chain and applies x_{i+2} and x_{i+1} to r_i itself; the other
-  // workgroups ("helpers") apply x_j, j >= i+3, each to the rows it owns, in
-  // step j-1, so that r_i is complete in yv two steps before workgroup 0
-  // needs it.  Everything a step reads that does not come from the step just
-  // finished (factor tiles, the completed r_i) is loaded into LDS while the
-  // workgroup waits at the barrier: L_kk, L_{k+1,k}, L_{k+2,k} and r_k by
-  // workgroup 0, the tile of its first row by each helper, which also keeps
-  // that row's r in registers.
-  const int first = (G > 1) ? 1 : 0, stride = (G > 1) ? G - 1 : 1;
-  const int i0 = (int)blockIdx.x - first;
-  const bool helper_pf = G > 1 && blockIdx.x >= 1;
-  double r_own = 0.0;  // helper: r of row i0 (lane tid < NB)
-  if (helper_pf && i0 + 3 <= T - 1 && tid < NB) r_own = yv[(size_t)i0 * NB + tid];
-  double r_next = 0.0;  // workgroup 0: r_k of the coming step (lane tid & 63)
-  if (blockIdx.x == 0) {
+  // workgroups ("helpers") apply x_j, j >= i+3, to the rows they own as soon
+  // as x_j is published, so r_i is complete two steps before workgroup 0
+  // needs it.  No grid barriers: workgroup 0 publishes x_k through a flag
+  // (bar[1] = epoch + T - k) and each helper publishes a completed row i
+  // through its own flag (bar[2 + i] = epoch + 1).  Workgroup 0 loads the next
+  // step's factor tiles and completed r while it is still working; a helper
+  // keeps its first row's r in registers and loads its next tile ahead.
+  const unsigned long long epoch = bar_base;
+  unsigned long long* xflag = bar + 1;
+  unsigned long long* rflag = bar + 2;
+  if (G == 1) {
+    // one workgroup: every term by workgroup 0, left-looking per block
+    for (int k = T - 1; k >= 0; --k) {
+      double br = yv[(size_t)k * NB + (tid & 63)];
+      for (int j = T - 1; j > k; --j) {
+        if (tid < NB) sm.w[tid] = xs[(size_t)j * NB + tid];
+        __syncthreads();
+        br -= tile_tmatvec(sm, tile(Lf, j, k), ld, sm.w);
+      }
+      load_lower(sm.P, tile(Lf, k, k), ld);
+      __syncthreads();
+      tile_backward(LDS(sm.P), br, xs + (size_t)k * NB, LDS(sm.v));
+      __syncthreads();
+      if (tid < NB) {
+        const int q = k * NB + tid;
+        if (q < n) dp[q] = sm.v[tid];
+      }
+      __syncthreads();
+    }
+  } else if (blockIdx.x == 0) {
     load_lower(sm.P, tile(Lf, T - 1, T - 1), ld);
-    r_next = yv[(size_t)(T - 1) * NB + (tid & 63)];
-  }
-  __syncthreads();
-  for (int k = T - 1; k >= 0; --k) {
-    if (blockIdx.x == 0) {
-      // sm.w = x_{k+1}, sm.rd = x_{k+2} (LDS vectors from the last two steps)
+    double r_next = yv[(size_t)(T - 1) * NB + (tid & 63)];
+    __syncthreads();
+    for (int k = T - 1; k >= 0; --k) {
+      // sm.w = x_{k+1}, sm.rd = x_{k+2}; sm.X = L_{k+1,k}, sm.Y = L_{k+2,k}
       double br = r_next;
       if (k + 2 < T) br -= tile_tmatvec(sm, sm.Y, TP, sm.rd);
       if (k + 1 < T) br -= tile_tmatvec(sm, sm.X, TP, sm.w);
@@ -859,38 +900,48 @@ __global__ __launch_bounds__(256) void k_chol_solve(double* __restrict__ A, int 
         const int q = k * NB + tid;
         if (q < n) dp[q] = sm.v[tid];
       }
+      if (k == 0) break;
+      // x_k was stored by wave 0 (tile_backward): publish it
+      if (tid == 0) flag_publish(xflag, epoch + (unsigned long long)(T - k));
+      double v0[16], v1[16], v2[16];
+      tile_regs(v0, tile(Lf, k - 1, k - 1), ld);
+      tile_regs(v1, tile(Lf, k, k - 1), ld);
+      if (k + 1 < T) tile_regs(v2, tile(Lf, k + 1, k - 1), ld);
+      if (k - 1 + 3 <= T - 1) flag_wait(rflag + (k - 1), epoch + 1, fail);
+      r_next = yv[(size_t)(k - 1) * NB + (tid & 63)];
+      __syncthreads();
+      regs_to_lds(sm.P, v0, true);
+      regs_to_lds(sm.X, v1, false);
+      if (k + 1 < T) regs_to_lds(sm.Y, v2, false);
       __syncthreads();
     }
-    // helpers: x_{k+1} to the rows i <= k-2 they own (i = i0, i0 + stride, ..)
-    if (k + 1 < T && (int)blockIdx.x >= first && i0 <= k - 2) {
-      if (tid < NB) sm.v[tid] = xs[(size_t)(k + 1) * NB + tid];
+  } else {
+    // helper: rows i0, i0 + stride, .. (those with i + 3 <= T - 1)
+    const int i0 = (int)blockIdx.x - 1, stride = G - 1;
+    if (i0 + 3 <= T - 1) {
+      double r_own = (tid < NB) ? yv[(size_t)i0 * NB + tid] : 0.0;
+      load_full(sm.X, tile(Lf, T - 1, i0), ld);
       __syncthreads();
-      for (int i = i0; i <= k - 2; i += stride) {
-        if (helper_pf && i == i0) {
-          const double t = tile_tmatvec(sm, sm.X, TP, sm.v);
-          r_own -= t;
-          if (i0 == k - 2 && tid < NB) yv[(size_t)i0 * NB + tid] = r_own;
-        } else {
-          const double t = tile_tmatvec(sm, tile(Lf, k + 1, i), ld, sm.v);
-          if (tid < NB) yv[(size_t)i * NB + tid] -= t;
+      for (int j = T - 1; j >= i0 + 3; --j) {
+        flag_wait(xflag, epoch + (unsigned long long)(T - j), fail);
+        if (tid < NB) sm.v[tid] = xs[(size_t)j * NB + tid];
+        __syncthreads();
+        for (int i = i0; i + 3 <= j; i += stride) {
+          if (i == i0) {
+            const double t = tile_tmatvec(sm, sm.X, TP, sm.v);
+            r_own -= t;
+            if (i + 3 == j && tid < NB) yv[(size_t)i * NB + tid] = r_own;
+          } else {
+            const double t = tile_tmatvec(sm, tile(Lf, j, i), ld, sm.v);
+            if (tid < NB) yv[(size_t)i * NB + tid] -= t;
+          }
+          if (i + 3 == j && tid == 0) flag_publish(rflag + i, epoch + 1);  // stores by wave 0
+        }
+        if (j - 1 >= i0 + 3) {
+          load_full(sm.X, tile(Lf, j - 1, i0), ld);
+          __syncthreads();
         }
       }
-    }
-    if (k > 0) {
-      grid_arrive(bar);
-      if (blockIdx.x == 0) {
-        double v0[16], v1[16], v2[16];
-        tile_regs(v0, tile(Lf, k - 1, k - 1), ld);
-        tile_regs(v1, tile(Lf, k, k - 1), ld);
-        if (k + 1 < T) tile_regs(v2, tile(Lf, k + 1, k - 1), ld);
-        r_next = yv[(size_t)(k - 1) * NB + (tid & 63)];
-        regs_to_lds(sm.P, v0, true);
-        regs_to_lds(sm.X, v1, false);
-        if (k + 1 < T) regs_to_lds(sm.Y, v2, false);
-      } else if (helper_pf && i0 <= k - 3) {
-        load_full(sm.X, tile(Lf, k, i0), ld);
-      }
-      grid_wait(bar, bar_base + (unsigned long long)G * (++nbar), fail);
     }
   }
   if (blockIdx.x == 0 && tid == 0) {
@@ -966,7 +1017,7 @@ int launch_factor_solve(psba_ctx* c) {
   }
   int G = c->chol_grid;
   unsigned long long base = c->bar_base;
-  c->bar_base += (unsigned long long)G * (unsigned long long)(2 * T - 1);
+  c->bar_base += (unsigned long long)G * (unsigned long long)T;
   double* ok = c->d_scal + SC_FACTOR_OK;
   int ld = c->n_pad, n = c->n;
   probe_begin(c, PSBA_PROBE_FACTOR);
