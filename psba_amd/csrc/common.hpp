@@ -116,6 +116,31 @@ inline void launch_sum_partials(hipStream_t st, const double* partials, int nb, 
 
 inline unsigned ceil_div(long a, long b) { return (unsigned)((a + b - 1) / b); }
 
+// Sum over the four lanes of a quad (DPP quad permutes), the same value on
+// all four: (v0 + v1) + (v2 + v3) in a fixed order.  Every lane of the wave
+// must take part.
+__device__ __forceinline__ double quad_partner(double v, int ctrl_is_xor2) {
+  const long long bits = __double_as_longlong(v);
+  int lo, hi;
+  if (ctrl_is_xor2) {
+    lo = __builtin_amdgcn_mov_dpp((int)bits, 0x4E, 0xF, 0xF, true);
+    hi = __builtin_amdgcn_mov_dpp((int)(bits >> 32), 0x4E, 0xF, 0xF, true);
+  } else {
+    lo = __builtin_amdgcn_mov_dpp((int)bits, 0xB1, 0xF, 0xF, true);
+    hi = __builtin_amdgcn_mov_dpp((int)(bits >> 32), 0xB1, 0xF, 0xF, true);
+  }
+  return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+__device__ __forceinline__ float quad_partner(float v, int ctrl_is_xor2) {
+  return __int_as_float(ctrl_is_xor2 ? __builtin_amdgcn_mov_dpp(__float_as_int(v), 0x4E, 0xF, 0xF, true)
+                                     : __builtin_amdgcn_mov_dpp(__float_as_int(v), 0xB1, 0xF, 0xF, true));
+}
+template <typename R>
+__device__ __forceinline__ R quad_sum(R v) {
+  const R a = v + quad_partner(v, 0);  // lanes (0,1) and (2,3)
+  return a + quad_partner(a, 1);       // the two pair sums
+}
+
 // Launch order of the Schur pair chunks: the chunks of camera pairs in the
 // same block of 8 x 8 cameras and the same range of 4096 points run together,
 // so the W rows they gather come back from L2 instead of HBM (with the XCD

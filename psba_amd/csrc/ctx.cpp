@@ -231,6 +231,7 @@ int psba_create(const psba_desc* d, psba_ctx** out) {
     z(hipMemsetAsync(c->ea, 0, sizeof(double) * (size_t)c->n_pad, c->stream));
     z(hipMemsetAsync(c->dp, 0, sizeof(double) * (size_t)c->ntot, c->stream));
     z(hipMemsetAsync(c->d_scal, 0, sizeof(double) * SC_COUNT, c->stream));
+    z(hipMemsetAsync(c->chol_fail, 0, sizeof(int), c->stream));
     z(hipMemsetAsync(c->bar, 0, sizeof(unsigned long long) * (2 + (size_t)(c->n_pad / kNB)), c->stream));
     z(hipMemsetAsync(c->Lf, 0, sizeof(double) * (size_t)c->n_pad * c->n_pad, c->stream));
     if (c->n_pad > c->n)

@@ -465,42 +465,66 @@ __device__ __forceinline__ double quad_bcast(double v) {
 PSBA_OUTLINE void trsm_rows(lds_dbl* out, const double* A, int ld, const lds_dbl* L, const lds_dbl* rd) {
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   const int r = 16 * w + (lane >> 2), g = lane & 3;
-  double x[16], rdv[16], Lc[16];
+  const int li = lane & 15, lk = lane >> 4;
+  double x[16];
 #pragma unroll
-  for (int q = 0; q < 16; ++q) {
-    x[q] = A[(size_t)r * ld + 4 * q + g];
-    rdv[q] = rd[4 * q + g];
-    Lc[q] = L[(4 * q + g) * TP];
-  }
-  // Column c: the owner lane (g == c % 4) scales its accumulated value, the
-  // quad shares it, and every later column of the row takes its term at once
-  // (right-looking), so a column's dependent chain is a multiply, a quad
-  // broadcast and one fused multiply-add.  Column c accumulates
-  // a_c - x_0 L[c][0] - x_1 L[c][1] - ... in ascending order, times 1/L[c][c].
+  for (int q = 0; q < 16; ++q) x[q] = A[(size_t)r * ld + 4 * q + g];
+  // Four blocks of 16 columns.  Inside block b, column c: the owner lane
+  // (g == c % 4) scales its accumulated value, the quad shares it, and the
+  // block's later columns take their terms (right-looking: a multiply, a quad
+  // broadcast and one FMA on the chain).  The block then goes to LDS and the
+  // later blocks take X_b L_{b',b}^T by FP64 MFMA (the wave's own 16 rows, so
+  // no workgroup barrier), through LDS back into the quad layout.
 #pragma unroll
-  for (int c = 0; c < NB; ++c) {
-    const int q0 = c >> 2, g0 = c & 3;
-    const double xo = x[q0] * rdv[q0];
-    double xc;
-    if (g0 == 0) xc = quad_bcast<0>(xo);
-    else if (g0 == 1) xc = quad_bcast<1>(xo);
-    else if (g0 == 2) xc = quad_bcast<2>(xo);
-    else xc = quad_bcast<3>(xo);
-    const double fm = fma(-xc, Lc[q0], x[q0]);
-    x[q0] = (g == g0) ? xc : ((g > g0) ? fm : x[q0]);
+  for (int b = 0; b < 4; ++b) {
+    double rdv[4], Lc[4];
 #pragma unroll
-    for (int q = q0 + 1; q < 16; ++q) x[q] = fma(-xc, Lc[q], x[q]);
-    if (c + 1 < NB) {
-#pragma unroll
-      for (int q = (c + 1) >> 2; q < 16; ++q) Lc[q] = L[(4 * q + g) * TP + c + 1];
+    for (int q = 0; q < 4; ++q) {
+      rdv[q] = rd[16 * b + 4 * q + g];
+      Lc[q] = L[(16 * b + 4 * q + g) * TP + 16 * b];
     }
-    // pin this column's updates here: left alone, the compiler defers the
-    // off-chain updates to their last use and runs out of registers
 #pragma unroll
-    for (int q = q0; q < 16; ++q) __asm__ volatile("" : "+v"(x[q]));
+    for (int cb = 0; cb < 16; ++cb) {
+      const int q0 = cb >> 2, g0 = cb & 3, c = 16 * b + cb;
+      const double xo = x[4 * b + q0] * rdv[q0];
+      double xc;
+      if (g0 == 0) xc = quad_bcast<0>(xo);
+      else if (g0 == 1) xc = quad_bcast<1>(xo);
+      else if (g0 == 2) xc = quad_bcast<2>(xo);
+      else xc = quad_bcast<3>(xo);
+      const double fm = fma(-xc, Lc[q0], x[4 * b + q0]);
+      x[4 * b + q0] = (g == g0) ? xc : ((g > g0) ? fm : x[4 * b + q0]);
+#pragma unroll
+      for (int q = q0 + 1; q < 4; ++q) x[4 * b + q] = fma(-xc, Lc[q], x[4 * b + q]);
+      if (cb + 1 < 16) {
+#pragma unroll
+        for (int q = (cb + 1) >> 2; q < 4; ++q) Lc[q] = L[(16 * b + 4 * q + g) * TP + c + 1];
+      }
+#pragma unroll
+      for (int q = q0; q < 4; ++q) __asm__ volatile("" : "+v"(x[4 * b + q]));
+    }
+#pragma unroll
+    for (int q = 0; q < 4; ++q) out[r * TP + 16 * b + 4 * q + g] = x[4 * b + q];
+    if (b < 3) {
+      // P = X_b L_{b',b}^T for every later block b' (MFMA layout: register k of
+      // lane l is row 4k + l/16, column l%16), stored over block b' of `out`
+      // and read back in the quad layout
+#pragma unroll
+      for (int bp = b + 1; bp < 4; ++bp) {
+        dvec4 acc = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+        for (int k0 = 0; k0 < 16; k0 += 4) {
+          const double av = out[(16 * w + li) * TP + 16 * b + k0 + lk];
+          const double bv = L[(16 * bp + li) * TP + 16 * b + k0 + lk];
+          acc = __builtin_amdgcn_mfma_f64_16x16x4f64(av, bv, acc, 0, 0, 0);
+        }
+#pragma unroll
+        for (int k = 0; k < 4; ++k) out[(16 * w + 4 * k + lk) * TP + 16 * bp + li] = acc[k];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) x[4 * bp + q] -= out[r * TP + 16 * bp + 4 * q + g];
+      }
+    }
   }
-#pragma unroll
-  for (int q = 0; q < 16; ++q) out[r * TP + 4 * q + g] = x[q];
 }
 
 // value of v on lane N of each 16-lane row (DPP row_newbcast)
@@ -817,8 +841,9 @@ __global__ __launch_bounds__(256) void k_chol_solve(double* __restrict__ A, int 
       __syncthreads();
       acc_update(acc, sm.X, (i != j) ? sm.Y : sm.X, i == j);
       if (j == k + 1) {
-        // store L_ik; b_i -= L_ik y_k (4 threads per row, fixed order)
-        store_tile(sm.X, tile(Lf, i, k), ld);
+        // store L_ik (the diagonal task stores it later, beside its forward
+        // solve); b_i -= L_ik y_k (4 threads per row, fixed order)
+        if (i != j) store_tile(sm.X, tile(Lf, i, k), ld);
         double s = 0.0;
 #pragma unroll
         for (int c = 0; c < 16; ++c) s += sm.X[br_row * TP + 16 * br_p + c] * sm.v[16 * br_p + c];
@@ -841,6 +866,7 @@ __global__ __launch_bounds__(256) void k_chol_solve(double* __restrict__ A, int 
         const int bad = tile_potrf(*(lds_smem*)&sm, LDS(Dt), carry);
         if (bad && tid == 0) __hip_atomic_fetch_or(fail, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         store_tile_w123(Dt, tile(Lf, i, i), ld);
+        store_tile_w123(sm.X, tile(Lf, i, k), ld);
         tile_forward(LDS(Dt), LDS(sm.part[0]), yv + (size_t)i * NB, LDS(sm.w));
       } else {
         acc_store(acc, tile(A, i, j), ld);
@@ -947,10 +973,12 @@ __global__ __launch_bounds__(256) void k_chol_solve(double* __restrict__ A, int 
   if (blockIdx.x == 0 && tid == 0) {
     const int f = __hip_atomic_load(fail, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     *ok = f ? 0.0 : 1.0;
+    // cleared for the next launch (every other workgroup is past its last
+    // wait: workgroup 0 consumed every helper's last flag)
+    __hip_atomic_store(fail, 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   }
 }
 
-__global__ void k_zero_int(int* p) { *p = 0; }
 
 // S x = b with the factor L already in Lf (one workgroup; used by the mixed
 // path's FP64 refinement): forward L y = b, then back L^T x = y, tile by tile.
@@ -1021,7 +1049,6 @@ int launch_factor_solve(psba_ctx* c) {
   double* ok = c->d_scal + SC_FACTOR_OK;
   int ld = c->n_pad, n = c->n;
   probe_begin(c, PSBA_PROBE_FACTOR);
-  hipLaunchKernelGGL(k_zero_int, dim3(1), dim3(1), 0, c->stream, c->chol_fail);
   hipLaunchKernelGGL(k_chol_solve, dim3(G), dim3(256), 0, c->stream, c->S, ld, T, c->Lf, c->ea, c->bw, c->yv,
                      c->xsol, c->dp, n, ok, c->bar, base, c->chol_fail);
   probe_end(c, PSBA_PROBE_FACTOR);

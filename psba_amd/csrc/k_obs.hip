@@ -136,27 +136,36 @@ __global__ __launch_bounds__(256) void k_point_normal(const int* __restrict__ pt
                                                       double* __restrict__ gb, double* __restrict__ uvd,
                                                       const int* __restrict__ ownr,
                                                       const unsigned char* __restrict__ run) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= np) return;
-  if (run && !run[ownr[i]]) return;
+  // a quad per point: lane p sums a quarter of the point's observations (a
+  // long track is split four ways), the quad adds the quarters in a fixed order
+  const long gt = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  const int i = (int)(gt >> 2), p = (int)(gt & 3);
+  const bool ok = i < np && (!run || run[ownr[i]]);
   R v[6] = {0, 0, 0, 0, 0, 0}, g[3] = {0, 0, 0};
-  // unrolled so that several observations' loads are in flight at once (a
-  // long track otherwise waits one load latency per observation)
+  if (ok) {
+    const int beg = pt_ptr[i], cnt = pt_ptr[i + 1] - beg;
+    const int o0 = beg + ((cnt * p) >> 2), o1 = beg + ((cnt * (p + 1)) >> 2);
 #pragma unroll 4
-  for (int o = pt_ptr[i]; o < pt_ptr[i + 1]; ++o) {
-    const R* b = JB + (size_t)o * 6;
-    R b0 = b[0], b1 = b[1], b2 = b[2], b3 = b[3], b4 = b[4], b5 = b[5];
-    R e0 = e[2 * o], e1 = e[2 * o + 1];
-    v[0] += b0 * b0 + b3 * b3;
-    v[1] += b0 * b1 + b3 * b4;
-    v[2] += b0 * b2 + b3 * b5;
-    v[3] += b1 * b1 + b4 * b4;
-    v[4] += b1 * b2 + b4 * b5;
-    v[5] += b2 * b2 + b5 * b5;
-    g[0] += b0 * e0 + b3 * e1;
-    g[1] += b1 * e0 + b4 * e1;
-    g[2] += b2 * e0 + b5 * e1;
+    for (int o = o0; o < o1; ++o) {
+      const R* b = JB + (size_t)o * 6;
+      R b0 = b[0], b1 = b[1], b2 = b[2], b3 = b[3], b4 = b[4], b5 = b[5];
+      R e0 = e[2 * o], e1 = e[2 * o + 1];
+      v[0] += b0 * b0 + b3 * b3;
+      v[1] += b0 * b1 + b3 * b4;
+      v[2] += b0 * b2 + b3 * b5;
+      v[3] += b1 * b1 + b4 * b4;
+      v[4] += b1 * b2 + b4 * b5;
+      v[5] += b2 * b2 + b5 * b5;
+      g[0] += b0 * e0 + b3 * e1;
+      g[1] += b1 * e0 + b4 * e1;
+      g[2] += b2 * e0 + b5 * e1;
+    }
   }
+#pragma unroll
+  for (int k = 0; k < 6; ++k) v[k] = quad_sum(v[k]);
+#pragma unroll
+  for (int k = 0; k < 3; ++k) g[k] = quad_sum(g[k]);
+  if (!ok || p != 0) return;
 #pragma unroll
   for (int k = 0; k < 6; ++k) V[(size_t)i * 6 + k] = coeff * (double)v[k];
 #pragma unroll
@@ -194,7 +203,12 @@ __global__ __launch_bounds__(256) void k_cam_chunks(const int* __restrict__ cam_
     dst[2 * CNP + 1] = e[2 * o + 1];
   }
   __syncthreads();
-  for (int t = threadIdx.x; t < NE; t += blockDim.x) {
+  // a quad per output: lane p of the quad sums observations p, p + 4, ..
+  // (neighbouring records: no LDS bank conflicts inside the quad), the quad
+  // adds the four in a fixed order
+  const int p = threadIdx.x & 3;
+  for (int base = 0; base < NE; base += (int)blockDim.x >> 2) {
+    const int t = base + ((int)threadIdx.x >> 2);
     R acc = 0;
     if (t < NU) {
       int r = 0, rem = t;
@@ -203,18 +217,19 @@ __global__ __launch_bounds__(256) void k_cam_chunks(const int* __restrict__ cam_
         ++r;
       }
       int cc = r + rem;
-      for (int s = 0; s < cnt; ++s) {
+      for (int s = p; s < cnt; s += 4) {
         const R* a = sJ + s * REC;
         acc += a[r] * a[cc] + a[CNP + r] * a[CNP + cc];
       }
-    } else {
+    } else if (t < NE) {
       int r = t - NU;
-      for (int s = 0; s < cnt; ++s) {
+      for (int s = p; s < cnt; s += 4) {
         const R* a = sJ + s * REC;
         acc += a[r] * a[2 * CNP] + a[CNP + r] * a[2 * CNP + 1];
       }
     }
-    part[(size_t)b * NE + t] = acc;
+    acc = quad_sum(acc);
+    if (t < NE && p == 0) part[(size_t)b * NE + t] = acc;
   }
 }
 
@@ -256,7 +271,7 @@ static void run_jac_normal_t(psba_ctx* c, double coeff, double coeff_g, R* e, R*
                      c->cams, c->pts, c->meas, c->obs_cam, c->obs_pt, c->fix, c->no, coeff, e, JA, JB, W,
                      (const int*)nullptr, (const unsigned char*)nullptr, c->loss, c->loss_scale);
   probe_end(c, PSBA_PROBE_JACOBIAN);
-  hipLaunchKernelGGL(k_point_normal<R>, dim3(ceil_div(c->np, 256)), dim3(256), 0, c->stream, c->pt_ptr, JB, e,
+  hipLaunchKernelGGL(k_point_normal<R>, dim3(ceil_div(4L * c->np, 256)), dim3(256), 0, c->stream, c->pt_ptr, JB, e,
                      c->np, coeff, coeff_g, c->V, c->g + c->n, c->uvdiag + c->n, (const int*)nullptr,
                      (const unsigned char*)nullptr);
   if (c->n_cchunk > 0)
@@ -844,7 +859,7 @@ static void run_b_jac_t(psba_batch* b) {
   hipLaunchKernelGGL((k_jacobian<CNP, R>), dim3(ceil_div(b->NO, bs)), dim3(bs), 0, b->stream, b->K, b->q0, b->cams,
                      b->pts, b->meas, b->obs_cam, b->obs_pt, b->fix, (int)b->NO, 1.0, e, JA, JB, W, b->obs_prob,
                      b->need_jac, PSBA_LOSS_NONE, 1.0);
-  hipLaunchKernelGGL(k_point_normal<R>, dim3(ceil_div(b->NP, b->pt_block)), dim3(b->pt_block), 0, b->stream, b->pt_ptr, JB, e,
+  hipLaunchKernelGGL(k_point_normal<R>, dim3(ceil_div(4L * b->NP, b->pt_block)), dim3(b->pt_block), 0, b->stream, b->pt_ptr, JB, e,
                      (int)b->NP, 1.0, 1.0, b->V, b->g + b->N, b->uvdiag + b->N, b->pt_prob, b->need_jac);
   if (b->n_cchunk > 0)
     hipLaunchKernelGGL((k_cam_chunks<CNP, R>), dim3(b->n_cchunk), dim3(256), 0, b->stream, b->cam_obs, b->cchunk_b,

@@ -196,10 +196,15 @@ __global__ __launch_bounds__(256) void k_ea_chunks(const int* __restrict__ cam_o
     sP[task] = w[0] * (R)zi[0] + w[1] * (R)zi[1] + w[2] * (R)zi[2];
   }
   __syncthreads();
-  if (threadIdx.x < CNP) {
+  // a quad per output, lane p over observations p, p + 4, .., added in a
+  // fixed order
+  if (threadIdx.x < 64) {
+    const int t = threadIdx.x >> 2, p = threadIdx.x & 3;
     R acc = 0;
-    for (int s = 0; s < cnt; ++s) acc += sP[s * CNP + threadIdx.x];
-    part[(size_t)b * CNP + threadIdx.x] = acc;
+    if (t < CNP)
+      for (int s = p; s < cnt; s += 4) acc += sP[s * CNP + t];
+    acc = quad_sum(acc);
+    if (t < CNP && p == 0) part[(size_t)b * CNP + t] = acc;
   }
 }
 
