@@ -458,17 +458,17 @@ __device__ __forceinline__ double quad_bcast(double v) {
   return quad_perm<G0 * 0x55>(v);
 }
 
-// X = A L^-T (A a global 64x64 tile, L = sm.P lower with zeros above): rows
+// X = A L^-T in place (A an LDS tile, L = sm.P lower with zeros above): rows
 // are independent; wave w owns rows 16w..16w+15, the four lanes of a quad
 // share a row and hold its columns c = 4q + g (g = lane % 4).  Branch-free;
 // no workgroup barriers.
-PSBA_OUTLINE void trsm_rows(lds_dbl* out, const double* A, int ld, const lds_dbl* L, const lds_dbl* rd) {
+PSBA_OUTLINE void trsm_rows(lds_dbl* out, const lds_dbl* L, const lds_dbl* rd) {
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   const int r = 16 * w + (lane >> 2), g = lane & 3;
   const int li = lane & 15, lk = lane >> 4;
   double x[16];
 #pragma unroll
-  for (int q = 0; q < 16; ++q) x[q] = A[(size_t)r * ld + 4 * q + g];
+  for (int q = 0; q < 16; ++q) x[q] = out[r * TP + 4 * q + g];
   // Four blocks of 16 columns.  Inside block b, column c: the owner lane
   // (g == c % 4) scales its accumulated value, the quad shares it, and the
   // block's later columns take their terms (right-looking: a multiply, a quad
@@ -477,29 +477,28 @@ PSBA_OUTLINE void trsm_rows(lds_dbl* out, const double* A, int ld, const lds_dbl
   // no workgroup barrier), through LDS back into the quad layout.
 #pragma unroll
   for (int b = 0; b < 4; ++b) {
-    double rdv[4], Lc[4];
+    // this lane's rows of the diagonal block L_bb, all loaded up front (the
+    // column chain then waits on no LDS access)
+    double rdv[4], Lb[4][16];
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
       rdv[q] = rd[16 * b + 4 * q + g];
-      Lc[q] = L[(16 * b + 4 * q + g) * TP + 16 * b];
+#pragma unroll
+      for (int cb = 0; cb < 16; ++cb) Lb[q][cb] = (4 * q + g > cb) ? L[(16 * b + 4 * q + g) * TP + 16 * b + cb] : 0.0;
     }
 #pragma unroll
     for (int cb = 0; cb < 16; ++cb) {
-      const int q0 = cb >> 2, g0 = cb & 3, c = 16 * b + cb;
+      const int q0 = cb >> 2, g0 = cb & 3;
       const double xo = x[4 * b + q0] * rdv[q0];
       double xc;
       if (g0 == 0) xc = quad_bcast<0>(xo);
       else if (g0 == 1) xc = quad_bcast<1>(xo);
       else if (g0 == 2) xc = quad_bcast<2>(xo);
       else xc = quad_bcast<3>(xo);
-      const double fm = fma(-xc, Lc[q0], x[4 * b + q0]);
+      const double fm = fma(-xc, Lb[q0][cb], x[4 * b + q0]);
       x[4 * b + q0] = (g == g0) ? xc : ((g > g0) ? fm : x[4 * b + q0]);
 #pragma unroll
-      for (int q = q0 + 1; q < 4; ++q) x[4 * b + q] = fma(-xc, Lc[q], x[4 * b + q]);
-      if (cb + 1 < 16) {
-#pragma unroll
-        for (int q = (cb + 1) >> 2; q < 4; ++q) Lc[q] = L[(16 * b + 4 * q + g) * TP + c + 1];
-      }
+      for (int q = q0 + 1; q < 4; ++q) x[4 * b + q] = fma(-xc, Lb[q][cb], x[4 * b + q]);
 #pragma unroll
       for (int q = q0; q < 4; ++q) __asm__ volatile("" : "+v"(x[4 * b + q]));
     }
@@ -722,6 +721,20 @@ __device__ __forceinline__ void acc_update(dvec4 (&acc)[2][2], const double* X, 
   }
 }
 
+// A tile task's inputs: A_ik (and A_jk) into sm.X (sm.Y) for the in-place
+// solves and A_ij into the accumulators, every load issued before the first
+// is waited for
+__device__ __forceinline__ void task_load(dvec4 (&acc)[2][2], CholSmem& sm, const double* Aij, const double* Aik,
+                                          const double* Ajk, int ld) {
+  double va[16], vb[16];
+  tile_regs(va, Aik, ld);
+  if (Ajk) tile_regs(vb, Ajk, ld);
+  acc_load(acc, Aij, ld);
+  regs_to_lds(sm.X, va, false);
+  if (Ajk) regs_to_lds(sm.Y, vb, false);
+  __syncthreads();
+}
+
 // y = L^-1 b (L = lower tile in LDS), wave 0, lane = row; the lane's row of
 // L is loaded first, the solved value is broadcast by readlane, and every
 // lane updates unconditionally (a lane past its own row no longer matters,
@@ -835,9 +848,9 @@ __global__ __launch_bounds__(256) void k_chol_solve(double* __restrict__ A, int 
       const int br_row = tid >> 2, br_p = tid & 3;
       const double b_old = (j == k + 1 && br_p == 0) ? bw[(size_t)i * NB + br_row] : 0.0;
       dvec4 acc[2][2];
-      acc_load(acc, tile(A, i, j), ld);
-      trsm_rows(LDS(sm.X), tile(A, i, k), ld, LDS(sm.P), LDS(sm.rd));
-      if (i != j) trsm_rows(LDS(sm.Y), tile(A, j, k), ld, LDS(sm.P), LDS(sm.rd));
+      task_load(acc, sm, tile(A, i, j), tile(A, i, k), (i != j) ? tile(A, j, k) : nullptr, ld);
+      trsm_rows(LDS(sm.X), LDS(sm.P), LDS(sm.rd));
+      if (i != j) trsm_rows(LDS(sm.Y), LDS(sm.P), LDS(sm.rd));
       __syncthreads();
       acc_update(acc, sm.X, (i != j) ? sm.Y : sm.X, i == j);
       if (j == k + 1) {
@@ -913,6 +926,12 @@ __global__ __launch_bounds__(256) void k_chol_solve(double* __restrict__ A, int 
     double r_next = yv[(size_t)(T - 1) * NB + (tid & 63)];
     __syncthreads();
     for (int k = T - 1; k >= 0; --k) {
+      // r_{k-1} is complete two steps ahead: load it now, use it next step
+      double r_pre = 0.0;
+      if (k >= 1) {
+        if (k - 1 + 3 <= T - 1) flag_wait(rflag + (k - 1), epoch + 1, fail);
+        r_pre = yv[(size_t)(k - 1) * NB + (tid & 63)];
+      }
       // sm.w = x_{k+1}, sm.rd = x_{k+2}; sm.X = L_{k+1,k}, sm.Y = L_{k+2,k}
       double br = r_next;
       if (k + 2 < T) br -= tile_tmatvec(sm, sm.Y, TP, sm.rd);
@@ -933,8 +952,7 @@ __global__ __launch_bounds__(256) void k_chol_solve(double* __restrict__ A, int 
       tile_regs(v0, tile(Lf, k - 1, k - 1), ld);
       tile_regs(v1, tile(Lf, k, k - 1), ld);
       if (k + 1 < T) tile_regs(v2, tile(Lf, k + 1, k - 1), ld);
-      if (k - 1 + 3 <= T - 1) flag_wait(rflag + (k - 1), epoch + 1, fail);
-      r_next = yv[(size_t)(k - 1) * NB + (tid & 63)];
+      r_next = r_pre;
       __syncthreads();
       regs_to_lds(sm.P, v0, true);
       regs_to_lds(sm.X, v1, false);
@@ -1119,9 +1137,9 @@ __global__ __launch_bounds__(256) void k_b_chol_solve(double* __restrict__ S, do
       while (cum + (m - a) <= t) { cum += m - a; ++a; }
       const int j = k + 1 + a, i = j + (t - cum);
       dvec4 acc[2][2];
-      acc_load(acc, tile(A, i, j), ld);
-      trsm_rows(LDS(sm.X), tile(A, i, k), ld, LDS(sm.P), LDS(sm.rd));
-      if (i != j) trsm_rows(LDS(sm.Y), tile(A, j, k), ld, LDS(sm.P), LDS(sm.rd));
+      task_load(acc, sm, tile(A, i, j), tile(A, i, k), (i != j) ? tile(A, j, k) : nullptr, ld);
+      trsm_rows(LDS(sm.X), LDS(sm.P), LDS(sm.rd));
+      if (i != j) trsm_rows(LDS(sm.Y), LDS(sm.P), LDS(sm.rd));
       __syncthreads();
       acc_update(acc, sm.X, (i != j) ? sm.Y : sm.X, i == j);
       if (j == k + 1) {
@@ -1264,9 +1282,9 @@ __global__ __launch_bounds__(256) void k_b_chol_phase(int k, const int* __restri
   while (cum + (m - a) <= t) { cum += m - a; ++a; }
   const int j = k + 1 + a, i = j + (t - cum);
   dvec4 acc[2][2];
-  acc_load(acc, tile(A, i, j), ld);
-  trsm_rows(LDS(sm.X), tile(A, i, k), ld, LDS(sm.P), LDS(sm.rd));
-  if (i != j) trsm_rows(LDS(sm.Y), tile(A, j, k), ld, LDS(sm.P), LDS(sm.rd));
+  task_load(acc, sm, tile(A, i, j), tile(A, i, k), (i != j) ? tile(A, j, k) : nullptr, ld);
+  trsm_rows(LDS(sm.X), LDS(sm.P), LDS(sm.rd));
+  if (i != j) trsm_rows(LDS(sm.Y), LDS(sm.P), LDS(sm.rd));
   __syncthreads();
   acc_update(acc, sm.X, (i != j) ? sm.Y : sm.X, i == j);
   if (j == k + 1) {
