@@ -526,13 +526,11 @@ PSBA_OUTLINE void trsm_rows(lds_dbl* out, const lds_dbl* L, const lds_dbl* rd) {
   }
 }
 
-// value of v on lane N of each 16-lane row (DPP row_newbcast)
+// value of v on lane N of each 16-lane row (DPP row_newbcast; CDNA's 64-bit
+// DPP takes this control, so the move is one v_mov_b64_dpp, not two halves)
 template <int N>
 __device__ __forceinline__ double row_bcast(double v) {
-  const long long b = __double_as_longlong(v);
-  const int lo = __builtin_amdgcn_mov_dpp((int)b, 0x150 + N, 0xF, 0xF, true);
-  const int hi = __builtin_amdgcn_mov_dpp((int)(b >> 32), 0x150 + N, 0xF, 0xF, true);
-  return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+  return __builtin_amdgcn_mov_dpp(v, 0x150 + N, 0xF, 0xF, true);
 }
 
 // 1/x for a normal, finite x: hardware reciprocal estimate refined by two
