@@ -24,6 +24,7 @@ sys.path.insert(0, ROOT)
 
 METRIC = "time-to-converge vs plain FP64 at equal RMS; solves/s + p99 latency (stream)"
 FP64_PEAK_TFLOPS = 78.6  # MI355X dense FP64 (matrix and vector)
+HBM_PEAK_GBS = 8000.0     # MI355X HBM3E
 
 
 def log(*a):
@@ -290,6 +291,43 @@ def stream_section(P, seed=1004, rates=(200, 1000, 2000, 4000), duration=2.0, se
     return out
 
 
+def stage_rates(P, ctx, solve_on, sc, info, r, solve_s):
+    """Absolute rates of the headline solve: LM iterations and trials per
+    second, and the Jacobian and dense-solve stages, each timed by HIP events
+    on the context's stream over one further solve, with their roofline
+    fractions.  Jacobian bytes per observation: the reads (camera, point,
+    measurement, indices: 40 B) and the writes (residual 2, J_A 2C, J_B 6 and
+    W 3C doubles).  Dense solve flops: n_pad^3 / 3 for the factor plus 2
+    n_pad^2 for the two triangular solves."""
+    out = {"lm_iterations_per_s": r["itno"] / solve_s, "lm_trials_per_s": len(r["trace"]) / solve_s}
+    timed = {}
+    for name, which in (("jacobian", P.PROBE_JACOBIAN), ("dense_solve", P.PROBE_FACTOR)):
+        ctx.probe(which)
+        ctx.probe_read()
+        solve_on(ctx)
+        timed[name] = ctx.probe_read()
+        ctx.probe(P.PROBE_OFF)
+    C = sc.cnp
+    ms, n = timed["jacobian"]
+    avg = ms * 1e-3 / max(n, 1)
+    nbytes = ((2 + 2 * C + 6 + 3 * C) * 8 + 40) * sc.n_obs
+    out["jacobian"] = {"launches": n, "avg_launch_us": avg * 1e6, "obs_per_s": sc.n_obs / avg,
+                       "bytes_per_launch": nbytes, "achieved_GBps": nbytes / avg / 1e9,
+                       "frac_of_hbm_peak": nbytes / avg / 1e9 / HBM_PEAK_GBS}
+    ms, n = timed["dense_solve"]
+    avg = ms * 1e-3 / max(n, 1)
+    npad = float(info["n_pad"])
+    flops = npad ** 3 / 3.0 + 2.0 * npad * npad
+    out["dense_solve"] = {"launches": n, "avg_launch_us": avg * 1e6, "n_pad": int(npad), "flops_per_launch": flops,
+                          "achieved_GFLOPs": flops / avg / 1e9,
+                          "frac_of_fp64_peak": flops / avg / 1e12 / FP64_PEAK_TFLOPS}
+    log("stages: %.0f LM iterations/s; Jacobian %.0f us (%.2e obs/s, %.0f GB/s); dense solve %.0f us "
+        "(%.0f GFLOP/s)" % (out["lm_iterations_per_s"], out["jacobian"]["avg_launch_us"],
+                            out["jacobian"]["obs_per_s"], out["jacobian"]["achieved_GBps"],
+                            out["dense_solve"]["avg_launch_us"], out["dense_solve"]["achieved_GFLOPs"]))
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -374,6 +412,7 @@ def main():
                        "flops_per_launch": flops, "traffic_source": traffic_src}
     log("schur: %d launches, %.1f us avg, %.2f TFLOP/s (%.1f%% of FP64 peak)" % (
         schur_n, avg_s * 1e6, achieved, 100 * achieved / FP64_PEAK_TFLOPS))
+    out["stages"] = stage_rates(P, ctx, solve_on, sc, info, r, dt / a.steps)
     if "mixed" in sections:
         out["mixed"] = mixed_section(P, sc, a, dt / a.steps, rms)
     if "index" in sections:
