@@ -15,6 +15,8 @@
 
 namespace psba {
 
+constexpr int kJacBlock = 128;  // threads per Jacobian workgroup (context launch)
+
 // ---------------------------------------------------------------------------
 // residual and cost
 // ---------------------------------------------------------------------------
@@ -82,49 +84,83 @@ __global__ __launch_bounds__(256) void k_jacobian(const double* __restrict__ K, 
                                                   R* __restrict__ JB, R* __restrict__ W,
                                                   const int* __restrict__ ownr, const unsigned char* __restrict__ run,
                                                   int loss, double loss_d) {
-  int o = blockIdx.x * blockDim.x + threadIdx.x;
-  if (o >= no) return;
-  if (run && !run[ownr[o]]) return;  // batched problems: only those that need it
-  int j = obs_cam[o], i = obs_pt[o];
-  double cam[CNP], M[3], uv[2], ja[2 * CNP], jb[6];
+  // The workgroup's rows of e, J_A, J_B and W go out through LDS: each thread
+  // puts its row in the staging buffer (stride L + 1 against bank
+  // conflicts), then the workgroup writes the block of rows as one contiguous
+  // range, so every store instruction covers consecutive addresses instead of
+  // one element in each of 64 rows.  Rows of batched problems that do not
+  // need a Jacobian (run) are left as they are.
+  extern __shared__ double sJac[];
+  R* const sb = (R*)sJac;
+  unsigned char* const sAct = (unsigned char*)(sJac) + (size_t)blockDim.x * (3 * CNP + 1) * sizeof(R);
+  const int o0 = blockIdx.x * blockDim.x, nrow = min((int)blockDim.x, no - o0);
+  const int o = o0 + threadIdx.x;
+  const bool act = o < no && (!run || run[ownr[o]]);
+  sAct[threadIdx.x] = act;
+  auto put = [&](int L, const R* v) {
+    if (act)
+      for (int k = 0; k < L; ++k) sb[threadIdx.x * (L + 1) + k] = v[k];
+  };
+  auto flush = [&](int L, R* dst) {
+    __syncthreads();
+    for (int f = threadIdx.x; f < nrow * L; f += blockDim.x) {
+      const int rr = f / L;
+      if (sAct[rr]) dst[(size_t)o0 * L + f] = sb[rr * (L + 1) + (f - rr * L)];
+    }
+    __syncthreads();
+  };
+  // every thread reaches the same barriers (a wave holds active and inactive
+  // rows at once); only active threads compute and stage a row
+  R jr[2 * CNP], br[6], er[2], wr[3 * CNP];
+  if (act) {
+    int j = obs_cam[o], i = obs_pt[o];
+    double cam[CNP], M[3], uv[2], ja[2 * CNP], jb[6];
 #pragma unroll
-  for (int k = 0; k < CNP; ++k) cam[k] = cams[j * CNP + k];
+    for (int k = 0; k < CNP; ++k) cam[k] = cams[j * CNP + k];
 #pragma unroll
-  for (int k = 0; k < 3; ++k) M[k] = pts[3 * i + k];
-  project<double, CNP, true>(K + 5 * j, q0 + 4 * j, cam, M, uv, ja, jb);
+    for (int k = 0; k < 3; ++k) M[k] = pts[3 * i + k];
+    project<double, CNP, true>(K + 5 * j, q0 + 4 * j, cam, M, uv, ja, jb);
 #pragma unroll
-  for (int k = 0; k < CNP; ++k)
-    if (fix[j * CNP + k]) ja[k] = ja[CNP + k] = 0.0;
-  double e0 = meas[2 * o] - uv[0], e1 = meas[2 * o + 1] - uv[1];
-  if (loss) {
-    // reweighted rows: residual and Jacobian scaled by sqrt(rho'(s))
-    double rho, drho;
-    loss_eval(loss, loss_d, e0 * e0 + e1 * e1, rho, drho);
-    const double w = sqrt(drho);
+    for (int k = 0; k < CNP; ++k)
+      if (fix[j * CNP + k]) ja[k] = ja[CNP + k] = 0.0;
+    double e0 = meas[2 * o] - uv[0], e1 = meas[2 * o + 1] - uv[1];
+    if (loss) {
+      // reweighted rows: residual and Jacobian scaled by sqrt(rho'(s))
+      double rho, drho;
+      loss_eval(loss, loss_d, e0 * e0 + e1 * e1, rho, drho);
+      const double w = sqrt(drho);
 #pragma unroll
-    for (int k = 0; k < 2 * CNP; ++k) ja[k] *= w;
+      for (int k = 0; k < 2 * CNP; ++k) ja[k] *= w;
 #pragma unroll
-    for (int k = 0; k < 6; ++k) jb[k] *= w;
-    e0 *= w;
-    e1 *= w;
+      for (int k = 0; k < 6; ++k) jb[k] *= w;
+      e0 *= w;
+      e1 *= w;
+    }
+    er[0] = (R)e0;
+    er[1] = (R)e1;
+#pragma unroll
+    for (int k = 0; k < 2 * CNP; ++k) jr[k] = (R)ja[k];
+#pragma unroll
+    for (int k = 0; k < 6; ++k) br[k] = (R)jb[k];
+#pragma unroll
+    for (int r = 0; r < CNP; ++r)
+#pragma unroll
+      for (int cc = 0; cc < 3; ++cc) wr[3 * r + cc] = (R)coeff * (jr[r] * br[cc] + jr[CNP + r] * br[3 + cc]);
   }
-  e[2 * o] = (R)e0;
-  e[2 * o + 1] = (R)e1;
-  R* pa = JA + (size_t)o * 2 * CNP;
-#pragma unroll
-  for (int k = 0; k < 2 * CNP; ++k) pa[k] = (R)ja[k];
-#pragma unroll
-  for (int k = 0; k < 6; ++k) JB[(size_t)o * 6 + k] = (R)jb[k];
-  R* pw = W + (size_t)o * 3 * CNP;
-  R jr[2 * CNP], br[6];
-#pragma unroll
-  for (int k = 0; k < 2 * CNP; ++k) jr[k] = (R)ja[k];
-#pragma unroll
-  for (int k = 0; k < 6; ++k) br[k] = (R)jb[k];
-#pragma unroll
-  for (int r = 0; r < CNP; ++r)
-#pragma unroll
-    for (int cc = 0; cc < 3; ++cc) pw[3 * r + cc] = (R)coeff * (jr[r] * br[cc] + jr[CNP + r] * br[3 + cc]);
+  put(2, er);
+  flush(2, e);
+  put(2 * CNP, jr);
+  flush(2 * CNP, JA);
+  put(6, br);
+  flush(6, JB);
+  put(3 * CNP, wr);
+  flush(3 * CNP, W);
+}
+
+// dynamic LDS of k_jacobian for a block of bs threads
+template <int CNP, typename R>
+constexpr size_t jacobian_lds(int bs) {
+  return (size_t)bs * (3 * CNP + 1) * sizeof(R) + (size_t)bs;
 }
 
 // V_i (6 unique entries), g_b,i and the V part of UVdiag, one thread per point
@@ -267,7 +303,8 @@ __global__ __launch_bounds__(256) void k_cam_finish(const int* __restrict__ ccp,
 template <int CNP, typename R>
 static void run_jac_normal_t(psba_ctx* c, double coeff, double coeff_g, R* e, R* JA, R* JB, R* W, R* ucpart) {
   probe_begin(c, PSBA_PROBE_JACOBIAN);
-  hipLaunchKernelGGL((k_jacobian<CNP, R>), dim3(ceil_div(c->no, 256)), dim3(256), 0, c->stream, c->K, c->q0,
+  hipLaunchKernelGGL((k_jacobian<CNP, R>), dim3(ceil_div(c->no, kJacBlock)), dim3(kJacBlock),
+                     (jacobian_lds<CNP, R>(kJacBlock)), c->stream, c->K, c->q0,
                      c->cams, c->pts, c->meas, c->obs_cam, c->obs_pt, c->fix, c->no, coeff, e, JA, JB, W,
                      (const int*)nullptr, (const unsigned char*)nullptr, c->loss, c->loss_scale);
   probe_end(c, PSBA_PROBE_JACOBIAN);
@@ -855,8 +892,10 @@ namespace psba {
 template <int CNP, typename R>
 static void run_b_jac_t(psba_batch* b) {
   R *e = (R*)b->e, *JA = (R*)b->JA, *JB = (R*)b->JB, *W = (R*)b->W, *ucpart = (R*)b->ucpart;
-  const int bs = b->obs_block;
-  hipLaunchKernelGGL((k_jacobian<CNP, R>), dim3(ceil_div(b->NO, bs)), dim3(bs), 0, b->stream, b->K, b->q0, b->cams,
+  // staging buffer within 64 KB (cnp 16 in FP64 with 256 threads would not be)
+  const int bs = jacobian_lds<CNP, R>(b->obs_block) <= 65536 ? b->obs_block : 128;
+  hipLaunchKernelGGL((k_jacobian<CNP, R>), dim3(ceil_div(b->NO, bs)), dim3(bs), (jacobian_lds<CNP, R>(bs)), b->stream,
+                     b->K, b->q0, b->cams,
                      b->pts, b->meas, b->obs_cam, b->obs_pt, b->fix, (int)b->NO, 1.0, e, JA, JB, W, b->obs_prob,
                      b->need_jac, PSBA_LOSS_NONE, 1.0);
   hipLaunchKernelGGL(k_point_normal<R>, dim3(ceil_div(4L * b->NP, b->pt_block)), dim3(b->pt_block), 0, b->stream, b->pt_ptr, JB, e,
