@@ -1013,30 +1013,92 @@ __device__ __forceinline__ double tile_matvec(CholSmem& sm, const double* Lt, in
   return t;
 }
 
+// LDS of k_tri_solve: the diagonal tile, and the whole solution vector (y in
+// the forward pass, overwritten block by block by x in the backward pass)
+struct TriSmem {
+  double P[NB * TP];
+  double vec[kMaxSolveN];
+  double part[4][NB];
+  double v[NB];
+};
+
 __global__ __launch_bounds__(256) void k_tri_solve(const double* __restrict__ Lf, int ld, int T, int n,
                                                    const double* __restrict__ b, double* __restrict__ y,
                                                    double* __restrict__ x, double* __restrict__ out) {
-  __shared__ CholSmem sm;
-  const int tid = threadIdx.x, lane = tid & 63;
+  __shared__ TriSmem sm;
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
   auto tile = [&](int i, int j) { return Lf + (size_t)i * NB * ld + (size_t)j * NB; };
-  for (int i = 0; i < T; ++i) {
-    const int q = i * NB + lane;
-    double br = (q < n) ? b[q] : 0.0;
-    for (int k = 0; k < i; ++k) br -= tile_matvec(sm, tile(i, k), ld, y + (size_t)k * NB);
-    load_lower(sm.P, tile(i, i), ld);
-    if (tid < NB) sm.v[tid] = br;
+  // fixed-order total of the four waves' partial sums for lane `lane`
+  auto total = [&](double s) {
+    sm.part[w][lane] = s;
     __syncthreads();
-    tile_forward(LDS(sm.P), LDS(sm.v), y + (size_t)i * NB, LDS(sm.w));
+    const double t = ((sm.part[0][lane] + sm.part[1][lane]) + sm.part[2][lane]) + sm.part[3][lane];
+    __syncthreads();
+    return t;
+  };
+  // forward: y_i = L_ii^-1 (b_i - sum_{k<i} L_ik y_k).  Thread (lane, w) takes
+  // row `lane`, columns 16w .. 16w + 15 of every tile; four tiles' loads are
+  // in flight together (the solve is bound by their latency, not bandwidth).
+  for (int i = 0; i < T; ++i) {
+    const double* Li = tile(i, 0) + (size_t)lane * ld + 16 * w;
+    double s = 0.0;
+    int k = 0;
+    for (; k + 4 <= i; k += 4) {
+      double l[4][16];
+#pragma unroll
+      for (int u = 0; u < 4; ++u)
+#pragma unroll
+        for (int c = 0; c < 16; ++c) l[u][c] = Li[(k + u) * NB + c];
+#pragma unroll
+      for (int u = 0; u < 4; ++u)
+#pragma unroll
+        for (int c = 0; c < 16; ++c) s += l[u][c] * sm.vec[(k + u) * NB + 16 * w + c];
+    }
+    for (; k < i; ++k) {
+      double l[16];
+#pragma unroll
+      for (int c = 0; c < 16; ++c) l[c] = Li[k * NB + c];
+#pragma unroll
+      for (int c = 0; c < 16; ++c) s += l[c] * sm.vec[k * NB + 16 * w + c];
+    }
+    const double t = total(s);
+    const int q = i * NB + lane;
+    load_lower(sm.P, tile(i, i), ld);
+    if (tid < NB) sm.v[tid] = ((q < n) ? b[q] : 0.0) - t;
+    __syncthreads();
+    tile_forward(LDS(sm.P), LDS(sm.v), y + (size_t)i * NB, LDS(sm.vec) + (size_t)i * NB);
     __syncthreads();
   }
+  // backward: x_i = L_ii^-T (y_i - sum_{k>i} L_ki^T x_k); thread (lane, w) takes
+  // column `lane`, rows w, w + 4, .. of every tile
   for (int i = T - 1; i >= 0; --i) {
-    double br = y[(size_t)i * NB + lane];
-    for (int k = i + 1; k < T; ++k) br -= tile_tmatvec(sm, tile(k, i), ld, x + (size_t)k * NB);
+    double s = 0.0;
+    int k = i + 1;
+    for (; k + 4 <= T; k += 4) {
+      double l[4][16];
+#pragma unroll
+      for (int u = 0; u < 4; ++u)
+#pragma unroll
+        for (int r = 0; r < 16; ++r) l[u][r] = tile(k + u, i)[(size_t)(w + 4 * r) * ld + lane];
+#pragma unroll
+      for (int u = 0; u < 4; ++u)
+#pragma unroll
+        for (int r = 0; r < 16; ++r) s += l[u][r] * sm.vec[(k + u) * NB + w + 4 * r];
+    }
+    for (; k < T; ++k) {
+      double l[16];
+#pragma unroll
+      for (int r = 0; r < 16; ++r) l[r] = tile(k, i)[(size_t)(w + 4 * r) * ld + lane];
+#pragma unroll
+      for (int r = 0; r < 16; ++r) s += l[r] * sm.vec[k * NB + w + 4 * r];
+    }
+    const double t = total(s);
+    const double br = sm.vec[i * NB + lane] - t;
     load_lower(sm.P, tile(i, i), ld);
     __syncthreads();
-    tile_backward(LDS(sm.P), br, x + (size_t)i * NB, LDS(sm.v));
+    tile_backward(LDS(sm.P), br, x + (size_t)i * NB, LDS(sm.vec) + (size_t)i * NB);
     __syncthreads();
-    if (tid < NB && i * NB + tid < n) out[i * NB + tid] = sm.v[tid];
+    if (tid < NB && i * NB + tid < n) out[i * NB + tid] = sm.vec[i * NB + tid];
   }
 }
 

@@ -735,18 +735,37 @@ __global__ __launch_bounds__(256) void k_refine_obs(const double* __restrict__ K
   }
 }
 
-// per camera chunk: sums of rA and gA over the chunk's observations (in order)
+// per camera chunk: sums of rA and gA over the chunk's observations.  Output
+// t (2 cnp of them) is summed by eight threads p = 0..7 over observations p,
+// p + 8, .. (their loads independent, so many are in flight), and the eight
+// partial sums are added in p order.
 template <int CNP>
-__global__ __launch_bounds__(64) void k_refine_cam_chunks(const int* __restrict__ cam_obs, const int* __restrict__ cb,
-                                                          const int* __restrict__ ce, const double* __restrict__ rA,
-                                                          const double* __restrict__ gA, double* __restrict__ part) {
-  const int b = blockIdx.x, t = threadIdx.x;
-  if (t >= 2 * CNP) return;
-  const double* src = t < CNP ? rA : gA;
-  const int k = t < CNP ? t : t - CNP;
-  double acc = 0.0;
-  for (int s = cb[b]; s < ce[b]; ++s) acc += src[(size_t)cam_obs[s] * CNP + k];
-  part[(size_t)b * 2 * CNP + t] = acc;
+__global__ __launch_bounds__(256) void k_refine_cam_chunks(const int* __restrict__ cam_obs, const int* __restrict__ cb,
+                                                           const int* __restrict__ ce, const double* __restrict__ rA,
+                                                           const double* __restrict__ gA, double* __restrict__ part) {
+  static_assert(8 * 2 * CNP <= 256, "eight threads per output");
+  __shared__ int sO[kCamChunk];
+  __shared__ double sP[8][2 * CNP];
+  const int b = blockIdx.x, tid = threadIdx.x;
+  const int beg = cb[b], cnt = ce[b] - beg;
+  for (int s = tid; s < cnt; s += blockDim.x) sO[s] = cam_obs[beg + s];
+  __syncthreads();
+  if (tid < 8 * 2 * CNP) {
+    const int p = tid / (2 * CNP), t = tid - p * (2 * CNP);
+    const double* src = t < CNP ? rA : gA;
+    const int k = t < CNP ? t : t - CNP;
+    double acc = 0.0;
+#pragma unroll 4
+    for (int s = p; s < cnt; s += 8) acc += src[(size_t)sO[s] * CNP + k];
+    sP[p][t] = acc;
+  }
+  __syncthreads();
+  if (tid < 2 * CNP) {
+    double a = sP[0][tid];
+#pragma unroll
+    for (int p = 1; p < 8; ++p) a += sP[p][tid];
+    part[(size_t)b * 2 * CNP + tid] = a;
+  }
 }
 
 template <int CNP>
@@ -804,7 +823,7 @@ static void refine_pass(psba_ctx* c, double mu, int diag, double* r, double* z) 
                      c->pts, c->meas, c->obs_cam, c->obs_pt, c->fix, c->no, c->n, c->dp, c->loss, c->loss_scale, rA, rB,
                      gA, gB, diag);
   if (c->n_cchunk > 0)
-    hipLaunchKernelGGL(k_refine_cam_chunks<CNP>, dim3(c->n_cchunk), dim3(64), 0, c->stream, c->cam_obs,
+    hipLaunchKernelGGL(k_refine_cam_chunks<CNP>, dim3(c->n_cchunk), dim3(256), 0, c->stream, c->cam_obs,
                        c->cchunk_begin, c->cchunk_end, rA, gA, c->ref_part);
   hipLaunchKernelGGL(k_refine_cam_finish<CNP>, dim3(ceil_div((long)c->nc * CNP, 64)), dim3(64), 0, c->stream,
                      c->cam_cchunk_ptr, c->ref_part, c->nc, mu, c->dp, r, c->g);
