@@ -81,6 +81,7 @@ struct psba_ctx {
   int n_lblocks = 0;            // nc*(nc+1)/2 lower block positions
   int* lblock_pair = nullptr;   // pair index of each lower block position or -1
   int* schunk_order = nullptr;  // launch order of the pair chunks (schunk_launch_key)
+  int* pt_order = nullptr;      // points by descending track length (ties: ascending id)
   double sum_t2 = 0;            // sum over points of track length squared
 
   bool mixed() const { return precision == PSBA_PREC_MIXED; }
@@ -150,6 +151,7 @@ int launch_axpby(psba_ctx* c, double* out, double alpha, const double* a, double
 
 // ---- launchers (k_schur.hip) ----
 int launch_schur(psba_ctx* c, double mu);                 // S (lower), ea
+int build_point_order(psba_ctx* c);                        // pt_order (index builds)
 int launch_ea_rhs(psba_ctx* c, const double* ga, const double* z, double* out);  // ga - sum W z
 
 // ---- launchers (k_dense.hip) ----

@@ -67,6 +67,7 @@ static void free_index(psba_ctx* c) {
   dfree(c->schunk_end);
   dfree(c->pair_schunk_ptr);
   dfree(c->schunk_order);
+  dfree(c->pt_order);
   dfree(c->lblock_pair);
   dfree(c->ucpart);
   dfree(c->eapart);
@@ -429,6 +430,7 @@ int alloc_index(psba_ctx* c, long nitems, int npairs, int n_cchunk, int n_schunk
   PSBA_TRY(dalloc(&c->schunk_end, n_schunk));
   PSBA_TRY(dalloc(&c->pair_schunk_ptr, npairs + 1));
   PSBA_TRY(dalloc(&c->schunk_order, n_schunk));
+  PSBA_TRY(dalloc(&c->pt_order, np));
   PSBA_TRY(dalloc(&c->lblock_pair, nlb));
   if (c->mixed()) {
     PSBA_TRY(dalloc(&c->ucpart32, (size_t)n_cchunk * NE));
@@ -553,6 +555,7 @@ int psba_set_index(psba_ctx* c, const int* iidx, const int* jidx) {
   PSBA_TRY(up(c->pair_schunk_ptr, psp.data(), sizeof(int) * (npairs + 1)));
   PSBA_TRY(up(c->lblock_pair, lbp.data(), sizeof(int) * nlb));
   PSBA_TRY(up(c->schunk_order, sord.data(), sizeof(int) * sord.size()));
+  PSBA_TRY(build_point_order(c));
   PSBA_HIP_TRY(hipStreamSynchronize(c->stream));
   c->indexed = true;
   return PSBA_OK;

@@ -365,9 +365,48 @@ int psba_set_index_device(psba_ctx* c, const int* iidx, const int* jidx) {
   }
   IX_TRY(hipGetLastError());
   IX_TRY(hipStreamSynchronize(st));  // the temporaries are freed on return
+  PSBA_TRY(build_point_order(c));
   c->indexed = true;
   return PSBA_OK;
 }
+
+}  // extern "C"
+
+namespace psba {
+
+__global__ void k_point_order_keys(const int* __restrict__ pt_ptr, int np, unsigned* __restrict__ key,
+                                   int* __restrict__ val) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= np) return;
+  key[i] = ~(unsigned)(pt_ptr[i + 1] - pt_ptr[i]);  // ascending key = descending length
+  val[i] = i;
+}
+
+// Points by descending track length, ties in ascending id (a stable radix
+// sort).  The back substitution takes its points in this order, so that the
+// points one wave handles in lockstep have tracks of similar length.
+int build_point_order(psba_ctx* c) {
+  const int np = c->np;
+  hipStream_t st = c->stream;
+  DevBuf<unsigned> key, key_s;
+  DevBuf<int> val;
+  PSBA_TRY(key.alloc(np));
+  PSBA_TRY(key_s.alloc(np));
+  PSBA_TRY(val.alloc(np));
+  hipLaunchKernelGGL(k_point_order_keys, dim3((np + 255) / 256), dim3(256), 0, st, c->pt_ptr, np, key.p, val.p);
+  size_t tb = 0;
+  IX_TRY(rocprim::radix_sort_pairs(nullptr, tb, key.p, key_s.p, val.p, c->pt_order, (size_t)np, 0, 32, st));
+  Scratch tmp;
+  PSBA_TRY(tmp.ensure(tb));
+  IX_TRY(rocprim::radix_sort_pairs(tmp.p, tb, key.p, key_s.p, val.p, c->pt_order, (size_t)np, 0, 32, st));
+  IX_TRY(hipGetLastError());
+  IX_TRY(hipStreamSynchronize(st));  // the temporaries are freed on return
+  return PSBA_OK;
+}
+
+}  // namespace psba
+
+extern "C" {
 
 int psba_generate_idxs_device(int device, int nCams, int n3Dpts, int n2Dprojs, const char* vmask,
                               int* comm3DIdxCnt, int* iidx, int* jidx, int* blk_idx) {

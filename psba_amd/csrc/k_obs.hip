@@ -442,7 +442,8 @@ __global__ __launch_bounds__(256) void k_backsub(const int* __restrict__ pt_ptr,
                                                  const R* __restrict__ W, const double* __restrict__ Vinv,
                                                  const double* __restrict__ gb, int np, int n,
                                                  double* __restrict__ dp, double* __restrict__ eb,
-                                                 const int* __restrict__ ownr, const unsigned char* __restrict__ run) {
+                                                 const int* __restrict__ ownr, const unsigned char* __restrict__ run,
+                                                 const int* __restrict__ order) {
   extern __shared__ double sDa[];
   if constexpr (LDS_DPA) {
     for (int q = threadIdx.x; q < n; q += blockDim.x) sDa[q] = dp[q];
@@ -451,8 +452,13 @@ __global__ __launch_bounds__(256) void k_backsub(const int* __restrict__ pt_ptr,
   const int lane = threadIdx.x & 63;
   const int grp = lane / 3;
   const int cc = lane - 3 * grp;
-  const int i = (blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6)) * kBacksubPtsPerWave + grp;
-  const bool ok = grp < kBacksubPtsPerWave && i < np && (!run || run[ownr[i]]);
+  // slot -> point: with `order` (points by descending track length) the
+  // points of a wave, which step through their observations together, have
+  // tracks of similar length
+  const int slot = (blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6)) * kBacksubPtsPerWave + grp;
+  const bool in = grp < kBacksubPtsPerWave && slot < np;
+  const int i = (in && order) ? order[slot] : slot;
+  const bool ok = in && (!run || run[ownr[i]]);
   R s = 0;
   if (ok) {
     // camera indices are fetched two observations ahead, so each step waits
@@ -491,22 +497,23 @@ __global__ __launch_bounds__(256) void k_backsub(const int* __restrict__ pt_ptr,
 template <int CNP, typename R>
 static void backsub_launch(hipStream_t st, const int* pt_ptr, const int* obs_cam, const R* W, const double* Vinv,
                            const double* gb, int np, int n, double* dp, double* eb, const int* pt_ownr,
-                           const unsigned char* run, int block) {
+                           const unsigned char* run, int block, const int* order = nullptr) {
   const int per_block = (block / 64) * kBacksubPtsPerWave;
   if (n <= kBacksubLdsDpa)
     hipLaunchKernelGGL((k_backsub<CNP, R, true>), dim3(ceil_div(np, per_block)), dim3(block),
-                       sizeof(double) * (size_t)n, st, pt_ptr, obs_cam, W, Vinv, gb, np, n, dp, eb, pt_ownr, run);
+                       sizeof(double) * (size_t)n, st, pt_ptr, obs_cam, W, Vinv, gb, np, n, dp, eb, pt_ownr, run,
+                       order);
   else
     hipLaunchKernelGGL((k_backsub<CNP, R, false>), dim3(ceil_div(np, per_block)), dim3(block), 0, st, pt_ptr,
-                       obs_cam, W, Vinv, gb, np, n, dp, eb, pt_ownr, run);
+                       obs_cam, W, Vinv, gb, np, n, dp, eb, pt_ownr, run, order);
 }
 
 template <int CNP>
 static void run_backsub(psba_ctx* c) {
   if (c->mixed())
-    backsub_launch<CNP, float>(c->stream, c->pt_ptr, c->obs_cam, c->W32, c->Vinv, c->g + c->n, c->np, c->n, c->dp, c->eb, (const int*)nullptr, (const unsigned char*)nullptr, 256);
+    backsub_launch<CNP, float>(c->stream, c->pt_ptr, c->obs_cam, c->W32, c->Vinv, c->g + c->n, c->np, c->n, c->dp, c->eb, (const int*)nullptr, (const unsigned char*)nullptr, 256, c->pt_order);
   else
-    backsub_launch<CNP, double>(c->stream, c->pt_ptr, c->obs_cam, c->W, c->Vinv, c->g + c->n, c->np, c->n, c->dp, c->eb, (const int*)nullptr, (const unsigned char*)nullptr, 256);
+    backsub_launch<CNP, double>(c->stream, c->pt_ptr, c->obs_cam, c->W, c->Vinv, c->g + c->n, c->np, c->n, c->dp, c->eb, (const int*)nullptr, (const unsigned char*)nullptr, 256, c->pt_order);
 }
 
 int launch_backsub(psba_ctx* c) {
@@ -887,9 +894,9 @@ int launch_refine(psba_ctx* c, double mu, int sweeps) {
 template <int CNP>
 static void run_backsub_rhs(psba_ctx* c, const double* gb, double* dp_out) {
   if (c->mixed())
-    backsub_launch<CNP, float>(c->stream, c->pt_ptr, c->obs_cam, c->W32, c->Vinv, gb, c->np, c->n, dp_out, c->eb, (const int*)nullptr, (const unsigned char*)nullptr, 256);
+    backsub_launch<CNP, float>(c->stream, c->pt_ptr, c->obs_cam, c->W32, c->Vinv, gb, c->np, c->n, dp_out, c->eb, (const int*)nullptr, (const unsigned char*)nullptr, 256, c->pt_order);
   else
-    backsub_launch<CNP, double>(c->stream, c->pt_ptr, c->obs_cam, c->W, c->Vinv, gb, c->np, c->n, dp_out, c->eb, (const int*)nullptr, (const unsigned char*)nullptr, 256);
+    backsub_launch<CNP, double>(c->stream, c->pt_ptr, c->obs_cam, c->W, c->Vinv, gb, c->np, c->n, dp_out, c->eb, (const int*)nullptr, (const unsigned char*)nullptr, 256, c->pt_order);
 }
 
 int launch_backsub_rhs(psba_ctx* c, const double* gb, double* dp_out) {
