@@ -624,7 +624,9 @@ PSBA_OUTLINE int tile_potrf(lds_smem& sm, lds_dbl* Yt, bool write_rd) {
       const int row = b0 + 16 + threadIdx.x;
       // right-looking: column c is scaled, then every later column of the row
       // takes its term (a_m - x_0 L[m][0] - x_1 L[m][1] - ..., ascending)
-      double x[16], Lc[16], rdv[16];
+      // column c + 1 of the diagonal block is read from LDS before column c's
+      // updates, so the read latency overlaps them
+      double x[16], Lc[16], Ln[16], rdv[16];
 #pragma unroll
       for (int c = 0; c < 16; ++c) {
         x[c] = Yt[row * TP + b0 + c];
@@ -633,13 +635,15 @@ PSBA_OUTLINE int tile_potrf(lds_smem& sm, lds_dbl* Yt, bool write_rd) {
       }
 #pragma unroll
       for (int c = 0; c < 16; ++c) {
+        if (c + 1 < 16) {
+#pragma unroll
+          for (int m = c + 2; m < 16; ++m) Ln[m] = Yt[(b0 + m) * TP + b0 + c + 1];
+        }
         x[c] = x[c] * rdv[c];
 #pragma unroll
         for (int m = c + 1; m < 16; ++m) x[m] = fma(-x[c], Lc[m], x[m]);
-        if (c + 1 < 16) {
 #pragma unroll
-          for (int m = c + 2; m < 16; ++m) Lc[m] = Yt[(b0 + m) * TP + b0 + c + 1];
-        }
+        for (int m = c + 2; m < 16; ++m) Lc[m] = Ln[m];
 #pragma unroll
         for (int m = c; m < 16; ++m) __asm__ volatile("" : "+v"(x[m]));
       }
