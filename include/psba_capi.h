@@ -226,6 +226,7 @@ int psba_read_buffer(psba_ctx* ctx, int which, double* out, long n);
 #define PSBA_IDX_PAIR_SCHUNK_PTR 13
 #define PSBA_IDX_LBLOCK_PAIR 14   /* n_cams(n_cams+1)/2 */
 #define PSBA_IDX_SCHUNK_ORDER 15  /* launch order of the pair chunks (n_schunk) */
+#define PSBA_IDX_PT_ORDER 16      /* points by descending track length, ties ascending id (n_pts) */
 long psba_index_size(const psba_ctx* ctx, int which);
 int psba_read_index(psba_ctx* ctx, int which, int* out, long n);
 

@@ -35,7 +35,7 @@ LOSS_NONE, LOSS_HUBER, LOSS_CAUCHY = 0, 1, 2
 BUF = dict(EX=0, JA=1, JB=2, U=3, V=4, W=5, G=6, VINV=7, S=9, EA=10, DP=11, EB=12, L=13)
 INDEX = dict(OBS_PT=0, OBS_CAM=1, PT_PTR=2, CAM_PTR=3, CAM_OBS=4, CCHUNK_BEGIN=5, CCHUNK_END=6, CAM_CCHUNK_PTR=7,
              PAIR_ITEMS=8, PAIR_K=9, PAIR_L=10, SCHUNK_BEGIN=11, SCHUNK_END=12, PAIR_SCHUNK_PTR=13, LBLOCK_PAIR=14,
-             SCHUNK_ORDER=15)
+             SCHUNK_ORDER=15, PT_ORDER=16)
 
 
 class PsbaError(RuntimeError):

@@ -765,6 +765,7 @@ static const int* index_ptr(const psba_ctx* c, int which, long* n) {
     case PSBA_IDX_PAIR_SCHUNK_PTR: *n = c->npairs + 1; return c->pair_schunk_ptr;
     case PSBA_IDX_LBLOCK_PAIR: *n = c->n_lblocks; return c->lblock_pair;
     case PSBA_IDX_SCHUNK_ORDER: *n = c->n_schunk; return c->schunk_order;
+    case PSBA_IDX_PT_ORDER: *n = c->np; return c->pt_order;
     default: *n = -1; return nullptr;
   }
 }

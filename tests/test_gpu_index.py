@@ -43,7 +43,7 @@ def same_index(a, b):
 
 P_INDEX = ["OBS_PT", "OBS_CAM", "PT_PTR", "CAM_PTR", "CAM_OBS", "CCHUNK_BEGIN", "CCHUNK_END", "CAM_CCHUNK_PTR",
            "PAIR_ITEMS", "PAIR_K", "PAIR_L", "SCHUNK_BEGIN", "SCHUNK_END", "PAIR_SCHUNK_PTR", "LBLOCK_PAIR",
-           "SCHUNK_ORDER"]
+           "SCHUNK_ORDER", "PT_ORDER"]
 
 
 @pytest.mark.parametrize("config,kw", SCENES, ids=IDS)
@@ -61,6 +61,18 @@ def test_device_index_bit_exact(P, gpu, config, kw):
     ca, pa = a.read_params()
     cb, pb = b.read_params()
     assert np.array_equal(ca, cb) and np.array_equal(pa, pb)
+
+
+@pytest.mark.parametrize("config,kw", SCENES, ids=IDS)
+def test_point_order(P, gpu, config, kw):
+    """The back substitution's point order: a permutation of the points by
+    descending track length, ties in ascending point id (numpy's stable sort
+    of the negated lengths gives the same sequence)."""
+    sc = P.Scene(config, 2100 + config, **kw)
+    a, _ = build_pair(P, sc, sc.iidx, sc.jidx)
+    ptr = a.read_index("PT_PTR")
+    want = np.argsort(-np.diff(ptr), kind="stable")
+    assert np.array_equal(a.read_index("PT_ORDER"), want)
 
 
 def test_device_index_ragged(P, gpu):
