@@ -680,68 +680,89 @@ __global__ __launch_bounds__(256) void k_refine_obs(const double* __restrict__ K
                                                     const double* __restrict__ dp, int loss, double loss_d,
                                                     double* __restrict__ rA, double* __restrict__ rB,
                                                     double* __restrict__ gA, double* __restrict__ gB, int diag) {
-  const int o = blockIdx.x * blockDim.x + threadIdx.x;
-  if (o >= no) return;
-  const int j = obs_cam[o], i = obs_pt[o];
-  double cam[CNP], M[3], uv[2], ja[2 * CNP], jb[6];
-#pragma unroll
-  for (int k = 0; k < CNP; ++k) cam[k] = cams[j * CNP + k];
-#pragma unroll
-  for (int k = 0; k < 3; ++k) M[k] = pts[3 * i + k];
-  project<double, CNP, true>(K + 5 * j, q0 + 4 * j, cam, M, uv, ja, jb);
-#pragma unroll
-  for (int k = 0; k < CNP; ++k)
-    if (fix[j * CNP + k]) ja[k] = ja[CNP + k] = 0.0;
-  double e0 = meas[2 * o] - uv[0], e1 = meas[2 * o + 1] - uv[1];
-  if (loss) {
-    double rho, drho;
-    loss_eval(loss, loss_d, e0 * e0 + e1 * e1, rho, drho);
-    const double w = sqrt(drho);
-#pragma unroll
-    for (int k = 0; k < 2 * CNP; ++k) ja[k] *= w;
-#pragma unroll
-    for (int k = 0; k < 6; ++k) jb[k] *= w;
-    e0 *= w;
-    e1 *= w;
-  }
-  if (diag) {
-    // diagonal of J^T J (for mu0) in place of r; g as usual
-#pragma unroll
-    for (int k = 0; k < CNP; ++k) {
-      rA[(size_t)o * CNP + k] = ja[k] * ja[k] + ja[CNP + k] * ja[CNP + k];
-      gA[(size_t)o * CNP + k] = ja[k] * e0 + ja[CNP + k] * e1;
+  // rows go out through LDS as contiguous blocks (as in k_jacobian); every
+  // thread reaches the same barriers, only rows o < no are computed
+  extern __shared__ double sRef[];
+  const int o0 = blockIdx.x * blockDim.x, nrow = min((int)blockDim.x, no - o0);
+  const int o = o0 + threadIdx.x;
+  const bool act = o < no;
+  auto stage = [&](int L, const double* v, double* dst) {
+    if (act)
+      for (int k = 0; k < L; ++k) sRef[threadIdx.x * (L + 1) + k] = v[k];
+    __syncthreads();
+    for (int f = threadIdx.x; f < nrow * L; f += blockDim.x) {
+      const int rr = f / L;
+      dst[(size_t)o0 * L + f] = sRef[rr * (L + 1) + (f - rr * L)];
     }
+    __syncthreads();
+  };
+  double ra[CNP], ga[CNP], rb[3], gbv[3];
+  if (act) {
+    const int j = obs_cam[o], i = obs_pt[o];
+    double cam[CNP], M[3], uv[2], ja[2 * CNP], jb[6];
 #pragma unroll
-    for (int k = 0; k < 3; ++k) {
-      rB[(size_t)o * 3 + k] = jb[k] * jb[k] + jb[3 + k] * jb[3 + k];
-      gB[(size_t)o * 3 + k] = jb[k] * e0 + jb[3 + k] * e1;
+    for (int k = 0; k < CNP; ++k) cam[k] = cams[j * CNP + k];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) M[k] = pts[3 * i + k];
+    project<double, CNP, true>(K + 5 * j, q0 + 4 * j, cam, M, uv, ja, jb);
+#pragma unroll
+    for (int k = 0; k < CNP; ++k)
+      if (fix[j * CNP + k]) ja[k] = ja[CNP + k] = 0.0;
+    double e0 = meas[2 * o] - uv[0], e1 = meas[2 * o + 1] - uv[1];
+    if (loss) {
+      double rho, drho;
+      loss_eval(loss, loss_d, e0 * e0 + e1 * e1, rho, drho);
+      const double w = sqrt(drho);
+#pragma unroll
+      for (int k = 0; k < 2 * CNP; ++k) ja[k] *= w;
+#pragma unroll
+      for (int k = 0; k < 6; ++k) jb[k] *= w;
+      e0 *= w;
+      e1 *= w;
     }
-    return;
-  }
-  double t0 = 0.0, t1 = 0.0;
+    if (diag) {
+      // diagonal of J^T J (for mu0) in place of r; g as usual
 #pragma unroll
-  for (int k = 0; k < CNP; ++k) {
-    const double d = dp[j * CNP + k];
-    t0 += ja[k] * d;
-    t1 += ja[CNP + k] * d;
-  }
+      for (int k = 0; k < CNP; ++k) {
+        ra[k] = ja[k] * ja[k] + ja[CNP + k] * ja[CNP + k];
+        ga[k] = ja[k] * e0 + ja[CNP + k] * e1;
+      }
 #pragma unroll
-  for (int k = 0; k < 3; ++k) {
-    const double d = dp[n + 3 * i + k];
-    t0 += jb[k] * d;
-    t1 += jb[3 + k] * d;
-  }
-  const double u0 = e0 - t0, u1 = e1 - t1;
+      for (int k = 0; k < 3; ++k) {
+        rb[k] = jb[k] * jb[k] + jb[3 + k] * jb[3 + k];
+        gbv[k] = jb[k] * e0 + jb[3 + k] * e1;
+      }
+    } else {
+      double t0 = 0.0, t1 = 0.0;
 #pragma unroll
-  for (int k = 0; k < CNP; ++k) {
-    rA[(size_t)o * CNP + k] = ja[k] * u0 + ja[CNP + k] * u1;
-    gA[(size_t)o * CNP + k] = ja[k] * e0 + ja[CNP + k] * e1;
-  }
+      for (int k = 0; k < CNP; ++k) {
+        const double d = dp[j * CNP + k];
+        t0 += ja[k] * d;
+        t1 += ja[CNP + k] * d;
+      }
 #pragma unroll
-  for (int k = 0; k < 3; ++k) {
-    rB[(size_t)o * 3 + k] = jb[k] * u0 + jb[3 + k] * u1;
-    gB[(size_t)o * 3 + k] = jb[k] * e0 + jb[3 + k] * e1;
+      for (int k = 0; k < 3; ++k) {
+        const double d = dp[n + 3 * i + k];
+        t0 += jb[k] * d;
+        t1 += jb[3 + k] * d;
+      }
+      const double u0 = e0 - t0, u1 = e1 - t1;
+#pragma unroll
+      for (int k = 0; k < CNP; ++k) {
+        ra[k] = ja[k] * u0 + ja[CNP + k] * u1;
+        ga[k] = ja[k] * e0 + ja[CNP + k] * e1;
+      }
+#pragma unroll
+      for (int k = 0; k < 3; ++k) {
+        rb[k] = jb[k] * u0 + jb[3 + k] * u1;
+        gbv[k] = jb[k] * e0 + jb[3 + k] * e1;
+      }
+    }
   }
+  stage(CNP, ra, rA);
+  stage(CNP, ga, gA);
+  stage(3, rb, rB);
+  stage(3, gbv, gB);
 }
 
 // per camera chunk: sums of rA and gA over the chunk's observations.  Output
@@ -828,7 +849,8 @@ static void refine_pass(psba_ctx* c, double mu, int diag, double* r, double* z) 
   double* gA = rA + (size_t)C * c->no;
   double* rB = gA + (size_t)C * c->no;
   double* gB = rB + 3 * (size_t)c->no;
-  hipLaunchKernelGGL(k_refine_obs<CNP>, dim3(ceil_div(c->no, 256)), dim3(256), 0, c->stream, c->K, c->q0, c->cams,
+  hipLaunchKernelGGL(k_refine_obs<CNP>, dim3(ceil_div(c->no, 256)), dim3(256), sizeof(double) * 256 * (CNP + 1),
+                     c->stream, c->K, c->q0, c->cams,
                      c->pts, c->meas, c->obs_cam, c->obs_pt, c->fix, c->no, c->n, c->dp, c->loss, c->loss_scale, rA, rB,
                      gA, gB, diag);
   if (c->n_cchunk > 0)
