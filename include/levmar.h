@@ -9,3 +9,8 @@
 
 int levmar(PSBA_structPtr psbaPtr, int cnp, int pnp, int mnp, int n3Dpts, int nCams, int n2Dprojs,
            dtype* finalErr);
+
+/* Sum of squared residuals at the entry of the last levmar call
+ * (PSBA/levmar.cpp:16,95).  The application may define it, as PSBA/main.cpp:24
+ * does; the library's weak definition is used otherwise. */
+extern dtype initErr;

@@ -133,6 +133,9 @@ int psba_generate_idxs_device(int device, int nCams, int n3Dpts, int n2Dprojs, c
  * (< 0).  trace may be NULL. */
 int psba_levmar(psba_ctx* ctx, const psba_opts* opts, int* itno, double* finalErr,
                 psba_trace_rec* trace, int trace_cap, int* trace_len);
+/* the sum of squared residuals at the entry of the last psba_levmar call on
+ * ctx: the value PSBA/levmar.cpp:95 stores in the global initErr */
+int psba_levmar_entry_cost(psba_ctx* ctx, double* sse);
 /* replaces trust_region, PSBA/trust_region.h:5-11 (its blk_idx argument is
  * unused by the reference and absent here) */
 int psba_trust_region(psba_ctx* ctx, const psba_opts* opts, int* itno, double* finalErr,
@@ -187,6 +190,11 @@ int psba_stage_modchol(psba_ctx* ctx, double* lambda);
  * with the existing blocks and factor, dp += correction; g is replaced by the
  * FP64 J^T e.  No reference counterpart (the reference is FP64 throughout). */
 int psba_stage_refine(psba_ctx* ctx, double mu, int sweeps);
+/* test hook: the damped 3x3 inverses of compute_Vinv (PSBA/CL_files/
+ * compute_Vinv.cl:6-90, including its small-determinant branch :31-73) on
+ * n_pts caller-given blocks V9 (9 per point, row-major, symmetric part used);
+ * replaces the context's V and writes the inverses to out9 (9 per point) */
+int psba_stage_vinv(psba_ctx* ctx, const double* V9, double mu, double* out9);
 /* test hook: load an n x n matrix (n = cnp*n_cams, row-major, lower triangle
  * used) as S and, if b != NULL, b as e_a, so the dense stages can be checked
  * on arbitrary matrices */

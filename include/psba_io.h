@@ -20,7 +20,10 @@
  * Outputs follow the reference: motstruct = cameras (n_cams*cnp) then points
  * (n_pts*pnp); initrot = unit quaternion per camera rebuilt from the vector
  * part at cams[cnp-6..cnp-4] (PSBA/readparams.cpp:237-241); imgpts in file
- * order; covimgpts full mnp*mnp per projection or NULL; vmask[pt*n_cams+cam].
+ * order with each point's projections in ascending frame order (the order
+ * generate_idxs numbers them in, PSBA/misc.cpp:189-216; the reference keeps
+ * file order, PSBA/readparams.cpp:364-415, which disagrees for unsorted frame
+ * lists); covimgpts full mnp*mnp per projection or NULL; vmask[pt*n_cams+cam].
  */
 #ifndef PSBA_IO_H
 #define PSBA_IO_H
@@ -50,6 +53,19 @@ int psba_read_sba(const char* camsfname, const char* ptsfname, int cnp, int pnp,
                   int filecnp, psba_sba_data* out);
 
 void psba_sba_free(psba_sba_data* d);
+
+/* Writes solved parameters in the format psba_read_sba reads (no reference
+ * counterpart: PSBA/readparams.h:13-25 leaves its printSBA* writers commented
+ * out).  cams: cnp solver parameters per camera [intrinsics..., v, t] with v
+ * the local rotation relative to initrot (4 per camera), as the drivers leave
+ * them; each camera line holds K's 5 values (when K != NULL), the leading
+ * intrinsics, the whole rotation q_v (x) initrot as a unit quaternion with
+ * non-negative scalar, and t.  pts: 3 per point.  The projections (and
+ * covariances) come from obs, a reader result for the same problem.  Reading
+ * the files back with the quat2vec filter and filecnp = cnp + 1 (+5 with K)
+ * reproduces the parameters' projections.  Returns PSBA_OK or PSBA_ERR_ARG. */
+int psba_write_sba(const char* camsfname, const char* ptsfname, int n_cams, int n_pts, int cnp, const double* K,
+                   const double* initrot, const double* cams, const double* pts, const psba_sba_data* obs);
 
 /* PSBA/misc.cpp:21-49 (quat2vec): the camera-line filter the reference passes
  * to its reader; copies the leading nin-7 values, replaces the unit quaternion

@@ -86,6 +86,9 @@ def lib():
         "orc_dense_modchol": (ct.c_double, [ct.c_int, dp, dp, dp]),
         "orc_set_loss": (ct.c_int, [vp, ct.c_int, ct.c_double]),
         "orc_loss_eval": (None, [ct.c_int, ct.c_double, ct.c_double, dp, dp]),
+        "orc_vinv3": (ct.c_int, [dp, ct.c_double, dp]),
+        "orc_dogleg_step": (ct.c_double, [ct.c_int, ct.c_double, ct.c_double, ct.c_double, ct.c_double, dp, dp,
+                                          dp, dp]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -128,6 +131,24 @@ def dense_factor_solve(S, b, solve_mode=SOLVE_CHOLESKY):
     L = np.zeros((n, n))
     ok = lib().orc_dense_factor_solve(n, solve_mode, _d(Sm), _d(bb), _d(x), _d(L))
     return bool(ok), x, L
+
+
+def vinv3(V, mu=0.0):
+    """(inverse, small-determinant branch taken) of one damped 3x3 block."""
+    Vm = np.ascontiguousarray(V, np.float64).reshape(9)
+    out = np.empty(9)
+    br = lib().orc_vinv3(_d(Vm), float(mu), _d(out))
+    return out.reshape(3, 3), bool(br)
+
+
+def dogleg_step(P_U, P_B, g, pUtBpU, pUtBpB, pBtBpB, delta):
+    """(p, reported norm) of the oracle's dogleg / 2-D subspace step."""
+    PU = np.ascontiguousarray(P_U, np.float64)
+    PB = np.ascontiguousarray(P_B, np.float64)
+    gg = np.ascontiguousarray(g, np.float64)
+    p = np.empty_like(PU)
+    nrm = lib().orc_dogleg_step(PU.size, pUtBpU, pUtBpB, pBtBpB, delta, _d(PU), _d(PB), _d(gg), _d(p))
+    return p, nrm
 
 
 def dense_modchol(S):

@@ -582,11 +582,11 @@ double orc_max_uvdiag(const orc_ctx *c) {
  * compute_Vinv.cl:41 is always true).  Returns 1 if any point took the
  * fallback (the reference's racy `ret`, here OR-reduced).
  */
-int orc_stage_vinv(orc_ctx *c, double mu) {
+/* one point's damped 3x3 inverse (PSBA/CL_files/compute_Vinv.cl:6-90) into
+ * inv[9]; returns 1 when the small-determinant branch (:31-73) was taken */
+static int vinv3(const double *Vi, double mu, double *inv_out) {
   int any = 0;
-#pragma omp parallel for schedule(static) reduction(| : any)
-  for (int i = 0; i < c->np; i++) {
-    const double *Vi = c->V + 9L * i;
+  {
     double a11 = Vi[0] + mu, a12 = Vi[1], a13 = Vi[2];
     double a22 = Vi[4] + mu, a23 = Vi[5], a33 = Vi[8] + mu;
     double a21 = a12, a31 = a13, a32 = a23;
@@ -629,8 +629,17 @@ int orc_stage_vinv(orc_ctx *c, double mu) {
     }
     /* the reference keeps only the lower triangle; read it symmetrically */
     inv[1] = inv[3]; inv[2] = inv[6]; inv[5] = inv[7];
-    memcpy(c->Vinv + 9L * i, inv, sizeof(inv));
+    memcpy(inv_out, inv, sizeof(inv));
   }
+  return any;
+}
+
+int orc_vinv3(const double *V9, double mu, double *out9) { return vinv3(V9, mu, out9); }
+
+int orc_stage_vinv(orc_ctx *c, double mu) {
+  int any = 0;
+#pragma omp parallel for schedule(static) reduction(| : any)
+  for (int i = 0; i < c->np; i++) any |= vinv3(c->V + 9L * i, mu, c->Vinv + 9L * i);
   return any;
 }
 
@@ -1278,10 +1287,9 @@ static int compute_PB(orc_ctx *c, const orc_opts *o, double *lambda, double *P_B
 }
 
 /* PSBA/PSBA/trust_region.cpp:520-595 (p_norm bookkeeping of :564-569 kept) */
-static double compute_p_2(const orc_ctx *c, double pUtBpU, double pUtBpB, double pBtBpB,
+static double compute_p_2(int N, double pUtBpU, double pUtBpB, double pBtBpB,
                           double delta, const double *P_U, const double *P_B, double *p,
                           const double *g) {
-  int N = c->ntot;
   double pUg = dotp(P_U, g, N), pBg = dotp(P_B, g, N);
   double pUpU = dotp(P_U, P_U, N), pUpB = dotp(P_U, P_B, N), pBpB = dotp(P_B, P_B, N);
   (void)pUpU; (void)pUpB; (void)pBpB;
@@ -1329,6 +1337,11 @@ static double compute_p_2(const orc_ctx *c, double pUtBpU, double pUtBpB, double
     }
   }
   return p_norm;
+}
+
+double orc_dogleg_step(int N, double pUtBpU, double pUtBpB, double pBtBpB, double delta,
+                       const double *P_U, const double *P_B, const double *g, double *p) {
+  return compute_p_2(N, pUtBpU, pUtBpB, pBtBpB, delta, P_U, P_B, p, g);
 }
 
 /* PSBA/PSBA/trust_region.cpp:49-288 */
@@ -1390,7 +1403,7 @@ int orc_trust_region(orc_ctx *c, const orc_opts *oin, int *itno, double *finalEr
     iter_flag = ORC_ITER_CONTINUE;
     while (iter_flag == ORC_ITER_CONTINUE) {
       orc_trace_rec rec = {*itno, 1, 1, 0, NAN, dk, NAN, lambda, NAN};
-      double p_norm = compute_p_2(c, pUtBpU, pUtBpB, pBtBpB, dk, P_U, P_B, P, g);
+      double p_norm = compute_p_2(c->ntot, pUtBpU, pUtBpB, pBtBpB, dk, P_U, P_B, P, g);
       rec.dp_sq = p_norm * p_norm;
       orc_stage_newp(c, P);
       double act_ex_L2 = orc_stage_residual(c, 1, NULL);

@@ -118,6 +118,15 @@ void orc_stage_jacobian(orc_ctx *c);
 void orc_stage_normal(orc_ctx *c, double coeff_uvw, double coeff_g);
 double orc_max_uvdiag(const orc_ctx *c);
 int orc_stage_vinv(orc_ctx *c, double mu);
+/* one point's damped inverse of a 3x3 block V9 (row-major, symmetric part
+ * used) as orc_stage_vinv forms it; returns 1 for the small-determinant branch */
+int orc_vinv3(const double *V9, double mu, double *out9);
+/* the dogleg / 2-D subspace step of orc_trust_region (PSBA/trust_region.cpp:
+ * 520-595): P_U the Cauchy step, P_B the (damped) Newton step, the three
+ * curvature products x^T B y, radius delta; p receives the step, the return
+ * value is the "norm" the reference reports (with its :564-569 bookkeeping) */
+double orc_dogleg_step(int N, double pUtBpU, double pUtBpB, double pBtBpB, double delta,
+                       const double *P_U, const double *P_B, const double *g, double *p);
 void orc_stage_schur(orc_ctx *c, double mu);
 void orc_stage_ea(orc_ctx *c);
 int orc_stage_factor_solve(orc_ctx *c, int solve_mode);

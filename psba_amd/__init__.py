@@ -166,6 +166,7 @@ def lib():
         "psba_stage_modchol": (ct.c_int, [vp, dp]),
         "psba_stage_refine": (ct.c_int, [vp, ct.c_double, ct.c_int]),
         "psba_stage_load_S": (ct.c_int, [vp, dp, dp]),
+        "psba_stage_vinv": (ct.c_int, [vp, dp, ct.c_double, dp]),
         "psba_buffer_size": (ct.c_long, [vp, ct.c_int]),
         "psba_read_buffer": (ct.c_int, [vp, ct.c_int, dp, ct.c_long]),
         "psba_get_timing": (ct.c_int, [vp, ct.POINTER(Timing)]),
@@ -180,6 +181,9 @@ def lib():
                                      ct.POINTER(SbaData)]),
         "psba_sba_free": (None, [ct.POINTER(SbaData)]),
         "psba_quat2vec": (None, [dp, ct.c_int, dp, ct.c_int]),
+        "psba_write_sba": (ct.c_int, [ct.c_char_p, ct.c_char_p, ct.c_int, ct.c_int, ct.c_int, dp, dp, dp, dp,
+                                      ct.POINTER(SbaData)]),
+        "psba_levmar_entry_cost": (ct.c_int, [vp, dp]),
         "psba_problem_bytes": (ct.c_longlong, [ct.c_int, ct.c_int, ct.c_int, ct.c_int, ct.c_int, ct.c_longlong]),
         "psba_batch_create": (ct.c_int, [ct.POINTER(BatchDesc), ct.POINTER(vp)]),
         "psba_batch_destroy": (ct.c_int, [vp]),
@@ -263,6 +267,30 @@ def read_sba(cams_path, pts_path, cnp=11, pnp=3, mnp=2, quat_filter=True, filecn
     finally:
         L.psba_sba_free(ct.byref(d))
     return out
+
+
+def write_sba(cams_path, pts_path, cnp, cams, pts, initrot, imgpts, vmask, K=None, cov=None, mnp=2):
+    """Solved parameters in the sba text format through psba_write_sba: each
+    camera line holds K (when given), the leading intrinsics, the whole
+    rotation (local rotation composed with initrot) as a unit quaternion and
+    the translation; point lines repeat the measured projections."""
+    L = lib()
+    vmask = np.ascontiguousarray(vmask, dtype=np.int8)
+    npt, nc = vmask.shape
+    imgpts = np.ascontiguousarray(imgpts, dtype=np.float64)
+    cams = np.ascontiguousarray(cams, dtype=np.float64)
+    pts = np.ascontiguousarray(pts, dtype=np.float64)
+    initrot = np.ascontiguousarray(initrot, dtype=np.float64)
+    Karr = None if K is None else np.ascontiguousarray(K, dtype=np.float64)
+    covarr = None if cov is None else np.ascontiguousarray(cov, dtype=np.float64)
+    d = SbaData(n_cams=nc, n_pts=npt, n_obs=imgpts.size // mnp, cnp=cnp, pnp=3, mnp=mnp)
+    d.imgpts = _dptr(imgpts)
+    d.vmask = vmask.ctypes.data_as(ct.POINTER(ct.c_char))
+    if covarr is not None:
+        d.covimgpts = _dptr(covarr)
+    _check(L.psba_write_sba(os.fsencode(cams_path), os.fsencode(pts_path), nc, npt, cnp,
+                            None if Karr is None else _dptr(Karr), _dptr(initrot), _dptr(cams), _dptr(pts),
+                            ct.byref(d)), "psba_write_sba")
 
 
 class Scene:
@@ -413,6 +441,12 @@ class Context:
         refine_steps: FP64 refinement sweeps per step (mixed contexts)."""
         return self._run(lib().psba_levmar, trace_cap, self._opts(max_iter, 0, flags, rel_tol, refine_steps))
 
+    def levmar_entry_cost(self):
+        """SSE at the entry of the last levmar call (the reference's initErr)."""
+        v = ct.c_double()
+        _check(lib().psba_levmar_entry_cost(self._h, ct.byref(v)), "psba_levmar_entry_cost")
+        return v.value
+
     def trust_region(self, max_iter=50, trace_cap=4096, max_pb_tries=0):
         """PSBA/trust_region.h:5-11 on the device."""
         return self._run(lib().psba_trust_region, trace_cap, self._opts(max_iter, max_pb_tries))
@@ -466,6 +500,14 @@ class Context:
         Sm = np.ascontiguousarray(S, dtype=np.float64)
         bb = None if b is None else np.ascontiguousarray(b, dtype=np.float64)
         _check(lib().psba_stage_load_S(self._h, _dptr(Sm), None if bb is None else _dptr(bb)), "load_S")
+
+    def vinv_blocks(self, V, mu=0.0):
+        """Test hook: the damped 3x3 inverses k_vinv forms for n_pts given
+        blocks V (n_pts x 3 x 3); replaces the context's V."""
+        Vm = np.ascontiguousarray(V, dtype=np.float64).reshape(self.n_pts, 9)
+        out = np.empty((self.n_pts, 9))
+        _check(lib().psba_stage_vinv(self._h, _dptr(Vm), float(mu), _dptr(out)), "psba_stage_vinv")
+        return out.reshape(self.n_pts, 3, 3)
 
     def factor_only(self):
         """Factor + solve of the loaded S; returns (ok, x) with x = S^-1 b."""

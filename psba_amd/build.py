@@ -35,6 +35,52 @@ HIP_FLAGS = [
 ]
 
 
+STAMP = os.path.join(ROOT, "psba_amd", ".build_stamp")
+
+
+def source_hash():
+    """SHA-256 over every source the built files come from (the HIP/C++
+    library, its headers, the oracle, the drop-in program, this script)."""
+    import hashlib
+    h = hashlib.sha256()
+    files = [os.path.join(CSRC, f) for f in sorted(os.listdir(CSRC))]
+    inc = os.path.join(ROOT, "include")
+    files += [os.path.join(inc, f) for f in sorted(os.listdir(inc))]
+    files += [os.path.join(ORACLE_DIR, "oracle.c"), os.path.join(ORACLE_DIR, "oracle.h"),
+              os.path.join(ROOT, "tests", "dropin", "main_dropin.cpp"), os.path.abspath(__file__)]
+    for f in files:
+        if os.path.isfile(f):
+            h.update(os.path.relpath(f, ROOT).encode())
+            with open(f, "rb") as fh:
+                h.update(fh.read())
+    return h.hexdigest()[:16]
+
+
+def built_hash():
+    """The source hash recorded by the last complete build, or None."""
+    try:
+        with open(STAMP) as f:
+            return f.read().strip()
+    except OSError:
+        return None
+
+
+def is_current():
+    outs = (LIB, ORACLE_LIB, DROPIN_BIN)
+    return all(os.path.exists(p) for p in outs) and built_hash() == source_hash()
+
+
+def build_all(jobs=8, verbose=False):
+    """Everything, then the stamp of the sources it was built from."""
+    h = source_hash()
+    build_oracle()
+    build_lib(jobs, verbose)
+    build_dropin()
+    with open(STAMP, "w") as f:
+        f.write(h + "\n")
+    return h
+
+
 def _run(cmd):
     r = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
     out = "\n".join(l for l in r.stdout.splitlines()
@@ -93,9 +139,12 @@ def main(argv=None):
     ap.add_argument("-v", action="store_true")
     a = ap.parse_args(argv)
     both = not (a.lib or a.oracle)
-    if a.oracle or both:
+    if both:
+        print("built sources", build_all(a.j, a.v))
+        return
+    if a.oracle:
         print("built", build_oracle())
-    if a.lib or both:
+    if a.lib:
         print("built", build_lib(a.j, a.v))
         print("built", build_dropin())
 

@@ -83,6 +83,7 @@ struct psba_ctx {
   int* schunk_order = nullptr;  // launch order of the pair chunks (schunk_launch_key)
   int* pt_order = nullptr;      // points by descending track length (ties: ascending id)
   double sum_t2 = 0;            // sum over points of track length squared
+  double entry_cost = 0;        // SSE at the entry of the last psba_levmar call
 
   bool mixed() const { return precision == PSBA_PREC_MIXED; }
 

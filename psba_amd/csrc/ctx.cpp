@@ -601,6 +601,25 @@ int psba_stage_schur(psba_ctx* c, double mu) {
   return PSBA_OK;
 }
 
+int psba_stage_vinv(psba_ctx* c, const double* V9, double mu, double* out9) {
+  if (!c || !V9 || !out9) return fail(PSBA_ERR_ARG, "null argument");
+  PSBA_HIP_TRY(hipSetDevice(c->device));
+  // symmetric 3x3 blocks in the device's packed order (xx xy xz yy yz zz)
+  static const int kPack[6] = {0, 1, 2, 4, 5, 8};
+  std::vector<double> packed((size_t)6 * c->np), inv((size_t)6 * c->np);
+  for (int i = 0; i < c->np; ++i)
+    for (int k = 0; k < 6; ++k) packed[(size_t)6 * i + k] = V9[(size_t)9 * i + kPack[k]];
+  PSBA_HIP_TRY(hipMemcpyAsync(c->V, packed.data(), sizeof(double) * packed.size(), hipMemcpyHostToDevice, c->stream));
+  PSBA_HIP_TRY(hipMemsetAsync(c->g + c->n, 0, sizeof(double) * 3 * (size_t)c->np, c->stream));
+  PSBA_TRY(launch_vinv(c, mu));
+  PSBA_HIP_TRY(hipMemcpyAsync(inv.data(), c->Vinv, sizeof(double) * inv.size(), hipMemcpyDeviceToHost, c->stream));
+  PSBA_HIP_TRY(hipStreamSynchronize(c->stream));
+  static const int kFull[9] = {0, 1, 2, 1, 3, 4, 2, 4, 5};
+  for (int i = 0; i < c->np; ++i)
+    for (int k = 0; k < 9; ++k) out9[(size_t)9 * i + k] = inv[(size_t)6 * i + kFull[k]];
+  return PSBA_OK;
+}
+
 int psba_stage_factor_solve(psba_ctx* c, int* ok) {
   PSBA_TRY(ready_dense(c));
   PSBA_TRY(launch_factor_solve(c));

@@ -151,6 +151,7 @@ int psba_levmar(psba_ctx* c, const psba_opts* o, int* itno, double* finalErr, ps
   PSBA_TRY(launch_residual(c, false));
   PSBA_TRY(fetch(c));
   double ex_L2 = c->h_scal[SC_COST];
+  c->entry_cost = ex_L2;
   int flag = PSBA_ITER_CONTINUE;
   for (; *itno < max_iter && flag == PSBA_ITER_CONTINUE; (*itno)++) {
     PSBA_TRY(jacobian(c, 1.0, 1.0));
@@ -328,6 +329,12 @@ int compute_p_2(psba_ctx* c, double pUtBpU, double pUtBpB, double pBtBpB, double
 }  // namespace
 
 extern "C" {
+
+int psba_levmar_entry_cost(psba_ctx* c, double* sse) {
+  if (!c || !sse) return PSBA_ERR_ARG;
+  *sse = c->entry_cost;
+  return PSBA_OK;
+}
 
 int psba_trust_region(psba_ctx* c, const psba_opts* o, int* itno, double* finalErr, psba_trace_rec* trace, int cap,
                       int* len) {

@@ -16,6 +16,9 @@
 // The shared iteration counter.  The application normally defines it (as
 // PSBA/main.cpp:27 does); this weak definition is used otherwise.
 __attribute__((weak)) int itno = 0;
+// The SSE at the entry of the last levmar call (PSBA/levmar.cpp:95; defined by
+// the application at PSBA/main.cpp:24 and printed at :216).  Weak, like itno.
+__attribute__((weak)) dtype initErr = 0;
 
 static void die_on(int rc, const char* where) {
   if (rc >= 0) return;
@@ -72,6 +75,7 @@ int levmar(PSBA_structPtr p, int cnp, int pnp, int mnp, int n3Dpts, int nCams, i
     die_on(PSBA_ERR_ARG, "levmar");
   int rc = psba_levmar(p->ctx, nullptr, &itno, finalErr, nullptr, 0, nullptr);
   die_on(rc, "levmar");
+  die_on(psba_levmar_entry_cost(p->ctx, &initErr), "levmar");
   return rc;
 }
 
@@ -89,18 +93,77 @@ int trust_region(PSBA_structPtr p, int cnp, int pnp, int mnp, int n3Dpts, int nC
 // PSBA/misc.cpp:21-49
 void quat2vec(dtype* inp, int nin, dtype* outp, int nout) { psba_quat2vec(inp, nin, outp, nout); }
 
-// PSBA/misc.cpp:60-85
+// PSBA/misc.cpp:60-85.  Layout [lead..., v1 v2 v3, tail...] with lead = nin - 6
+// leading values (none for a bare pose); the quaternion's scalar part is the
+// one that makes it a unit quaternion, taken non-negative.
 void vec2quat(dtype* inp, int nin, dtype* outp, int nout) {
-  int i = 0;
-  if (nin > 6)
-    for (; i < nin - 6; ++i) outp[i] = inp[i];
-  const dtype* v = inp + i;
-  outp[i + 1] = v[0];
-  outp[i + 2] = v[1];
-  outp[i + 3] = v[2];
-  outp[i] = std::sqrt(1.0 - v[0] * v[0] - v[1] * v[1] - v[2] * v[2]);
-  i += 4;
-  for (; i < nout; ++i) outp[i] = inp[i - 1];
+  const int lead = nin > 6 ? nin - 6 : 0;
+  const dtype* v = inp + lead;
+  dtype* q = outp + lead;
+  for (int k = 0; k < lead; ++k) outp[k] = inp[k];
+  const dtype vv = v[0] * v[0] + v[1] * v[1] + v[2] * v[2];
+  q[1] = v[0];
+  q[2] = v[1];
+  q[3] = v[2];
+  q[0] = std::sqrt(1.0 - vv);
+  const int tail = nout - lead - 4;  // values after the quaternion (translation)
+  for (int k = 0; k < tail; ++k) q[4 + k] = v[3 + k];
+}
+
+// PSBA/misc.cpp:91-113.  R(q) for the unit quaternion q = (s, x, y, z):
+//   R = (s^2 - |v|^2) I + 2 v v^T + 2 s [v]_x, with s^2 = 1 - |v|^2 written as
+//   1 - 2(...) on the diagonal.
+void quat2matrix(dtype* in, int nin, dtype* out) {
+  dtype s, x, y, z;
+  if (nin == 3) {
+    x = in[0];
+    y = in[1];
+    z = in[2];
+    s = std::sqrt(1 - x * x - y * y - z * z);
+  } else {
+    s = in[0];
+    x = in[1];
+    y = in[2];
+    z = in[3];
+  }
+  const dtype xx = 2 * x * x, yy = 2 * y * y, zz = 2 * z * z;
+  const dtype xy = 2 * x * y, xz = 2 * x * z, yz = 2 * y * z;
+  const dtype sx = 2 * x * s, sy = 2 * y * s, sz = 2 * z * s;
+  dtype* r0 = out;
+  dtype* r1 = out + 3;
+  dtype* r2 = out + 6;
+  r0[0] = 1 - yy - zz;
+  r1[1] = 1 - xx - zz;
+  r2[2] = 1 - xx - yy;
+  r0[1] = xy - sz;
+  r1[0] = xy + sz;
+  r0[2] = xz + sy;
+  r2[0] = xz - sy;
+  r1[2] = yz - sx;
+  r2[1] = yz + sx;
+}
+
+// PSBA/misc.cpp:119-133: nin cameras of [quaternion (dim values), translation
+// (stubForTrans values)] -> [R row-major (9 values), translation].
+void camsFormatTrans(dtype* in, int nin, int dim, int stubForTrans, dtype* out) {
+  const int sin = dim + stubForTrans, sout = 9 + stubForTrans;
+  for (int j = 0; j < nin; ++j) {
+    dtype* src = in + (size_t)j * sin;
+    dtype* dst = out + (size_t)j * sout;
+    quat2matrix(src, dim, dst);
+    for (int k = 0; k < stubForTrans; ++k) dst[9 + k] = src[dim + k];
+  }
+}
+
+// PSBA/misc.cpp:135-146
+void* emalloc_(const char* file, int line, size_t sz) {
+  void* ptr = malloc(sz);
+  if (!ptr) {
+    fprintf(stderr, "SBA: memory allocation request for %zu bytes failed in file %s, line %d, exiting", sz, file,
+            line);
+    exit(1);
+  }
+  return ptr;
 }
 
 dtype compute_L2_sq(int n, dtype* x) {

@@ -2,6 +2,7 @@
 // with their projections (PSBA/readparams.cpp:444-541 is the reference's).
 // The whole file is read into memory and tokenised line by line with strtod;
 // errors name the file and line and return a status code.
+#include <algorithm>
 #include <cerrno>
 #include <cmath>
 #include <cstdio>
@@ -97,18 +98,20 @@ enum { NOCOV = 0, FULLCOV = 1, TRICOV = 2 };
 
 }  // namespace
 
+// Layout [lead..., q0 q1 q2 q3, tail...] with lead = nin - 7 leading values
+// (intrinsics, distortion; none for a bare pose).  q and -q are the same
+// rotation; the representative with q0 >= 0 is kept and normalised, and only
+// its vector part goes out.  The tail (translation) is copied.
 extern "C" void psba_quat2vec(double* inp, int nin, double* outp, int nout) {
-  int i = 0;
-  if (nin > 7)
-    for (; i < nin - 7; ++i) outp[i] = inp[i];
-  double mag = std::sqrt(inp[i] * inp[i] + inp[i + 1] * inp[i + 1] + inp[i + 2] * inp[i + 2] + inp[i + 3] * inp[i + 3]);
-  double sg = (inp[i] >= 0.0) ? 1.0 : -1.0;
-  mag = sg / mag;
-  outp[i] = inp[i + 1] * mag;
-  outp[i + 1] = inp[i + 2] * mag;
-  outp[i + 2] = inp[i + 3] * mag;
-  i += 3;
-  for (; i < nout; ++i) outp[i] = inp[i + 1];
+  const int lead = nin > 7 ? nin - 7 : 0;
+  const double* q = inp + lead;
+  double* v = outp + lead;
+  for (int k = 0; k < lead; ++k) outp[k] = inp[k];
+  const double qq = q[0] * q[0] + q[1] * q[1] + q[2] * q[2] + q[3] * q[3];
+  const double scale = (q[0] < 0.0 ? -1.0 : 1.0) / std::sqrt(qq);
+  for (int k = 0; k < 3; ++k) v[k] = q[1 + k] * scale;
+  const int tail = nout - lead - 3;
+  for (int k = 0; k < tail; ++k) v[3 + k] = q[4 + k];
 }
 
 extern "C" void psba_sba_free(psba_sba_data* d) {
@@ -184,12 +187,22 @@ extern "C" int psba_read_sba(const char* camsfname, const char* ptsfname, int cn
   std::vector<double> pts((size_t)npts * pnp), proj, cov;
   std::vector<char> vm((size_t)npts * ncams, 0);
   std::vector<double> cv(full > 0 ? full : 1);
+  // One point's projections as read, in file order: frame, then mnp values
+  // (and the full covariance) per projection in pproj / pcov.  generate_idxs
+  // numbers a point's observations in ascending camera order
+  // (PSBA/misc.cpp:189-216) while the reference stores them in file order
+  // (PSBA/readparams.cpp:364-415); the two agree only for sorted frame lists,
+  // so each point's projections are put in ascending frame order here.
+  std::vector<int> pframe, perm;
+  std::vector<double> pproj, pcov;
   for (int i = 0; i < npts; ++i) {
     Tok t{pl[i].text};
     for (int k = 0; k < pnp; ++k)
       if (!t.next_double(&pts[(size_t)i * pnp + k])) return bad(ptsfname, pl[i].no, "expecting the point's coordinates");
     int nframes = 0;
     if (!t.next_int(&nframes) || nframes < 0) return bad(ptsfname, pl[i].no, "expecting the number of frames of the point");
+    const size_t proj0 = proj.size(), cov0 = cov.size();
+    pframe.clear();
     for (int f = 0; f < nframes; ++f) {
       int frame = 0;
       if (!t.next_int(&frame)) return bad(ptsfname, pl[i].no, "line holds fewer projections than its frame count");
@@ -200,6 +213,7 @@ extern "C" int psba_read_sba(const char* camsfname, const char* ptsfname, int cn
       }
       if (vm[(size_t)i * ncams + frame]) return bad(ptsfname, pl[i].no, "point projected twice in one frame");
       vm[(size_t)i * ncams + frame] = 1;
+      pframe.push_back(frame);
       for (int k = 0; k < mnp; ++k) {
         double x;
         if (!t.next_double(&x)) return bad(ptsfname, pl[i].no, "line holds fewer projections than its frame count");
@@ -221,6 +235,19 @@ extern "C" int psba_read_sba(const char* camsfname, const char* ptsfname, int cn
       }
     }
     if (!t.at_end()) return bad(ptsfname, pl[i].no, "line holds more values than its frame count implies");
+    if (nframes > 1 && !std::is_sorted(pframe.begin(), pframe.end())) {
+      perm.resize(nframes);
+      for (int f = 0; f < nframes; ++f) perm[f] = f;
+      std::sort(perm.begin(), perm.end(), [&](int a, int b) { return pframe[a] < pframe[b]; });
+      pproj.assign(proj.begin() + proj0, proj.end());
+      for (int f = 0; f < nframes; ++f)
+        std::copy_n(pproj.begin() + (size_t)perm[f] * mnp, mnp, proj.begin() + proj0 + (size_t)f * mnp);
+      if (ncov) {
+        pcov.assign(cov.begin() + cov0, cov.end());
+        for (int f = 0; f < nframes; ++f)
+          std::copy_n(pcov.begin() + (size_t)perm[f] * full, full, cov.begin() + cov0 + (size_t)f * full);
+      }
+    }
   }
 
   const int nobs = (int)(proj.size() / mnp);
@@ -269,4 +296,70 @@ void readInitialSBAEstimate(char* camsfname, char* ptsfname, int cnp, int pnp, i
   *imgpts = d.imgpts;
   *covimgpts = d.covimgpts;
   *vmask = d.vmask;
+}
+
+// Writer of the same text format (the reference declares printSBA* writers but
+// leaves them commented out, PSBA/readparams.h:13-25).  A camera line is
+// [K (5 values, when given)] [the model's leading intrinsics] [q0' q1' q2' q3']
+// [t], where q' = q_v (x) q0 is the camera's whole rotation (the optimised
+// local rotation composed with its initial one, as the projection uses it),
+// normalised with q0' >= 0.  A point line is X Y Z, the number of frames and,
+// per frame in ascending order, the frame index and the measured projection
+// (with its full covariance when obs holds one).  Numbers use %.17g, so
+// reading the files back gives the same doubles.
+extern "C" int psba_write_sba(const char* camsfname, const char* ptsfname, int n_cams, int n_pts, int cnp,
+                              const double* K, const double* initrot, const double* cams, const double* pts,
+                              const psba_sba_data* obs) {
+  if (!camsfname || !ptsfname || n_cams <= 0 || n_pts < 0 || cnp < 6 || !initrot || !cams || !pts || !obs ||
+      obs->n_cams != n_cams || obs->n_pts != n_pts || !obs->vmask || (obs->n_obs > 0 && !obs->imgpts)) {
+    psba::set_last_error(__FILE__, __LINE__, "psba_write_sba: bad arguments");
+    return PSBA_ERR_ARG;
+  }
+  const int mnp = obs->mnp > 0 ? obs->mnp : 2;
+  FILE* fc = fopen(camsfname, "w");
+  if (!fc) return bad(camsfname, 0, "cannot create file");
+  for (int j = 0; j < n_cams; ++j) {
+    const double* c = cams + (size_t)j * cnp;
+    const double* v = c + (cnp - 6);
+    const double* q0 = initrot + (size_t)j * 4;
+    const double s = std::sqrt(1.0 - (v[0] * v[0] + v[1] * v[1] + v[2] * v[2]));
+    // q = (s, v) (x) q0, Hamilton product, scalar first
+    double q[4] = {s * q0[0] - v[0] * q0[1] - v[1] * q0[2] - v[2] * q0[3],
+                   s * q0[1] + v[0] * q0[0] + v[1] * q0[3] - v[2] * q0[2],
+                   s * q0[2] - v[0] * q0[3] + v[1] * q0[0] + v[2] * q0[1],
+                   s * q0[3] + v[0] * q0[2] - v[1] * q0[1] + v[2] * q0[0]};
+    const double nq = std::sqrt(q[0] * q[0] + q[1] * q[1] + q[2] * q[2] + q[3] * q[3]);
+    const double sc = (q[0] < 0 ? -1.0 : 1.0) / nq;
+    if (K)
+      for (int k = 0; k < 5; ++k) fprintf(fc, "%.17g ", K[(size_t)j * 5 + k]);
+    for (int k = 0; k < cnp - 6; ++k) fprintf(fc, "%.17g ", c[k]);
+    for (int k = 0; k < 4; ++k) fprintf(fc, "%.17g ", q[k] * sc);
+    fprintf(fc, "%.17g %.17g %.17g\n", c[cnp - 3], c[cnp - 2], c[cnp - 1]);
+  }
+  if (fclose(fc) != 0) return bad(camsfname, 0, "write failed");
+  FILE* fp = fopen(ptsfname, "w");
+  if (!fp) return bad(ptsfname, 0, "cannot create file");
+  long o = 0;
+  for (int i = 0; i < n_pts; ++i) {
+    const char* vm = obs->vmask + (size_t)i * n_cams;
+    int nf = 0;
+    for (int j = 0; j < n_cams; ++j) nf += vm[j] != 0;
+    fprintf(fp, "%.17g %.17g %.17g %d", pts[(size_t)i * 3], pts[(size_t)i * 3 + 1], pts[(size_t)i * 3 + 2], nf);
+    for (int j = 0; j < n_cams; ++j) {
+      if (!vm[j]) continue;
+      if (o >= obs->n_obs) {
+        fclose(fp);
+        return bad(ptsfname, i + 1, "vmask lists more projections than obs->n_obs");
+      }
+      fprintf(fp, "  %d", j);
+      for (int k = 0; k < mnp; ++k) fprintf(fp, " %.17g", obs->imgpts[(size_t)o * mnp + k]);
+      if (obs->covimgpts)
+        for (int k = 0; k < mnp * mnp; ++k) fprintf(fp, " %.17g", obs->covimgpts[(size_t)o * mnp * mnp + k]);
+      ++o;
+    }
+    fprintf(fp, "\n");
+  }
+  if (fclose(fp) != 0) return bad(ptsfname, 0, "write failed");
+  if (o != obs->n_obs) return bad(ptsfname, 0, "vmask lists fewer projections than obs->n_obs");
+  return PSBA_OK;
 }

@@ -20,17 +20,19 @@ def pytest_configure(config):
 
 
 def _ensure_built():
-    lib = os.path.join(ROOT, "psba_amd", "libpsba.so")
-    orc = os.path.join(ROOT, "oracle", "liboracle.so")
-    dropin = os.path.join(ROOT, "tests", "dropin", "main_dropin")
-    if not (os.path.exists(lib) and os.path.exists(orc) and os.path.exists(dropin)):
-        from psba_amd import build
-        build.build_oracle()
-        build.build_lib()
-        build.build_dropin()
+    """Build when a library is missing or was built from other sources than
+    the tree's (the stamp holds the source hash of the last full build)."""
+    from psba_amd import build
+    if not build.is_current():
+        build.build_all()
 
 
 _ensure_built()
+
+
+def pytest_report_header(config):
+    from psba_amd import build
+    return "psba sources %s, libraries built from %s" % (build.source_hash(), build.built_hash())
 
 
 @pytest.fixture(scope="session")

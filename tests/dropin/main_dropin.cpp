@@ -6,12 +6,16 @@
 //
 //   main_dropin CAMS PTS OUT [origin_cnp=11]
 //
-// Prints iter_flag / itno / initial and final error; writes the final camera
+// Prints the reference's report (iter_flag, "initial error", "final error",
+// "total iteration", PSBA/main.cpp:210-218) plus exact itno / init_err /
+// final_err lines; uses the library's emalloc and the levmar-set initErr as
+// the reference program does.  Writes the final camera
 // (cnp = origin_cnp - 5) and point parameters to OUT as raw doubles.
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <ctime>
 
 #include "cl_psba.h"
 #include "levmar.h"
@@ -20,16 +24,8 @@
 #include "readparams.h"
 #include "trust_region.h"
 
-int itno = 0;  // the shared iteration counter the drivers advance (PSBA/main.cpp:27)
-
-static void* emalloc(size_t n) {
-  void* p = malloc(n ? n : 1);
-  if (!p) {
-    fprintf(stderr, "out of memory\n");
-    exit(1);
-  }
-  return p;
-}
+dtype initErr;  // set by levmar (PSBA/main.cpp:24, PSBA/levmar.cpp:95)
+int itno = 0;   // the shared iteration counter the drivers advance (PSBA/main.cpp:27)
 
 int main(int argc, char** argv) {
   if (argc < 4) {
@@ -70,20 +66,28 @@ int main(int argc, char** argv) {
 
   dtype finalErr = 0;
   int iter_flag;
+  clock_t start_time = clock();
   while (true) {
     iter_flag = levmar(&ps, cnp, pnp, mnp, n3Dpts, nCams, n2Dprojs, &finalErr);
     if (iter_flag != ITER_TURN_TO_TR) break;
     iter_flag = trust_region(&ps, cnp, pnp, mnp, n3Dpts, nCams, n2Dprojs, blk_idx, &finalErr);
     if (iter_flag != ITER_TURN_TO_LM) break;
   }
+  clock_t end_time = clock();
   dtype* cams = (dtype*)emalloc(sizeof(dtype) * nCams * cnp);
   dtype* pts = (dtype*)emalloc(sizeof(dtype) * n3Dpts * pnp);
   read_params(&ps, cams, pts);
   release_buffer(&ps);
+  // the report of PSBA/main.cpp:210-218
   printf("iter_flag=%d\n", iter_flag);
+  printf("time eclipse %lf s\n", double(end_time - start_time) / CLOCKS_PER_SEC);
+  printf("initial error: %.15E \n", sqrt(initErr) / n2Dprojs);
+  printf("final error: %.15E \n", sqrt(finalErr) / n2Dprojs);
+  printf("total iteration: %d\n", itno);
+  // exact values for the tests
   printf("itno=%d\n", itno);
+  printf("init_err=%.17g\n", initErr);
   printf("final_err=%.17g\n", finalErr);
-  printf("final_rms_per_proj=%.17g\n", std::sqrt(finalErr) / n2Dprojs);
   FILE* f = fopen(argv[3], "wb");
   if (!f) return 3;
   fwrite(cams, sizeof(dtype), (size_t)nCams * cnp, f);

@@ -14,7 +14,7 @@ INCLUDE = os.path.join(ROOT, "include")
 LIB = os.path.join(ROOT, "psba_amd", "libpsba.so")
 DROPIN_NAMES = {"setup_cl", "release_buffer", "fill_initBuffer2", "fill_idxBuffer", "read_params", "levmar",
                 "trust_region", "quat2vec", "vec2quat", "compute_L2_sq", "dotProduct", "generate_idxs",
-                "readInitialSBAEstimate"}
+                "readInitialSBAEstimate", "camsFormatTrans", "quat2matrix", "emalloc_"}
 
 
 def declared_functions():
@@ -54,6 +54,7 @@ def test_every_declared_function_is_exported():
             missing.append(n)
     assert not missing, missing
     assert "itno" in plain  # the shared iteration counter (weak, PSBA/main.cpp:27)
+    assert "initErr" in plain  # the SSE at levmar's entry (weak, PSBA/main.cpp:24, levmar.cpp:95)
 
 
 def test_reference_style_program_links(tmp_path):

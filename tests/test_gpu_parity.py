@@ -353,3 +353,27 @@ def test_mixed_context_rejects_dogleg(P, gpu):
     g = P.Context.from_scene(sc, precision=P.PREC_MIXED)
     with pytest.raises(P.PsbaError, match="FP64 contexts only"):
         g.trust_region()
+
+
+def test_vinv_small_determinant_branch_on_device(P, O, gpu):
+    """k_vinv on hand-built blocks (tests/test_oracle_pins.py derives their
+    results by hand): the small-determinant branch of compute_Vinv.cl:31-73
+    with its sign quirk and zero pivot, the damped and regular cases, each
+    against the oracle's inverse."""
+    from test_oracle_pins import HAND, S
+    sc = P.Scene(0, 3)
+    blocks = [S * M for M, _ in HAND]
+    blocks.append(S * np.array([[1.0, 2, 0], [2, 5, 0], [0, 0, 1]]))  # zero pivot: adj / 0
+    blocks.append(np.array([[4.0, 1, 0.5], [1, 3, 0.2], [0.5, 0.2, 2]]))  # regular
+    V = np.array([blocks[k % len(blocks)] for k in range(sc.n_pts)])
+    with P.Context.from_scene(sc) as g:
+        for mu in (0.0, 1e-7):
+            out = g.vinv_blocks(V, mu)
+            for i in range(len(blocks)):
+                ref, _ = O.vinv3(V[i], mu)
+                fin = np.isfinite(ref)
+                assert np.array_equal(np.isfinite(out[i]), fin), i
+                assert np.allclose(out[i][fin], ref[fin], rtol=1e-12, atol=0), (i, mu)
+                if mu == 0.0 and i < len(HAND):
+                    M, sign = HAND[i]
+                    assert np.allclose(out[i], sign * np.linalg.inv(V[i]), rtol=1e-12, atol=0)

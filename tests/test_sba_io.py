@@ -1,7 +1,8 @@
 """The sba text reader (psba_read_sba / readInitialSBAEstimate, the reference's
 PSBA/readparams.cpp:444-541) on the reference's own data files and on small
-files written here: counts, file-order projections, visibility mask, the
-quat2vec camera filter, covariance forms and the error paths."""
+files written here: counts, projections in ascending frame order per point,
+visibility mask, the quat2vec camera filter, covariance forms, the error
+paths, and the writer of solved parameters (psba_write_sba)."""
 import os
 
 import numpy as np
@@ -11,7 +12,8 @@ from conftest import GOLDEN
 import sbafmt
 
 CASES = [("7camsvarK.txt", "3pts.txt"), ("7camsvarK.txt", "5pts.txt"), ("7camsvarK.txt", "7pts.txt"),
-         ("9camsvarK.txt", "9pts.txt")]
+         ("9camsvarK.txt", "9pts.txt"), ("54camsvarK.txt", "54pts.txt"),
+         ("Trafalgar-21-11315-cams.txt", "Trafalgar-21-11315-pts.txt")]
 
 
 def quat_vec(q):
@@ -35,8 +37,9 @@ def test_reads_reference_files(P, cams_name, pts_name):
         assert abs(np.linalg.norm(q) - 1) < 1e-12 and q[0] >= 0
         assert np.allclose(q[1:], cams[j, 5:8], atol=0)
     assert np.array_equal(d["pts"].reshape(-1, 3), X)
-    # projections stay in file order; the mask marks every (point, frame)
-    flat = np.array([[x, y] for fr in tracks for _, x, y in fr]).reshape(-1)
+    # projections in ascending frame order per point (the shipped files are
+    # already sorted); the mask marks every (point, frame)
+    flat = np.array([[x, y] for fr in tracks for _, x, y in sorted(fr, key=lambda r: r[0])]).reshape(-1)
     assert np.array_equal(d["imgpts"], flat)
     assert np.array_equal(d["vmask"], sbafmt.vmask_of(tracks, len(rows)))
     assert d["cov"] is None
@@ -109,3 +112,62 @@ def test_camera_line_length_and_missing_files(P, tmp_path):
         P.read_sba(c, p, cnp=6, quat_filter=False)
     with pytest.raises(P.PsbaError, match="cannot open"):
         P.read_sba(str(tmp_path / "none.txt"), p, cnp=6, quat_filter=False)
+
+
+def test_unsorted_frames_are_put_in_camera_order(P, tmp_path):
+    """generate_idxs numbers a point's observations camera-ascending
+    (PSBA/misc.cpp:189-216); a file listing frames out of order must not
+    shift projections onto the wrong cameras (covariances move with them)."""
+    c = write(tmp_path, "c.txt", CAMS6)
+    p = write(tmp_path, "p.txt", "1 2 3 3  2 20 21 5 0 0 6   0 10 11 1 0 0 1   1 14 15 2 0 0 3\n"
+                                 "4 5 6 2  1 30 31 7 0 0 8   0 40 41 9 0 0 10\n")
+    d = P.read_sba(c, p, cnp=6, quat_filter=False)
+    assert np.array_equal(d["imgpts"], np.array([10, 11, 14, 15, 20, 21, 40, 41, 30, 31], float))
+    cov = d["cov"].reshape(-1, 4)
+    assert np.array_equal(cov[:, 0], np.array([1, 2, 5, 9, 7], float))
+    assert np.array_equal(d["vmask"], np.array([[1, 1, 1], [1, 1, 0]], np.int8))
+    # the sorted file reads identically
+    p2 = write(tmp_path, "p2.txt", "1 2 3 3  0 10 11 1 0 0 1  1 14 15 2 0 0 3  2 20 21 5 0 0 6\n"
+                                  "4 5 6 2  0 40 41 9 0 0 10  1 30 31 7 0 0 8\n")
+    d2 = P.read_sba(c, p2, cnp=6, quat_filter=False)
+    for k in ("imgpts", "cov", "vmask", "pts", "cams"):
+        assert np.array_equal(d[k], d2[k])
+
+
+def _qmul(a, b):
+    s1, v1 = a[0], np.asarray(a[1:])
+    s2, v2 = b[0], np.asarray(b[1:])
+    return np.concatenate([[s1 * s2 - v1 @ v2], s1 * v2 + s2 * v1 + np.cross(v1, v2)])
+
+
+@pytest.mark.parametrize("with_k", [True, False])
+def test_writer_round_trip(P, tmp_path, with_k):
+    """Solved parameters (local rotation v relative to initrot) written and read
+    back: the read rotation is q_v (x) q0 (independently composed here), every
+    other value and every projection comes back as the same double."""
+    src = P.read_sba(os.path.join(GOLDEN, "7camsvarK.txt"), os.path.join(GOLDEN, "7pts.txt"))
+    nc = src["n_cams"]
+    rng = np.random.default_rng(11)
+    c11 = src["cams"].reshape(nc, 11)
+    K = c11[:, :5].copy()
+    solved = c11[:, 5:].copy()  # cnp 6: [v t]
+    solved[:, :3] = rng.uniform(-0.05, 0.05, (nc, 3))
+    solved[:, 3:] += rng.standard_normal((nc, 3)) * 0.1
+    pts = src["pts"] + rng.standard_normal(src["pts"].shape) * 0.01
+    cp, pp = str(tmp_path / "c.txt"), str(tmp_path / "p.txt")
+    P.write_sba(cp, pp, 6, solved.reshape(-1), pts, src["initrot"], src["imgpts"], src["vmask"],
+                K=K.reshape(-1) if with_k else None)
+    back = P.read_sba(cp, pp, cnp=11 if with_k else 6)
+    cb = back["cams"].reshape(nc, -1)
+    for j in range(nc):
+        v = solved[j, :3]
+        qv = np.concatenate([[np.sqrt(1 - v @ v)], v])
+        q = _qmul(qv, src["initrot"][4 * j:4 * j + 4])
+        q = q / np.linalg.norm(q) * np.sign(q[0])
+        assert np.allclose(back["initrot"][4 * j:4 * j + 4], q, rtol=0, atol=2e-15)
+        assert np.array_equal(cb[j, -3:], solved[j, 3:])
+        if with_k:
+            assert np.array_equal(cb[j, :5], K[j])
+    assert np.array_equal(back["pts"], pts)
+    assert np.array_equal(back["imgpts"], src["imgpts"])
+    assert np.array_equal(back["vmask"], src["vmask"])
