@@ -44,20 +44,52 @@ def profiled_traffic(kernel):
     return None, None
 
 
-def cpu_baseline(scene, kw):
-    """The oracle (the CPU FP64 restatement of the reference's LM) on this
-    host's cores: one complete solve of the same scene with the same stopping
-    rule, timed end to end."""
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(P, scene, gpu_trials, reps=5):
+    """The oracle (the CPU FP64 restatement of the reference's LM, with the
+    reference's explicit S^-1 solve, PSBA/CL_files/SPD_inv.cl:248-411 +
+    matVec_mul.cl:7-18) on this host's cores.  Bounded sample: one LM
+    iteration (Jacobian, normal equations, explicit-inverse step, trial cost)
+    from the starting point of each scene, median of `reps` runs, on one
+    thread and on all cores used here; configs 0, 1 (cnp 11), 2 and the
+    three request classes of config 4.  The headline value scales the
+    all-core iteration time of config 2 by the GPU solve's trial count."""
     from oracle import binding as O
-    threads = int(os.environ.get("OMP_NUM_THREADS") or min(16, os.cpu_count() or 1))
-    O.set_threads(threads)
-    orc = O.Oracle.from_scene(scene)
-    t0 = time.perf_counter()
-    r = orc.levmar(max_iter=kw["max_iter"], flags=O.FLAG_NO_HANDOFF, rel_tol=kw["rel_tol"])
-    dt = time.perf_counter() - t0
-    return {"value": 1.0 / dt, "unit": "solves/s", "cores": threads, "kind": "port",
-            "sample": "one full LM solve of the same scene and stopping rule (%d iterations, %d trials, "
-                      "%.1f s, final cost %.10g)" % (r["itno"], len(r["trace"]), dt, r["final_err"])}
+    nproc = os.cpu_count() or 1
+    allc = int(os.environ.get("OMP_NUM_THREADS") or min(16, nproc))
+    cases = [("config2", scene), ("config0", P.Scene(0, 1000)), ("config1", P.Scene(1, 1001)),
+             ("config4_small", P.Scene(4, 1004, problem=0, size_class=0)),
+             ("config4_medium", P.Scene(4, 1004, problem=10000, size_class=1)),
+             ("config4_large", P.Scene(4, 1004, problem=20000, size_class=2))]
+    per = {}
+    for name, sc in cases:
+        row = {"n_cams": sc.n_cams, "n_pts": sc.n_pts, "n_obs": sc.n_obs}
+        for th in sorted({1, allc}):
+            O.set_threads(th)
+            ts = []
+            for _ in range(reps):
+                o = O.Oracle.from_scene(sc)
+                t0 = time.perf_counter()
+                o.levmar(max_iter=1, solve_mode=O.SOLVE_INVERSE, flags=O.FLAG_NO_HANDOFF)
+                ts.append(time.perf_counter() - t0)
+            row["ms_per_iteration_threads_%d" % th] = 1e3 * float(np.median(ts))
+        per[name] = row
+        log("cpu baseline %s: %s" % (name, ", ".join("%s %.1f" % kv for kv in row.items() if kv[0].startswith("ms"))))
+    it_s = per["config2"]["ms_per_iteration_threads_%d" % allc] * 1e-3
+    return {"value": 1.0 / (it_s * gpu_trials), "unit": "solves/s", "cores": allc, "kind": "port",
+            "sample": "median of %d runs of one LM iteration (explicit-inverse step) of the config-2 scene on %d "
+                      "threads, %.3f s, scaled by the %d trials of the GPU solve; 1-thread and other configs in "
+                      "per_config" % (reps, allc, it_s, gpu_trials),
+            "cpu_model": cpu_model(), "nproc": nproc, "per_config": per}
 
 
 def mixed_section(P, sc, a, t_plain, rms):
@@ -293,15 +325,17 @@ def stream_section(P, seed=1004, rates=(200, 1000, 2000, 4000), duration=2.0, se
 
 def stage_rates(P, ctx, solve_on, sc, info, r, solve_s):
     """Absolute rates of the headline solve: LM iterations and trials per
-    second, and the Jacobian and dense-solve stages, each timed by HIP events
-    on the context's stream over one further solve, with their roofline
-    fractions.  Jacobian bytes per observation: the reads (camera, point,
-    measurement, indices: 40 B) and the writes (residual 2, J_A 2C, J_B 6 and
-    W 3C doubles).  Dense solve flops: n_pad^3 / 3 for the factor plus 2
-    n_pad^2 for the two triangular solves."""
+    second, and the Jacobian, Schur-assembly and back-substitution stages,
+    each timed by HIP events on the context's stream over one further solve,
+    with their roofline fractions.  Jacobian bytes per observation: the reads
+    (camera, point, measurement, indices: 40 B) and the writes (residual 2,
+    J_A 2C, J_B 6 and W 3C doubles).  Schur flops: per (point, camera pair
+    k >= l of its track) one C x 3 by 3 x C product, 6 C^2 flops.  Back
+    substitution bytes: W rows (3C doubles) and camera indices per
+    observation, V^-1, g_b and the outputs per point."""
     out = {"lm_iterations_per_s": r["itno"] / solve_s, "lm_trials_per_s": len(r["trace"]) / solve_s}
     timed = {}
-    for name, which in (("jacobian", P.PROBE_JACOBIAN), ("dense_solve", P.PROBE_FACTOR)):
+    for name, which in (("jacobian", P.PROBE_JACOBIAN), ("schur", P.PROBE_SCHUR), ("backsub", P.PROBE_BSOLVE)):
         ctx.probe(which)
         ctx.probe_read()
         solve_on(ctx)
@@ -314,18 +348,52 @@ def stage_rates(P, ctx, solve_on, sc, info, r, solve_s):
     out["jacobian"] = {"launches": n, "avg_launch_us": avg * 1e6, "obs_per_s": sc.n_obs / avg,
                        "bytes_per_launch": nbytes, "achieved_GBps": nbytes / avg / 1e9,
                        "frac_of_hbm_peak": nbytes / avg / 1e9 / HBM_PEAK_GBS}
-    ms, n = timed["dense_solve"]
+    ms, n = timed["schur"]
     avg = ms * 1e-3 / max(n, 1)
-    npad = float(info["n_pad"])
-    flops = npad ** 3 / 3.0 + 2.0 * npad * npad
-    out["dense_solve"] = {"launches": n, "avg_launch_us": avg * 1e6, "n_pad": int(npad), "flops_per_launch": flops,
-                          "achieved_GFLOPs": flops / avg / 1e9,
-                          "frac_of_fp64_peak": flops / avg / 1e12 / FP64_PEAK_TFLOPS}
-    log("stages: %.0f LM iterations/s; Jacobian %.0f us (%.2e obs/s, %.0f GB/s); dense solve %.0f us "
-        "(%.0f GFLOP/s)" % (out["lm_iterations_per_s"], out["jacobian"]["avg_launch_us"],
-                            out["jacobian"]["obs_per_s"], out["jacobian"]["achieved_GBps"],
-                            out["dense_solve"]["avg_launch_us"], out["dense_solve"]["achieved_GFLOPs"]))
+    flops = 6.0 * C * C * info["n_pair_items"]
+    traffic, src = profiled_traffic("k_schur_chunks")
+    out["schur"] = {"kernel": "k_schur_chunks", "launches": n, "avg_launch_us": avg * 1e6, "flops_per_launch": flops,
+                    "achieved_TFLOPs": flops / avg / 1e12, "frac_of_fp64_peak": flops / avg / 1e12 / FP64_PEAK_TFLOPS,
+                    "traffic": traffic, "traffic_source": src}
+    ms, n = timed["backsub"]
+    avg = ms * 1e-3 / max(n, 1)
+    nbytes = (3 * C * 8 + 4) * sc.n_obs + (6 + 3 + 3 + 3) * 8 * sc.n_pts
+    out["backsub"] = {"launches": n, "avg_launch_us": avg * 1e6, "bytes_per_launch": nbytes,
+                      "achieved_GBps": nbytes / avg / 1e9, "frac_of_hbm_peak": nbytes / avg / 1e9 / HBM_PEAK_GBS}
+    log("stages: %.0f LM iterations/s; Jacobian %.0f us (%.2e obs/s, %.0f GB/s); Schur %.0f us (%.2f TFLOP/s); "
+        "back substitution %.0f us (%.0f GB/s)" % (
+            out["lm_iterations_per_s"], out["jacobian"]["avg_launch_us"], out["jacobian"]["obs_per_s"],
+            out["jacobian"]["achieved_GBps"], out["schur"]["avg_launch_us"], out["schur"]["achieved_TFLOPs"],
+            out["backsub"]["avg_launch_us"], out["backsub"]["achieved_GBps"]))
     return out
+
+
+def path_roofline(sc, info, r, solve_s):
+    """T_roof / T_measured for the whole solve: every stage at its own roof
+    (HBM bytes at 8 TB/s or FP64 flops at 78.6 TFLOP/s, whichever is longer),
+    summed over the solve's Jacobian passes (accepted iterations + 1) and
+    trials.  Bytes and flops per stage are the algorithmic counts of the
+    reference's stage graph with the Jacobian fused into U/V/W/g (FP64,
+    s = 8 B): residual 40 B/obs; Jacobian + normal equations
+    n_o (8 + s (2(C+3) + 3C + 2)) + s (n_c C^2 + 9 n_p); V^-1 18 s n_p;
+    Schur 6 C^2 sum_i t_i (t_i + 1) / 2 flops; e_a, e_b, dp_b 2 s 3C n_o;
+    factor and solves n^3 / 3 + 2 n^2 flops; step statistics and commit
+    3 s n_tot."""
+    C, no, nc, np_ = sc.cnp, sc.n_obs, sc.n_cams, sc.n_pts
+    s = 8.0
+    n = float(C * nc)
+    ntot = n + 3 * np_
+    bw, fl = HBM_PEAK_GBS * 1e9, FP64_PEAK_TFLOPS * 1e12
+    t_res = 40.0 * no / bw
+    t_jac = (no * (8 + s * (2 * (C + 3) + 3 * C + 2)) + s * (nc * C * C + 9 * np_)) / bw
+    t_trial = (18 * s * np_ / bw + 6.0 * C * C * info["n_pair_items"] / fl + 2 * s * 3 * C * no / bw
+               + (n ** 3 / 3 + 2 * n * n) / fl + 3 * s * ntot / bw + t_res)
+    trials = len(r["trace"])
+    jac = r["itno"] + 1
+    t_roof = t_res + jac * t_jac + trials * t_trial
+    return {"t_roof_ms": 1e3 * t_roof, "t_measured_ms": 1e3 * solve_s, "frac": t_roof / solve_s,
+            "trials": trials, "jacobian_passes": jac,
+            "per_trial_roof_us": 1e6 * t_trial, "per_jacobian_roof_us": 1e6 * t_jac}
 
 
 def main():
@@ -362,15 +430,16 @@ def main():
 
     for _ in range(a.warmup):
         solve()
-    # HIP events around every Schur-assembly launch on the context's stream
-    ctx.probe(P.PROBE_SCHUR)
+    # HIP events around every launch of the dense factor + solve (the kernel
+    # with the largest share of the solve's device time) on the context's stream
+    ctx.probe(P.PROBE_FACTOR)
     ctx.probe_read()
     res = []
     t0 = time.perf_counter()
     for _ in range(a.steps):
         res.append(solve())  # returns after its final device read-back (stream drained)
     dt = time.perf_counter() - t0
-    schur_ms, schur_n = ctx.probe_read()
+    fac_ms, fac_n = ctx.probe_read()
     ctx.probe(P.PROBE_OFF)
     info = ctx.info()
     r = res[-1]
@@ -398,21 +467,24 @@ def main():
                    "n_obs": sc.n_obs, "sum_t2": sc.sum_t2, "lm_iterations": r["itno"], "lm_trials": trials,
                    "final_rms_px": rms},
     }
-    # Schur assembly S = U - sum_i W_ij V_i^-1 W_ik^T over the lower camera-pair
-    # blocks: per (point, pair j >= k of its cameras) one C x 3 by 3 x C product,
-    # 3 C^2 FMAs = 6 C^2 flops; sum over points of t(t+1)/2 such products.
-    C = sc.cnp
-    flops = 6.0 * C * C * info["n_pair_items"]
-    avg_s = schur_ms * 1e-3 / max(schur_n, 1)
+    # Dense Cholesky of the reduced camera matrix (order n = C n_c) and the two
+    # triangular solves: n^3 / 3 + 2 n^2 flops per launch (n, not the padded
+    # storage order)
+    n = float(info["n"])
+    flops = n ** 3 / 3.0 + 2.0 * n * n
+    avg_s = fac_ms * 1e-3 / max(fac_n, 1)
     achieved = flops / avg_s / 1e12
-    traffic, traffic_src = profiled_traffic("k_schur_chunks")
+    traffic, traffic_src = profiled_traffic("k_chol_solve")
     out["roofline"] = {"bound": "mfma", "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
                        "frac": achieved / FP64_PEAK_TFLOPS, "traffic": traffic,
-                       "kernel": "k_schur_chunks", "avg_launch_us": avg_s * 1e6, "launches": schur_n,
-                       "flops_per_launch": flops, "traffic_source": traffic_src}
-    log("schur: %d launches, %.1f us avg, %.2f TFLOP/s (%.1f%% of FP64 peak)" % (
-        schur_n, avg_s * 1e6, achieved, 100 * achieved / FP64_PEAK_TFLOPS))
+                       "kernel": "k_chol_solve", "avg_launch_us": avg_s * 1e6, "launches": fac_n,
+                       "flops_per_launch": flops, "n": int(n), "traffic_source": traffic_src}
+    log("dense factor+solve: %d launches, %.1f us avg, %.3f TFLOP/s (%.2f%% of FP64 peak)" % (
+        fac_n, avg_s * 1e6, achieved, 100 * achieved / FP64_PEAK_TFLOPS))
     out["stages"] = stage_rates(P, ctx, solve_on, sc, info, r, dt / a.steps)
+    out["path_roofline"] = path_roofline(sc, info, r, dt / a.steps)
+    log("path roofline: T_roof %.2f ms vs measured %.2f ms per solve (%.1f%%)" % (
+        out["path_roofline"]["t_roof_ms"], out["path_roofline"]["t_measured_ms"], 100 * out["path_roofline"]["frac"]))
     if "mixed" in sections:
         out["mixed"] = mixed_section(P, sc, a, dt / a.steps, rms)
     if "index" in sections:
@@ -422,7 +494,7 @@ def main():
     if "stream" in sections:
         out["stream"] = stream_section(P)
     if not a.no_cpu:
-        out["cpu_baseline"] = cpu_baseline(sc, kw)
+        out["cpu_baseline"] = cpu_baseline(P, sc, trials)
     print(json.dumps(out), flush=True)
 
 

@@ -59,6 +59,8 @@ typedef struct {
 
 /* psba_opts.flags */
 #define PSBA_FLAG_NO_HANDOFF 1 /* LM never returns ITER_TURN_TO_TR (pure LM solve) */
+#define PSBA_FLAG_HOST_CONTROL 2 /* psba_levmar decides each trial on the host (one
+                                    read-back per trial) instead of on the device */
 
 typedef struct {
   int max_iter;      /* shared iteration cap (reference: 50) */

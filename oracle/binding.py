@@ -61,6 +61,7 @@ def lib():
         "orc_create": (vp, [ct.POINTER(Problem)]),
         "orc_destroy": (None, [vp]),
         "orc_set_threads": (None, [ct.c_int]),
+        "orc_set_sum_order": (None, [ct.c_int]),
         "orc_get_threads": (ct.c_int, []),
         "orc_get_params": (None, [vp, dp, dp]),
         "orc_set_params": (None, [vp, dp, dp]),
@@ -108,6 +109,11 @@ def _i(a):
 
 def set_threads(n):
     lib().orc_set_threads(int(n))
+
+
+def set_sum_order(descending):
+    """Every reduction in ascending (False, the default) or descending index order."""
+    lib().orc_set_sum_order(int(bool(descending)))
 
 
 def project(cnp, K5, q0, cam, M, jac=False):

@@ -399,14 +399,29 @@ void orc_set_params(orc_ctx *c, const double *cams, const double *pts) {
 /* ------------------------------------------------------------------------ */
 /* reductions, PSBA/misc.cpp:151-169 (serial FP64, index order)              */
 /* ------------------------------------------------------------------------ */
+/* Summation order of every reduction below: ascending index (the
+ * reference's order, default) or, for experiments on how far two valid FP64
+ * evaluation orders stay together, descending.  RIDX maps the k-th step of a
+ * loop over [b, e) to the index it adds. */
+static int g_reverse = 0;
+#define RIDX(b, e, k) (g_reverse ? (e) - 1 - ((k) - (b)) : (k))
+
+void orc_set_sum_order(int descending) { g_reverse = descending ? 1 : 0; }
+
 static double l2sq(long n, const double *x) {
   double s = 0;
-  for (long i = 0; i < n; i++) s += x[i] * x[i];
+  for (long i0 = 0; i0 < n; i0++) {
+    const long i = RIDX(0, n, i0);
+    s += x[i] * x[i];
+  }
   return s;
 }
 static double dotp(const double *a, const double *b, long n) {
   double s = 0.0;
-  for (long i = 0; i < n; i++) s += a[i] * b[i];
+  for (long i0 = 0; i0 < n; i0++) {
+    const long i = RIDX(0, n, i0);
+    s += a[i] * b[i];
+  }
   return s;
 }
 
@@ -465,7 +480,8 @@ double orc_stage_residual(orc_ctx *c, int use_new, double *ex) {
   if (ex) memcpy(ex, c->ex, sizeof(double) * 2 * c->no);
   if (c->loss == ORC_LOSS_NONE) return l2sq(2L * c->no, c->ex);
   double sum = 0;
-  for (int k = 0; k < c->no; k++) {
+  for (int k0 = 0; k0 < c->no; k0++) {
+    const int k = RIDX(0, c->no, k0);
     double rho, drho;
     orc_loss_eval(c->loss, c->loss_scale, c->ex[2 * k] * c->ex[2 * k] + c->ex[2 * k + 1] * c->ex[2 * k + 1], &rho,
                   &drho);
@@ -515,7 +531,8 @@ void orc_stage_normal(orc_ctx *c, double coeff, double coeff_g) {
     for (int r = 0; r < C; r++) {
       for (int cc = 0; cc < C; cc++) {
         double sum = 0;
-        for (int a = c->cam_ptr[j]; a < c->cam_ptr[j + 1]; a++) {
+        for (int a0 = c->cam_ptr[j]; a0 < c->cam_ptr[j + 1]; a0++) {
+          const int a = RIDX(c->cam_ptr[j], c->cam_ptr[j + 1], a0);
           const double *ja = c->JA + (long)2 * C * c->cam_obs[a];
           sum = sum + ja[r] * ja[cc] + ja[C + r] * ja[C + cc];
         }
@@ -524,7 +541,8 @@ void orc_stage_normal(orc_ctx *c, double coeff, double coeff_g) {
         if (r == cc) c->uvdiag[C * j + r] = sum;
       }
       double sg = 0;
-      for (int a = c->cam_ptr[j]; a < c->cam_ptr[j + 1]; a++) {
+      for (int a0 = c->cam_ptr[j]; a0 < c->cam_ptr[j + 1]; a0++) {
+        const int a = RIDX(c->cam_ptr[j], c->cam_ptr[j + 1], a0);
         int o = c->cam_obs[a];
         const double *ja = c->JA + (long)2 * C * o;
         sg = sg + ja[r] * c->ex[2 * o] + ja[C + r] * c->ex[2 * o + 1];
@@ -537,7 +555,8 @@ void orc_stage_normal(orc_ctx *c, double coeff, double coeff_g) {
     for (int r = 0; r < 3; r++) {
       for (int cc = 0; cc < 3; cc++) {
         double sum = 0;
-        for (int o = c->pt_ptr[i]; o < c->pt_ptr[i + 1]; o++) {
+        for (int o0 = c->pt_ptr[i]; o0 < c->pt_ptr[i + 1]; o0++) {
+          const int o = RIDX(c->pt_ptr[i], c->pt_ptr[i + 1], o0);
           const double *jb = c->JB + 6L * o;
           sum = sum + jb[r] * jb[cc] + jb[3 + r] * jb[3 + cc];
         }
@@ -546,7 +565,8 @@ void orc_stage_normal(orc_ctx *c, double coeff, double coeff_g) {
         if (r == cc) c->uvdiag[c->n + 3 * i + r] = sum;
       }
       double sg = 0;
-      for (int o = c->pt_ptr[i]; o < c->pt_ptr[i + 1]; o++) {
+      for (int o0 = c->pt_ptr[i]; o0 < c->pt_ptr[i + 1]; o0++) {
+        const int o = RIDX(c->pt_ptr[i], c->pt_ptr[i + 1], o0);
         const double *jb = c->JB + 6L * o;
         sg = sg + jb[r] * c->ex[2 * o] + jb[3 + r] * c->ex[2 * o + 1];
       }
@@ -669,7 +689,8 @@ void orc_stage_schur(orc_ctx *c, double mu) {
     for (int r = 0; r < C; r++)
       for (int cc = 0; cc < C; cc++) {
         double sum = 0;
-        for (long it = c->pair_ptr[q]; it < c->pair_ptr[q + 1]; it++) {
+        for (long it0 = c->pair_ptr[q]; it0 < c->pair_ptr[q + 1]; it0++) {
+          const long it = RIDX(c->pair_ptr[q], c->pair_ptr[q + 1], it0);
           const double *y = c->Y + (long)3 * C * c->pair_ok[it] + 3 * r;
           const double *w = c->W + (long)3 * C * c->pair_ol[it] + 3 * cc;
           sum += y[0] * w[0] + y[1] * w[1] + y[2] * w[2];
@@ -712,7 +733,8 @@ void orc_stage_ea(orc_ctx *c) {
   for (int j = 0; j < c->nc; j++)
     for (int r = 0; r < C; r++) {
       double sum = 0;
-      for (int a = c->cam_ptr[j]; a < c->cam_ptr[j + 1]; a++) {
+      for (int a0 = c->cam_ptr[j]; a0 < c->cam_ptr[j + 1]; a0++) {
+        const int a = RIDX(c->cam_ptr[j], c->cam_ptr[j + 1], a0);
         int o = c->cam_obs[a];
         const double *y = c->Y + (long)3 * C * o + 3 * r;
         const double *gb = c->g + c->n + 3 * c->iidx[o];
@@ -901,7 +923,8 @@ void orc_stage_backsub(orc_ctx *c) {
     double eb[3];
     for (int cc = 0; cc < 3; cc++) {
       double sum = 0;
-      for (int o = c->pt_ptr[i]; o < c->pt_ptr[i + 1]; o++) {
+      for (int o0 = c->pt_ptr[i]; o0 < c->pt_ptr[i + 1]; o0++) {
+        const int o = RIDX(c->pt_ptr[i], c->pt_ptr[i + 1], o0);
         const double *w = c->W + (long)3 * C * o;
         const double *da = c->dp + C * c->jidx[o];
         for (int k = 0; k < C; k++) sum = sum + w[3 * k + cc] * da[k];
@@ -1169,7 +1192,10 @@ static double params_l2_full(const orc_ctx *c) {
 /* PSBA/PSBA/levmar.cpp:271-280 */
 static double compute_rho(const orc_ctx *c, double ex_L2, double new_ex_L2, double mu) {
   double sum = 0;
-  for (int q = 0; q < c->ntot; q++) sum += c->dp[q] * (mu * c->dp[q] + c->g[q]);
+  for (int q0 = 0; q0 < c->ntot; q0++) {
+    const int q = RIDX(0, c->ntot, q0);
+    sum += c->dp[q] * (mu * c->dp[q] + c->g[q]);
+  }
   return (ex_L2 - new_ex_L2) / sum;
 }
 

@@ -107,6 +107,9 @@ void orc_generate_idxs(int nCams, int n3Dpts, const char *vmask, int *comm3DIdx,
 orc_ctx *orc_create(const orc_problem *p);
 void orc_destroy(orc_ctx *c);
 void orc_set_threads(int n);
+/* order of every reduction: 0 ascending index (default, the reference's),
+ * 1 descending (a second valid FP64 evaluation order, for experiments) */
+void orc_set_sum_order(int descending);
 int orc_get_threads(void);
 /* current parameters */
 void orc_get_params(const orc_ctx *c, double *cams, double *pts);

@@ -48,6 +48,7 @@ class Desc(ct.Structure):
 
 
 FLAG_NO_HANDOFF = 1
+FLAG_HOST_CONTROL = 2  # psba_levmar decides each trial on the host
 
 
 class Opts(ct.Structure):

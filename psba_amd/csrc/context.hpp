@@ -44,6 +44,15 @@ enum Scalar {
 };
 
 
+// Device-held LM state of one problem (k_lm.hip) and the gates it sets for
+// the kernels of the next trial.
+struct LmDev {
+  double mu, ex_L2, p_L2, rel_tol, entry_cost;
+  int nu, gooditer, flag, itno, max_iter, first, p_pending, no_handoff;
+  int ntrace, trace_cap, trials, pad_;
+};
+enum { kGateActive = 0, kGateJac = 1, kGateAccept = 2, kGateCount = 4 };
+
 struct Timing {
   double jac = 0, schur = 0, factor = 0, solve = 0, backsub = 0, resid = 0, other = 0;
   int trials = 0, jacobians = 0;
@@ -104,10 +113,10 @@ struct psba_ctx {
   long ref_part_cap = 0;
   int refine_steps = 0;        // sweeps per trial (psba_opts.refine_steps)
   // fused factor + solve (k_chol_solve): factor, right-hand side copy, y = L^-1 b,
-  // grid-barrier counter (only grows; bar_base is its value before a launch)
+  // bar = [grid-barrier counter (only grows), x flag, T row flags, epoch
+  // (the counter's value before the next launch)]
   double *Lf = nullptr, *bw = nullptr, *yv = nullptr;
   unsigned long long* bar = nullptr;
-  unsigned long long bar_base = 0;
   int* chol_fail = nullptr;
   int chol_grid = 0, chol_resident = 0;
   const double* L_cur = nullptr;  // where the last factor lives (Lf, or S after the modified Cholesky)
@@ -117,6 +126,20 @@ struct psba_ctx {
   long red_part_cap = 0;
   double* d_scal = nullptr;
   double* h_scal = nullptr;  // pinned
+
+  // device-side LM control (psba_levmar without a host round trip per
+  // trial): state, gates, trace records, a zero-filled owner array (the
+  // batch kernels' per-element problem index, all 0 for one problem), pinned
+  // status copies and their events.  While `devctl` is set the launchers
+  // read the damping from lm->mu and pass the gates to their kernels.
+  bool devctl = false;
+  psba::LmDev* lm = nullptr;
+  unsigned char* gate = nullptr;
+  psba_trace_rec* lm_trace = nullptr;
+  int lm_trace_cap = 0;
+  int* zero_owner = nullptr;
+  psba::LmDev* h_lm = nullptr;  // pinned, two slots
+  hipEvent_t ev_lm[2] = {nullptr, nullptr};
 
   // kernel probe (events around one kernel's launches)
   int probe_id = 0;
@@ -170,6 +193,15 @@ int launch_backsub_rhs(psba_ctx* c, const double* gb, double* dp_out);
 int check_obs_order(int nc, int np, int no, const int* iidx, const int* jidx);  // host validation
 // frees the previous index and allocates every index buffer and the chunk partials
 int alloc_index(psba_ctx* c, long nitems, int npairs, int n_cchunk, int n_schunk);
+
+// ---- device-side LM control (k_lm.hip) ----
+int launch_lm_init(psba_ctx* c, int itno, int max_iter, double rel_tol, int no_handoff);
+int launch_lm_mu0(psba_ctx* c);
+int launch_lm_control(psba_ctx* c);
+// gates of the launchers in device-control mode (nullptr otherwise)
+inline const unsigned char* gate_of(const psba_ctx* c, int which) { return c->devctl ? c->gate + which : nullptr; }
+inline const int* owner_of(const psba_ctx* c) { return c->devctl ? c->zero_owner : nullptr; }
+inline const double* mu_of(const psba_ctx* c) { return c->devctl ? &c->lm->mu : nullptr; }
 
 // ---- launchers (k_tr.hip) ----
 int launch_cauchy(psba_ctx* c, double* out, const double* g, double gtg, double gtBg);

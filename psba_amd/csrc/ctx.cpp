@@ -215,7 +215,7 @@ int psba_create(const psba_desc* d, psba_ctx** out) {
   chk(dalloc(&c->Lf, (size_t)c->n_pad * c->n_pad));
   chk(dalloc(&c->bw, (size_t)c->n_pad));
   chk(dalloc(&c->yv, (size_t)c->n_pad));
-  chk(dalloc(&c->bar, (size_t)2 + (size_t)(c->n_pad / kNB)));  // barrier counter, back-substitution flags
+  chk(dalloc(&c->bar, (size_t)3 + (size_t)(c->n_pad / kNB)));  // barrier counter, back-substitution flags, epoch
   chk(dalloc(&c->chol_fail, (size_t)1));
   c->red_part_cap = 8 * (long)std::max(ceil_div(c->no, 256), ceil_div(c->ntot, 256)) + 64;
   chk(dalloc(&c->red_part, (size_t)c->red_part_cap));
@@ -233,7 +233,7 @@ int psba_create(const psba_desc* d, psba_ctx** out) {
     z(hipMemsetAsync(c->dp, 0, sizeof(double) * (size_t)c->ntot, c->stream));
     z(hipMemsetAsync(c->d_scal, 0, sizeof(double) * SC_COUNT, c->stream));
     z(hipMemsetAsync(c->chol_fail, 0, sizeof(int), c->stream));
-    z(hipMemsetAsync(c->bar, 0, sizeof(unsigned long long) * (2 + (size_t)(c->n_pad / kNB)), c->stream));
+    z(hipMemsetAsync(c->bar, 0, sizeof(unsigned long long) * (3 + (size_t)(c->n_pad / kNB)), c->stream));
     z(hipMemsetAsync(c->Lf, 0, sizeof(double) * (size_t)c->n_pad * c->n_pad, c->stream));
     if (c->n_pad > c->n)
       hipLaunchKernelGGL(k_pad_identity, dim3(1), dim3(64), 0, c->stream, c->S, c->n_pad, c->n, c->n_pad);
@@ -269,6 +269,13 @@ int psba_destroy(psba_ctx* c) {
   dfree(c->ref_part);
   dfree(c->bar);
   dfree(c->chol_fail);
+  if (c->lm) (void)hipFree(c->lm);
+  if (c->gate) (void)hipFree(c->gate);
+  if (c->lm_trace) (void)hipFree(c->lm_trace);
+  if (c->zero_owner) (void)hipFree(c->zero_owner);
+  if (c->h_lm) (void)hipHostFree(c->h_lm);
+  for (auto e : c->ev_lm)
+    if (e) (void)hipEventDestroy(e);
   if (c->h_scal) (void)hipHostFree(c->h_scal);
   for (auto& pr : c->probe_pending) {
     if (pr.first) (void)hipEventDestroy(pr.first);
@@ -625,6 +632,8 @@ int psba_stage_factor_solve(psba_ctx* c, int* ok) {
   PSBA_TRY(launch_factor_solve(c));
   PSBA_HIP_TRY(hipMemcpyAsync(c->h_scal, c->d_scal, sizeof(double) * SC_COUNT, hipMemcpyDeviceToHost, c->stream));
   PSBA_HIP_TRY(hipStreamSynchronize(c->stream));
+  if (c->h_scal[SC_FACTOR_OK] < 0.0)
+    return fail(PSBA_ERR_HIP, "dense solve: a grid-barrier wait timed out (workgroups not all resident)");
   if (ok) *ok = c->h_scal[SC_FACTOR_OK] != 0.0;
   return PSBA_OK;
 }

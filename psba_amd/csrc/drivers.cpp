@@ -9,6 +9,7 @@
 //   hybrid  PSBA/main.cpp:193-208
 // The damping is applied on the fly to the undamped U/V blocks, so the
 // reference's update_UV / restore_UVdiag pair has no counterpart.
+#include <algorithm>
 #include <chrono>
 #include <cmath>
 #include <cstring>
@@ -29,6 +30,20 @@ constexpr double kMaxDelta = 10000.0;  // PSBA/trust_region.cpp:18
 int fetch(psba_ctx* c) {
   PSBA_HIP_TRY(hipMemcpyAsync(c->h_scal, c->d_scal, sizeof(double) * SC_COUNT, hipMemcpyDeviceToHost, c->stream));
   PSBA_HIP_TRY(hipStreamSynchronize(c->stream));
+  return PSBA_OK;
+}
+
+// the dense solve's status after a fetch: a timed-out wait inside the launch
+// (its workgroups were not all resident) is an execution error, not a failed
+// factorisation
+int factor_status(psba_ctx* c, bool* ok) {
+  const double f = c->h_scal[SC_FACTOR_OK];
+  if (f < 0.0) {
+    set_last_error(__FILE__, __LINE__,
+                   "dense solve: a grid-barrier wait timed out (the launch's workgroups were not all resident)");
+    return PSBA_ERR_HIP;
+  }
+  *ok = f != 0.0;
   return PSBA_OK;
 }
 
@@ -101,6 +116,115 @@ int jacobian(psba_ctx* c, double coeff, double coeff_g) {
   return launch_jacobian_normal(c, coeff, coeff_g);
 }
 
+// ---------------------------------------------------------------------------
+// psba_levmar with the decision on the device (k_lm.hip).  A trial is one
+// fixed sequence of launches -- Jacobian + normal equations (gated: only when
+// the trial opens an iteration), mu0 (first iteration), V^-1, Schur
+// complement, dense factor + solve, back substitution, step statistics,
+// trial cost, the control kernel, the commit (gated: accepted trials) -- and
+// an asynchronous copy of the LM state to one of two pinned slots.  Two
+// trials are kept queued: while the GPU runs trial t+1 the host waits for
+// trial t's event and reads its state, so the device never waits for the
+// host.  The trial behind the one that stopped the solve finds the gates off
+// and runs as no-ops.  The trace records are read back once, at the end.
+// ---------------------------------------------------------------------------
+constexpr int kLmDepth = 2;
+
+int lm_device_alloc(psba_ctx* c, int trace_cap) {
+  if (!c->lm) {
+    PSBA_HIP_TRY(hipMalloc((void**)&c->lm, sizeof(LmDev)));
+    PSBA_HIP_TRY(hipMalloc((void**)&c->gate, kGateCount));
+    PSBA_HIP_TRY(hipHostMalloc((void**)&c->h_lm, sizeof(LmDev) * kLmDepth, hipHostMallocDefault));
+    for (auto& e : c->ev_lm) PSBA_HIP_TRY(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    // the batch kernels' per-element problem index (observations, points,
+    // cameras, camera chunks, pair chunks): 0 everywhere for one problem
+    const long nown = std::max<long>(std::max<long>(c->no, c->np), std::max<long>(std::max(c->nc, c->n_cchunk),
+                                                                                   c->n_schunk)) + 1;
+    PSBA_HIP_TRY(hipMalloc((void**)&c->zero_owner, sizeof(int) * (size_t)nown));
+    PSBA_HIP_TRY(hipMemsetAsync(c->zero_owner, 0, sizeof(int) * (size_t)nown, c->stream));
+  }
+  if (trace_cap > c->lm_trace_cap) {
+    if (c->lm_trace) (void)hipFree(c->lm_trace);
+    c->lm_trace = nullptr;
+    c->lm_trace_cap = 0;
+    PSBA_HIP_TRY(hipMalloc((void**)&c->lm_trace, sizeof(psba_trace_rec) * (size_t)trace_cap));
+    c->lm_trace_cap = trace_cap;
+  }
+  return PSBA_OK;
+}
+
+int lm_enqueue_trial(psba_ctx* c, int slot) {
+  PSBA_TRY(launch_jacobian_normal(c, 1.0, 1.0));
+  PSBA_TRY(launch_lm_mu0(c));
+  PSBA_TRY(launch_vinv(c, 0.0));
+  PSBA_TRY(launch_schur(c, 0.0));
+  PSBA_TRY(launch_factor_solve(c));
+  PSBA_TRY(launch_backsub(c));
+  PSBA_TRY(launch_step_stats(c, 0.0));
+  PSBA_TRY(launch_residual(c, true));
+  PSBA_TRY(launch_lm_control(c));
+  PSBA_TRY(launch_commit(c));
+  PSBA_HIP_TRY(hipMemcpyAsync(c->h_lm + slot, c->lm, sizeof(LmDev), hipMemcpyDeviceToHost, c->stream));
+  PSBA_HIP_TRY(hipEventRecord(c->ev_lm[slot], c->stream));
+  return PSBA_OK;
+}
+
+struct DevCtl {
+  psba_ctx* c;
+  explicit DevCtl(psba_ctx* cc) : c(cc) { c->devctl = true; }
+  ~DevCtl() { c->devctl = false; }
+};
+
+int lm_device(psba_ctx* c, const psba_opts* o, int* itno, double* finalErr, psba_trace_rec* trace, int cap,
+              int* len) {
+  const int max_iter = opt_max_iter(o);
+  const bool no_handoff = o && (o->flags & PSBA_FLAG_NO_HANDOFF);
+  const double rel_tol = o ? o->rel_tol : 0.0;
+  // at most 31 rejected trials per iteration (nu doubles from 2 to 2^31)
+  const long max_trials = (long)std::max(0, max_iter - *itno) * 32 + 1;
+  PSBA_TRY(lm_device_alloc(c, (int)std::min<long>(max_trials, 1L << 20)));
+  PSBA_TRY(launch_residual(c, false));
+  PSBA_TRY(launch_lm_init(c, *itno, max_iter, rel_tol, no_handoff ? 1 : 0));
+  {
+    DevCtl on(c);
+    long launched = 0, checked = 0;
+    if (*itno < max_iter) {
+      for (;;) {
+        while (launched - checked < kLmDepth && launched < max_trials) {
+          PSBA_TRY(lm_enqueue_trial(c, (int)(launched % kLmDepth)));
+          launched++;
+        }
+        const int slot = (int)(checked % kLmDepth);
+        PSBA_HIP_TRY(hipEventSynchronize(c->ev_lm[slot]));
+        const LmDev st = c->h_lm[slot];
+        checked++;
+        if (st.flag < 0 || st.flag != PSBA_ITER_CONTINUE || st.itno >= st.max_iter || checked >= max_trials) break;
+      }
+    }
+  }
+  PSBA_HIP_TRY(hipMemcpyAsync(c->h_lm, c->lm, sizeof(LmDev), hipMemcpyDeviceToHost, c->stream));
+  PSBA_HIP_TRY(hipStreamSynchronize(c->stream));
+  const LmDev st = c->h_lm[0];
+  if (st.flag < 0) {
+    set_last_error(__FILE__, __LINE__,
+                   "dense solve: a grid-barrier wait timed out (the launch's workgroups were not all resident)");
+    return st.flag;
+  }
+  if (trace && len) {
+    const int base = *len;
+    const int take = std::max(0, std::min(st.ntrace, std::min(st.trace_cap, cap - base)));
+    if (take > 0)
+      PSBA_HIP_TRY(hipMemcpy(trace + base, c->lm_trace, sizeof(psba_trace_rec) * (size_t)take,
+                             hipMemcpyDeviceToHost));
+    *len = base + st.ntrace;
+  }
+  c->entry_cost = st.entry_cost;
+  c->timing.trials += st.trials;
+  *itno = st.itno;
+  *finalErr = st.ex_L2;
+  return st.flag;
+}
+
 }  // namespace
 
 
@@ -145,6 +269,11 @@ int psba_levmar(psba_ctx* c, const psba_opts* o, int* itno, double* finalErr, ps
   const bool no_handoff = o && (o->flags & PSBA_FLAG_NO_HANDOFF);
   const double rel_tol = o ? o->rel_tol : 0.0;
   c->refine_steps = (o && c->mixed() && o->refine_steps > 0) ? o->refine_steps : 0;
+  // the decision on the device, unless the solve needs the host between
+  // kernels: refinement sweeps (a host contraction test per sweep), per-stage
+  // timers (an event wait per stage) or the host-loop option
+  if (c->refine_steps == 0 && !c->timing_on && !(o && (o->flags & PSBA_FLAG_HOST_CONTROL)))
+    return lm_device(c, o, itno, finalErr, trace, cap, len);
   double mu = 0.0, p_L2 = 0.0;
   int gooditer_cnt = 0, nu = 2;
   bool first = true, p_pending = false;
@@ -178,7 +307,8 @@ int psba_levmar(psba_ctx* c, const psba_opts* o, int* itno, double* finalErr, ps
         p_L2 = c->h_scal[SC_P2];
         p_pending = false;
       }
-      const bool ok = c->h_scal[SC_FACTOR_OK] != 0.0;
+      bool ok = false;
+      PSBA_TRY(factor_status(c, &ok));
       rec.factor_ok = ok;
       if (ok) {
         const double dp_L2 = c->h_scal[SC_DP2];
@@ -263,7 +393,9 @@ int compute_PB(psba_ctx* c, double* lambda, int* solved) {
   }
   PSBA_TRY(fetch(c));
   c->timing.trials++;
-  if (c->h_scal[SC_FACTOR_OK] == 0.0) {
+  bool factored = false;
+  PSBA_TRY(factor_status(c, &factored));
+  if (!factored) {
     if (*lambda == 0.0) {
       StageTimer t(c, &c->timing.other);
       PSBA_TRY(launch_modchol(c, c->Saux));

@@ -294,9 +294,13 @@ int launch_modchol(psba_ctx* c, double* A) {
 // blocks above.
 //
 // The factor goes to Lf (S is overwritten by the trailing updates); e_a is
-// left intact.  Barriers: a 64-bit arrival counter that only grows (the host
-// passes the base), each wait bounded; a timed-out wait or a non-positive
-// pivot clears d_scal[SC_FACTOR_OK].  The grid is at most one workgroup per
+// left intact.  Barriers: a 64-bit arrival counter that only grows; its value
+// before a launch (the epoch) is kept on the device beside the flags
+// (bar[2 + T]) and advanced by workgroup 0 at the end of the launch, so a
+// launch that a gate turns off leaves it unchanged.  Every wait is bounded.
+// d_scal[SC_FACTOR_OK] becomes 1 on success, 0 when a pivot is not finite
+// and positive, and -1 when a wait timed out (an execution error: the
+// workgroups were not all resident).  The grid is at most one workgroup per
 // CU, so on the context's in-order stream every workgroup is resident at once;
 // should another stream's kernels hold CUs, the waiting workgroups are
 // scheduled as those finish (the bounded waits cover the worst case).
@@ -796,10 +800,15 @@ __global__ __launch_bounds__(256) void k_chol_solve(double* __restrict__ A, int 
                                                     const double* __restrict__ ea, double* __restrict__ bw,
                                                     double* __restrict__ yv, double* __restrict__ xs,
                                                     double* __restrict__ dp, int n, double* __restrict__ ok,
-                                                    unsigned long long* bar, unsigned long long bar_base,
-                                                    int* __restrict__ fail) {
+                                                    unsigned long long* bar, int* __restrict__ fail,
+                                                    const unsigned char* __restrict__ gate) {
   __shared__ CholSmem sm;
+  if (gate && !*gate) return;
   const int G = gridDim.x;
+  // read before this workgroup's first arrival, so before workgroup 0 (which
+  // passes every barrier last) advances it at the end
+  const unsigned long long bar_base =
+      __hip_atomic_load(bar + 2 + T, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   const int tid = threadIdx.x;
   unsigned long long nbar = 0;
   auto tile = [&](double* base, int i, int j) { return base + (size_t)i * NB * ld + (size_t)j * NB; };
@@ -992,10 +1001,13 @@ __global__ __launch_bounds__(256) void k_chol_solve(double* __restrict__ A, int 
   }
   if (blockIdx.x == 0 && tid == 0) {
     const int f = __hip_atomic_load(fail, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    *ok = f ? 0.0 : 1.0;
+    *ok = (f & 2) ? -1.0 : (f ? 0.0 : 1.0);
     // cleared for the next launch (every other workgroup is past its last
-    // wait: workgroup 0 consumed every helper's last flag)
+    // wait: workgroup 0 consumed every helper's last flag), and the epoch of
+    // the next launch: this launch's T barriers of G arrivals each
     __hip_atomic_store(fail, 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __hip_atomic_store(bar + 2 + T, bar_base + (unsigned long long)G * T, __ATOMIC_RELAXED,
+                       __HIP_MEMORY_SCOPE_AGENT);
   }
 }
 
@@ -1126,13 +1138,11 @@ int launch_factor_solve(psba_ctx* c) {
     c->chol_grid = std::max(1, std::min(c->chol_resident, (T - 1) * T / 2));
   }
   int G = c->chol_grid;
-  unsigned long long base = c->bar_base;
-  c->bar_base += (unsigned long long)G * (unsigned long long)T;
   double* ok = c->d_scal + SC_FACTOR_OK;
   int ld = c->n_pad, n = c->n;
   probe_begin(c, PSBA_PROBE_FACTOR);
   hipLaunchKernelGGL(k_chol_solve, dim3(G), dim3(256), 0, c->stream, c->S, ld, T, c->Lf, c->ea, c->bw, c->yv,
-                     c->xsol, c->dp, n, ok, c->bar, base, c->chol_fail);
+                     c->xsol, c->dp, n, ok, c->bar, c->chol_fail, gate_of(c, kGateActive));
   probe_end(c, PSBA_PROBE_FACTOR);
   c->L_cur = c->Lf;
   PSBA_LAUNCH_TRY();

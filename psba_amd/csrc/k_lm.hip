@@ -1,0 +1,191 @@
+// Levenberg-Marquardt control of one problem on the device.
+//
+// The decision of PSBA/levmar.cpp:124-245 (gain ratio :271-280, damping
+// update :202-204, nu doubling :238-243, the |dp| tests :185-197) taken by
+// one thread after each trial, so that a solve's trials are queued back to
+// back with no host round trip between them.  The state lives in LmDev; three
+// one-byte gates steer the kernels of the next trial:
+//   gate[kGateActive] the solve is running (every trial kernel),
+//   gate[kGateJac]    the trial opens an iteration (Jacobian + normal equations),
+//   gate[kGateAccept] the trial was accepted (commit of p + dp).
+// The arithmetic and the order of the tests are those of psba_levmar
+// (drivers.cpp), which follows the reference line by line; both produce the
+// same trace records.
+#include "context.hpp"
+
+// No FMA contraction in this file: the damping update and the tests round
+// exactly as psba_levmar's host code does (x86-64 without FMA), so the device
+// and host decisions agree bit for bit.
+#pragma clang fp contract(off)
+
+namespace psba {
+
+namespace {
+
+constexpr double kInitMu = 1e-3;       // PSBA/psba.h:6
+constexpr double kStopThresh = 1e-12;  // PSBA/psba.h:7
+constexpr double kEps = 1e-12;         // PSBA/psba.h:8
+
+__device__ void lm_push(LmDev* s, psba_trace_rec* tr, const psba_trace_rec& r) {
+  if (s->ntrace < s->trace_cap) tr[s->ntrace] = r;
+  s->ntrace++;
+}
+
+// end of an LM iteration (PSBA/levmar.cpp:247-250 and the loop header :103)
+__device__ void lm_end_iteration(LmDev* s, unsigned char* gate) {
+  if (s->ex_L2 <= kStopThresh) s->flag = PSBA_ITER_ERR_SMALL_ENOUGH;
+  s->itno++;
+  const bool go = s->itno < s->max_iter && s->flag == PSBA_ITER_CONTINUE;
+  gate[kGateActive] = go ? 1 : 0;
+  gate[kGateJac] = go ? 1 : 0;
+}
+
+}  // namespace
+
+// state at the entry of psba_levmar: the cost just evaluated at the current
+// parameters (scal[SC_COST]); nothing runs when the iteration budget is spent
+__global__ void k_lm_init(LmDev* s, unsigned char* gate, const double* __restrict__ scal, int itno, int max_iter,
+                          double rel_tol, int no_handoff, int trace_cap) {
+  s->ex_L2 = scal[SC_COST];
+  s->entry_cost = s->ex_L2;
+  s->mu = 0.0;
+  s->p_L2 = 0.0;
+  s->rel_tol = rel_tol;
+  s->nu = 2;
+  s->gooditer = 0;
+  s->flag = PSBA_ITER_CONTINUE;
+  s->itno = itno;
+  s->max_iter = max_iter;
+  s->first = 1;
+  s->p_pending = 0;
+  s->no_handoff = no_handoff;
+  s->ntrace = 0;
+  s->trace_cap = trace_cap;
+  s->trials = 0;
+  const bool go = itno < max_iter;
+  gate[kGateActive] = go ? 1 : 0;
+  gate[kGateJac] = go ? 1 : 0;
+  gate[kGateAccept] = 0;
+}
+
+// mu0 = tau * max(UVdiag) on the solve's first iteration (PSBA/levmar.cpp:113-120)
+__global__ __launch_bounds__(256) void k_lm_mu0(LmDev* s, const unsigned char* __restrict__ gate,
+                                                const double* __restrict__ uvdiag, long ntot) {
+  __shared__ double sh[4];
+  if (!gate[kGateActive] || !s->first) return;
+  double m = 0.0;
+  for (long q = threadIdx.x; q < ntot; q += blockDim.x) m = fmax(m, uvdiag[q]);
+#pragma unroll
+  for (int off = 32; off >= 1; off >>= 1) m = fmax(m, __shfl_xor(m, off, 64));
+  if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = m;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    m = fmax(fmax(sh[0], sh[1]), fmax(sh[2], sh[3]));
+    s->mu = kInitMu * m;
+    s->first = 0;
+    s->p_L2 = 1e+3;
+    s->nu = 2;
+  }
+}
+
+// the decision after one trial (PSBA/levmar.cpp:124-245)
+__global__ void k_lm_control(LmDev* s, unsigned char* gate, const double* __restrict__ scal,
+                             psba_trace_rec* __restrict__ trace) {
+  gate[kGateAccept] = 0;
+  if (!gate[kGateActive]) {
+    gate[kGateJac] = 0;
+    return;
+  }
+  s->trials++;
+  gate[kGateJac] = 0;
+  if (s->p_pending) {
+    s->p_L2 = scal[SC_P2];
+    s->p_pending = 0;
+  }
+  const double nan = __builtin_nan("");
+  psba_trace_rec rec{s->itno, 0, 0, 0, nan, s->mu, nan, 0.0, nan};
+  // a timed-out wait inside the dense solve is an execution error, not a
+  // rejected trial
+  if (scal[SC_FACTOR_OK] < 0.0) {
+    s->flag = PSBA_ERR_HIP;
+    gate[kGateActive] = 0;
+    return;
+  }
+  const bool ok = scal[SC_FACTOR_OK] != 0.0;
+  rec.factor_ok = ok;
+  if (ok) {
+    const double dp_L2 = scal[SC_DP2];
+    rec.dp_sq = dp_L2;
+    if (dp_L2 < s->p_L2 * kStopThresh * kStopThresh) {
+      s->flag = PSBA_ITER_DP_NO_CHANGE;
+      lm_push(s, trace, rec);
+      lm_end_iteration(s, gate);
+      return;
+    }
+    if (dp_L2 >= (s->p_L2 + kStopThresh) / (kEps * kEps)) {
+      s->flag = PSBA_ITER_ERR;
+      lm_push(s, trace, rec);
+      lm_end_iteration(s, gate);
+      return;
+    }
+    const double new_ex_L2 = scal[SC_COST];
+    const double rho = (s->ex_L2 - new_ex_L2) / scal[SC_RHODEN];
+    rec.cost = new_ex_L2;
+    rec.rho = rho;
+    if (rho > 0) {
+      double tmp = 2 * rho - 1;
+      tmp = 1.0 - tmp * tmp * tmp;
+      s->mu = s->mu * ((tmp >= (1.0 / 3.0)) ? tmp : (1.0 / 3.0));
+      s->nu = 2;
+      gate[kGateAccept] = 1;
+      s->p_pending = 1;
+      const double rel_dec = (s->ex_L2 - new_ex_L2) / s->ex_L2;
+      s->ex_L2 = new_ex_L2;
+      rec.accepted = 1;
+      lm_push(s, trace, rec);
+      if (s->rel_tol > 0 && rel_dec < s->rel_tol) {
+        s->flag = PSBA_ITER_DP_NO_CHANGE;
+      } else if (s->no_handoff) {
+        s->gooditer = 0;
+      } else if (fabs(rho - 1) < (1.0 / 5.0)) {
+        s->gooditer++;
+        if (s->gooditer >= 5) s->flag = PSBA_ITER_TURN_TO_TR;
+      } else {
+        s->gooditer = 0;
+      }
+      lm_end_iteration(s, gate);
+      return;
+    }
+  } else {
+    s->gooditer = 0;
+  }
+  lm_push(s, trace, rec);
+  s->mu *= s->nu;
+  if (s->nu >= (1 << 30)) {  // PSBA/levmar.cpp:238-243: int nu doubles until 2*nu wraps
+    s->flag = PSBA_ITER_ERR;
+    lm_end_iteration(s, gate);
+    return;
+  }
+  s->nu = 2 * s->nu;
+}
+
+int launch_lm_init(psba_ctx* c, int itno, int max_iter, double rel_tol, int no_handoff) {
+  hipLaunchKernelGGL(k_lm_init, dim3(1), dim3(1), 0, c->stream, c->lm, c->gate, c->d_scal, itno, max_iter, rel_tol,
+                     no_handoff, c->lm_trace_cap);
+  PSBA_LAUNCH_TRY();
+  return PSBA_OK;
+}
+
+int launch_lm_mu0(psba_ctx* c) {
+  hipLaunchKernelGGL(k_lm_mu0, dim3(1), dim3(256), 0, c->stream, c->lm, c->gate, c->uvdiag, (long)c->ntot);
+  PSBA_LAUNCH_TRY();
+  return PSBA_OK;
+}
+
+int launch_lm_control(psba_ctx* c) {
+  hipLaunchKernelGGL(k_lm_control, dim3(1), dim3(1), 0, c->stream, c->lm, c->gate, c->d_scal, c->lm_trace);
+  PSBA_LAUNCH_TRY();
+  return PSBA_OK;
+}
+
+}  // namespace psba

@@ -306,17 +306,16 @@ static void run_jac_normal_t(psba_ctx* c, double coeff, double coeff_g, R* e, R*
   hipLaunchKernelGGL((k_jacobian<CNP, R>), dim3(ceil_div(c->no, kJacBlock)), dim3(kJacBlock),
                      (jacobian_lds<CNP, R>(kJacBlock)), c->stream, c->K, c->q0,
                      c->cams, c->pts, c->meas, c->obs_cam, c->obs_pt, c->fix, c->no, coeff, e, JA, JB, W,
-                     (const int*)nullptr, (const unsigned char*)nullptr, c->loss, c->loss_scale);
+                     owner_of(c), gate_of(c, kGateJac), c->loss, c->loss_scale);
   probe_end(c, PSBA_PROBE_JACOBIAN);
   hipLaunchKernelGGL(k_point_normal<R>, dim3(ceil_div(4L * c->np, 256)), dim3(256), 0, c->stream, c->pt_ptr, JB, e,
-                     c->np, coeff, coeff_g, c->V, c->g + c->n, c->uvdiag + c->n, (const int*)nullptr,
-                     (const unsigned char*)nullptr);
+                     c->np, coeff, coeff_g, c->V, c->g + c->n, c->uvdiag + c->n, owner_of(c),
+                     gate_of(c, kGateJac));
   if (c->n_cchunk > 0)
     hipLaunchKernelGGL((k_cam_chunks<CNP, R>), dim3(c->n_cchunk), dim3(256), 0, c->stream, c->cam_obs,
-                       c->cchunk_begin, c->cchunk_end, JA, e, ucpart, (const int*)nullptr,
-                       (const unsigned char*)nullptr);
+                       c->cchunk_begin, c->cchunk_end, JA, e, ucpart, owner_of(c), gate_of(c, kGateJac));
   hipLaunchKernelGGL((k_cam_finish<CNP, R>), dim3(c->nc), dim3(256), 0, c->stream, c->cam_cchunk_ptr, ucpart, c->nc,
-                     coeff, coeff_g, c->U, c->g, c->uvdiag, (const int*)nullptr, (const unsigned char*)nullptr);
+                     coeff, coeff_g, c->U, c->g, c->uvdiag, owner_of(c), gate_of(c, kGateJac));
 }
 
 template <int CNP>
@@ -417,8 +416,10 @@ __global__ __launch_bounds__(256) void k_vinv(const double* __restrict__ V, cons
 }
 
 int launch_vinv(psba_ctx* c, double mu) {
+  // in device-control mode the damping comes from the LM state and the kernel
+  // is gated with the solve
   hipLaunchKernelGGL(k_vinv, dim3(ceil_div(c->np, 256)), dim3(256), 0, c->stream, c->V, c->g + c->n, c->np, mu,
-                     c->Vinv, c->z, (const double*)nullptr, (const int*)nullptr, (const unsigned char*)nullptr);
+                     c->Vinv, c->z, mu_of(c), owner_of(c), gate_of(c, kGateActive));
   PSBA_LAUNCH_TRY();
   return PSBA_OK;
 }
@@ -513,9 +514,9 @@ static void run_backsub(psba_ctx* c) {
   if (c->mixed())
     // (FP32 W rows: the id order's locality is worth more than the balance,
     // 33 against 64-71 us at the 88-camera scene)
-    backsub_launch<CNP, float>(c->stream, c->pt_ptr, c->obs_cam, c->W32, c->Vinv, c->g + c->n, c->np, c->n, c->dp, c->eb, (const int*)nullptr, (const unsigned char*)nullptr, 256);
+    backsub_launch<CNP, float>(c->stream, c->pt_ptr, c->obs_cam, c->W32, c->Vinv, c->g + c->n, c->np, c->n, c->dp, c->eb, owner_of(c), gate_of(c, kGateActive), 256);
   else
-    backsub_launch<CNP, double>(c->stream, c->pt_ptr, c->obs_cam, c->W, c->Vinv, c->g + c->n, c->np, c->n, c->dp, c->eb, (const int*)nullptr, (const unsigned char*)nullptr, 256, c->pt_order);
+    backsub_launch<CNP, double>(c->stream, c->pt_ptr, c->obs_cam, c->W, c->Vinv, c->g + c->n, c->np, c->n, c->dp, c->eb, owner_of(c), gate_of(c, kGateActive), 256, c->pt_order);
 }
 
 int launch_backsub(psba_ctx* c) {
@@ -528,9 +529,11 @@ int launch_backsub(psba_ctx* c) {
 // step statistics, commit, dot products
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(256) void k_step_stats(const double* __restrict__ dp, const double* __restrict__ g,
-                                                    long ntot, double mu, double* partials) {
+                                                    long ntot, double mu, const double* __restrict__ mu_p,
+                                                    double* partials) {
   __shared__ double sh[8];
   double acc[2] = {0.0, 0.0};
+  if (mu_p) mu = *mu_p;
   long q = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (q < ntot) {
     double d = dp[q];
@@ -542,14 +545,19 @@ __global__ __launch_bounds__(256) void k_step_stats(const double* __restrict__ d
 
 int launch_step_stats(psba_ctx* c, double mu) {
   const unsigned nb = ceil_div(c->ntot, 256);
-  hipLaunchKernelGGL(k_step_stats, dim3(nb), dim3(256), 0, c->stream, c->dp, c->g, (long)c->ntot, mu, c->red_part);
+  hipLaunchKernelGGL(k_step_stats, dim3(nb), dim3(256), 0, c->stream, c->dp, c->g, (long)c->ntot, mu, mu_of(c),
+                     c->red_part);
   launch_sum_partials<2>(c->stream, c->red_part, (int)nb, c->d_scal + SC_DP2);
   PSBA_LAUNCH_TRY();
   return PSBA_OK;
 }
 
+// With `gate`, nothing is written unless *gate is set (the LM control accepted
+// the trial); the |p|^2 partials are then left as they were.
 __global__ __launch_bounds__(256) void k_commit(double* __restrict__ cams, double* __restrict__ pts,
-                                                const double* __restrict__ dp, int n, long ntot, double* partials) {
+                                                const double* __restrict__ dp, int n, long ntot, double* partials,
+                                                const unsigned char* __restrict__ gate) {
+  if (gate && !*gate) return;
   __shared__ double sh[4];
   double acc[1] = {0.0};
   long q = (long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -570,8 +578,14 @@ __global__ __launch_bounds__(256) void k_commit(double* __restrict__ cams, doubl
 int launch_commit(psba_ctx* c) {
   const unsigned nb = ceil_div(c->ntot, 256);
   hipLaunchKernelGGL(k_commit, dim3(nb), dim3(256), 0, c->stream, c->cams, c->pts, c->dp, c->n, (long)c->ntot,
-                     c->red_part);
-  launch_sum_partials<1>(c->stream, c->red_part, (int)nb, c->d_scal + SC_P2);
+                     c->red_part, gate_of(c, kGateAccept));
+  if (c->devctl) {
+    // the partials of a gated-off commit are stale: sum them only after an accept
+    hipLaunchKernelGGL(k_sum_partials_gated<1>, dim3(1), dim3(256), 0, c->stream, c->red_part, (int)nb,
+                       c->d_scal + SC_P2, gate_of(c, kGateAccept));
+  } else {
+    launch_sum_partials<1>(c->stream, c->red_part, (int)nb, c->d_scal + SC_P2);
+  }
   PSBA_LAUNCH_TRY();
   return PSBA_OK;
 }

@@ -134,10 +134,12 @@ template <int CNP, typename R>
 __global__ __launch_bounds__(256) void k_schur_finish(const int* __restrict__ lblock_pair,
                                                       const int* __restrict__ psp, const R* __restrict__ part,
                                                       const double* __restrict__ U,
-                                                      const unsigned char* __restrict__ fix, double mu, int ld,
-                                                      int nlb, int n, double* __restrict__ S) {
+                                                      const unsigned char* __restrict__ fix, double mu,
+                                                      const double* __restrict__ mu_p, int ld, int nlb, int n,
+                                                      double* __restrict__ S) {
   constexpr int C2 = CNP * CNP;
   const int b = blockIdx.x;
+  if (mu_p) mu = *mu_p;
   if (b >= nlb) {
     const int r = n + (b - nlb);
     for (int q = threadIdx.x; q <= r; q += blockDim.x) S[(size_t)r * ld + q] = (q == r) ? 1.0 : 0.0;
@@ -228,18 +230,18 @@ static void run_schur_t(psba_ctx* c, double mu, const R* W, R* schpart, R* eapar
     probe_begin(c, PSBA_PROBE_SCHUR);
     hipLaunchKernelGGL((k_schur_chunks<CNP, R>), dim3(PSBA_SCHUR_XCD ? xcd_grid(c->n_schunk) : c->n_schunk), dim3(256),
                        0, c->stream, c->pair_items, c->schunk_begin, c->schunk_end, c->obs_pt, W, c->Vinv, schpart,
-                       (const int*)nullptr, (const unsigned char*)nullptr, c->n_schunk, (const int*)c->schunk_order);
+                       owner_of(c), gate_of(c, kGateActive), c->n_schunk, (const int*)c->schunk_order);
     probe_end(c, PSBA_PROBE_SCHUR);
   }
   hipLaunchKernelGGL((k_schur_finish<CNP, R>), dim3(c->n_lblocks + (c->n_pad - c->n)),
                      dim3(CNP * CNP > 128 ? 256 : 128), 0, c->stream, c->lblock_pair, c->pair_schunk_ptr, schpart, c->U,
-                     c->fix, mu, c->n_pad, c->n_lblocks, c->n, c->S);
+                     c->fix, mu, mu_of(c), c->n_pad, c->n_lblocks, c->n, c->S);
   if (c->n_cchunk > 0)
     hipLaunchKernelGGL((k_ea_chunks<CNP, R>), dim3(c->n_cchunk), dim3(256), 0, c->stream, c->cam_obs,
-                       c->cchunk_begin, c->cchunk_end, c->obs_pt, W, c->z, eapart, (const int*)nullptr,
-                       (const unsigned char*)nullptr);
+                       c->cchunk_begin, c->cchunk_end, c->obs_pt, W, c->z, eapart, owner_of(c),
+                       gate_of(c, kGateActive));
   hipLaunchKernelGGL((k_ea_finish<CNP, R>), dim3(ceil_div((long)c->nc * CNP, 64)), dim3(64), 0, c->stream,
-                     c->cam_cchunk_ptr, eapart, c->g, c->nc, c->ea, (const int*)nullptr, (const unsigned char*)nullptr);
+                     c->cam_cchunk_ptr, eapart, c->g, c->nc, c->ea, owner_of(c), gate_of(c, kGateActive));
 }
 
 template <int CNP>

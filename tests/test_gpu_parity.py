@@ -377,3 +377,52 @@ def test_vinv_small_determinant_branch_on_device(P, O, gpu):
                 if mu == 0.0 and i < len(HAND):
                     M, sign = HAND[i]
                     assert np.allclose(out[i], sign * np.linalg.inv(V[i]), rtol=1e-12, atol=0)
+
+
+DEVCTL_CASES = [(0, {}, dict()), (0, {}, dict(max_iter=3)), (1, dict(n_pts=1500), dict()),
+                (2, dict(n_pts=6000, n_cams=30), dict(max_iter=100, rel_tol=1e-8)),
+                (2, dict(n_pts=6000, n_cams=30), dict(max_iter=100, rel_tol=1e-8, mixed=True))]
+
+
+@pytest.mark.parametrize("config,kw,opts", DEVCTL_CASES, ids=["c0", "c0-3it", "c1", "c2-rel", "c2-mixed"])
+def test_device_control_matches_host_control(P, gpu, config, kw, opts):
+    """psba_levmar with the decision on the device (the default: trials queued
+    back to back, no read-back per trial) against the same driver deciding on
+    the host after each trial: the same trace record for record, bit for bit,
+    and the same parameters."""
+    opts = dict(opts)
+    mixed = opts.pop("mixed", False)
+    sc = P.Scene(config, 3000 + config, **kw)
+    prec = P.PREC_MIXED if mixed else P.PREC_FP64
+    flags = P.FLAG_NO_HANDOFF if "rel_tol" in opts else 0
+    out = []
+    for extra in (0, P.FLAG_HOST_CONTROL):
+        with P.Context.from_scene(sc, precision=prec) as g:
+            r = g.levmar(flags=flags | extra, **opts)
+            out.append((r, np.concatenate(g.read_params()), g.levmar_entry_cost()))
+    (rd, pd, ed), (rh, ph, eh) = out
+    assert (rd["code"], rd["itno"], rd["final_err"], ed) == (rh["code"], rh["itno"], rh["final_err"], eh)
+    assert len(rd["trace"]) == len(rh["trace"]) > 0
+    for a, b in zip(rd["trace"], rh["trace"]):
+        fa = [getattr(a, f) for f, _ in a._fields_]
+        fb = [getattr(b, f) for f, _ in b._fields_]
+        assert np.array_equal(np.array(fa, dtype=float), np.array(fb, dtype=float), equal_nan=True), (fa, fb)
+    assert np.array_equal(pd, ph)
+
+
+def test_device_control_handoff_and_hybrid(P, O, gpu):
+    """Without PSBA_FLAG_NO_HANDOFF the device decision hands over to the dogleg
+    (ITER_TURN_TO_TR after five good steps) exactly as the host loop does, and
+    the hybrid solve through it matches the oracle."""
+    sc = fix_gauge(P.Scene(0, 4000))
+    with P.Context.from_scene(sc) as g:
+        r = g.levmar()
+    with P.Context.from_scene(sc) as g:
+        rh = g.levmar(flags=P.FLAG_HOST_CONTROL)
+    assert r["code"] == rh["code"] == P.ITER_TURN_TO_TR
+    assert r["itno"] == rh["itno"] and r["final_err"] == rh["final_err"]
+    g, o = pair(P, O, sc)
+    rg = g.hybrid()
+    ro = o.hybrid()
+    assert (rg["code"], rg["itno"]) == (ro["code"], ro["itno"])
+    assert abs(rg["final_err"] - ro["final_err"]) <= 1e-9 * ro["final_err"]
