@@ -178,3 +178,35 @@ def test_mixed_sizes(P, gpu):
     c.close()
     assert (mix[-1]["code"], mix[-1]["itno"]) == (g["code"], g["itno"])
     assert abs(mix[-1]["final_err"] - g["final_err"]) <= 1e-9 * g["final_err"]
+
+
+def test_config3_full_batch(P, O, gpu):
+    """The stated batch: 1,024 windows (5-30 cameras, 200-4,000 points) in one
+    batch.  Sixteen sampled windows solved alone give the same result bit for
+    bit, and eight of them match the oracle's LM on stop code, iteration count
+    and final cost within 1e-9."""
+    scenes = [window(P, k) for k in range(1024)]
+    b = P.Batch(6)
+    for sc in scenes:
+        b.add_scene(sc)
+    b.commit()
+    b.solve(**OPTS)
+    rng = np.random.default_rng(5)
+    sample = sorted(rng.choice(1024, 16, replace=False).tolist())
+    res = {k: b.result(k) for k in sample}
+    b.close()
+    assert sum(res[k]["final_err"] < res[k]["init_err"] for k in sample) == 16
+    one = P.Batch(6)
+    for k in sample:
+        one.reset()
+        one.add_scene(scenes[k])
+        one.commit()
+        one.solve(**OPTS)
+        same(one.result(0), res[k])
+    one.close()
+    O.set_threads(16)
+    for k in sample[:8]:
+        o = O.Oracle.from_scene(scenes[k]).levmar(max_iter=100, flags=O.FLAG_NO_HANDOFF, rel_tol=1e-8)
+        r = res[k]
+        assert (r["code"], r["itno"]) == (o["code"], o["itno"]), k
+        assert abs(r["final_err"] - o["final_err"]) <= 1e-9 * o["final_err"], k

@@ -102,12 +102,16 @@ def compare_traces(tg, to, tol=1e-9, damp_tol=1e-6):
         assert close(a.damp, b.damp, damp_tol), (a.itno, a.damp, b.damp)
 
 
-LM_SCENES = [(0, {}), (1, dict(n_pts=1500)), (1, dict(cnp=16, n_pts=1500)), (2, dict(n_pts=6000, n_cams=30))]
+LM_SCENES = [(0, {}), (1, dict(n_pts=1500)), (1, dict(cnp=16, n_pts=1500)), (2, dict(n_pts=6000, n_cams=30)),
+             (1, {}), (1, dict(cnp=16))]
 
 
-@pytest.mark.parametrize("config,kw", LM_SCENES, ids=IDS)
+@pytest.mark.parametrize("config,kw", LM_SCENES, ids=IDS + ["c1-cnp11-full", "c1-cnp16-full"])
 def test_levmar_trial_by_trial(P, O, gpu, config, kw):
     sc = P.Scene(config, 2000 + config, **kw)
+    if not kw and config == 1:
+        assert (sc.n_cams, sc.n_pts) == (54, 5200)  # the stated size of the variable-intrinsics scene
+    O.set_threads(16)
     g, o = pair(P, O, sc)
     rg = g.levmar()
     ro = o.levmar()
@@ -151,12 +155,14 @@ def test_levmar_with_outliers_first_iterations(P, O, gpu):
 # steps that agree to about cond * eps, so the free-intrinsics scene is held
 # to 1e-7 instead of 1e-9.
 TR_SCENES = [(0, {}, 1e-9), (1, dict(n_pts=800, n_cams=20), 1e-7),
-             (2, dict(n_pts=3000, n_cams=20, outlier_frac=0.0), 1e-9)]
-TR_IDS = ["c0-cnp6", "c1-cnp11", "c2-cnp9"]
+             (2, dict(n_pts=3000, n_cams=20, outlier_frac=0.0), 1e-9),
+             (1, {}, 1e-7), (1, dict(cnp=16), 1e-7)]
+TR_IDS = ["c0-cnp6", "c1-cnp11", "c2-cnp9", "c1-cnp11-full", "c1-cnp16-full"]
 
 
 @pytest.mark.parametrize("config,kw,tol", TR_SCENES, ids=TR_IDS)
 def test_dogleg_after_lm_with_gauge_fixed(P, O, gpu, config, kw, tol):
+    O.set_threads(16)
     sc = fix_gauge(P.Scene(config, 3000 + config, **kw))
     g, o = pair(P, O, sc)
     g.levmar()

@@ -107,3 +107,28 @@ def test_budget(P, gpu):
     assert res[3]["code"] == -3  # PSBA_ERR_NOMEM
     for sc, r in zip(small, res[:3] + res[4:]):
         same(r, alone(P, sc))
+
+
+@pytest.mark.parametrize("precision", [0, 1], ids=["fp64", "mixed"])
+def test_full_size_classes_alone_and_in_mix(P, gpu, precision):
+    """The stream's medium (50 cameras, 20,000 points) and large (138 cameras,
+    19,878 points, heavy-tailed tracks) requests at their stated sizes,
+    arriving among small ones while those run: each result equals the same
+    request solved alone, bit for bit."""
+    scenes = [req(P, 500 + k) for k in range(6)] + [req(P, 600, 1), req(P, 601, 1), req(P, 700, 2)]
+    assert (scenes[6].n_cams, scenes[6].n_pts) == (50, 20000)
+    assert (scenes[8].n_cams, scenes[8].n_pts) == (138, 19878)
+    srv = P.Server(6, precision=precision, lanes=3, max_batch=4)
+    ids, out = {}, {}
+    for n, k in enumerate([0, 6, 1, 8, 2, 3, 7, 4, 5]):
+        ids[k] = srv.submit_scene(scenes[k])
+        if n % 2 == 1:
+            srv.step()
+            collect(P, srv, ids.values(), out)
+    srv.drain()
+    collect(P, srv, ids.values(), out)
+    st = srv.stats()
+    srv.close()
+    assert st["completed"] == len(scenes) and st["rejected"] == 0
+    for k, sc in enumerate(scenes):
+        same(out[ids[k]], alone(P, sc, precision))
