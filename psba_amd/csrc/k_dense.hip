@@ -17,60 +17,6 @@ namespace psba {
 constexpr int NB = kNB;   // 64
 constexpr int TP = NB + 1; // padded LDS row length
 
-// ---------------------------------------------------------------------------
-// Tile building blocks (4 waves, lane = row, wave W owns columns 16W..16W+15).
-// ---------------------------------------------------------------------------
-// Diagonal tile: 4 waves, lane = row, wave W owns columns 16W..16W+15 of all
-// 64 rows (registers); the pivot column goes through double-buffered LDS with
-// one barrier per pivot.  The wave index is a template parameter so every
-// column test is resolved at compile time (no branches around the updates).
-template <int W>
-__device__ __forceinline__ void potrf_wave(double (&a)[16], double (*col)[NB], int lane, int* bad) {
-#pragma unroll 1
-  for (int kb = 0; kb < 4; ++kb) {
-#pragma unroll
-    for (int kk = 0; kk < 16; ++kk) {
-      const int k = 16 * kb + kk;
-      double* cb = col[kk & 1];
-      if (kb == W) {
-        const double dkk = __shfl(a[kk], k, 64);
-        const double lkk = sqrt(dkk);
-        const double v = (lane == k) ? lkk : a[kk] / lkk;
-        a[kk] = v;
-        cb[lane] = v;
-        if (lane == 0 && !(dkk > 0.0 && isfinite(dkk))) *bad = 1;
-      }
-      __syncthreads();
-      const double lik = cb[lane];
-      if (W > kb) {
-#pragma unroll
-        for (int q = 0; q < 16; ++q) a[q] -= lik * cb[16 * W + q];
-      } else if (W == kb) {
-#pragma unroll
-        for (int q = kk + 1; q < 16; ++q) a[q] -= lik * cb[16 * W + q];
-      }
-    }
-  }
-}
-
-template <int W>
-__device__ __forceinline__ void trsm_wave(double (&x)[16], double (*col)[NB], const double* L, const double* rd,
-                                          int lane) {
-#pragma unroll
-  for (int c = 0; c < NB; ++c) {
-    double* cb = col[c & 1];
-    if (W == (c >> 4)) {
-      x[c & 15] *= rd[c];
-      cb[lane] = x[c & 15];
-    }
-    __syncthreads();
-    const double xc = cb[lane];
-#pragma unroll
-    for (int q = 0; q < 16; ++q)
-      if (16 * W + q > c) x[q] -= xc * L[(16 * W + q) * TP + c];
-  }
-}
-
 __global__ void k_copy_mat(double* __restrict__ dst, const double* __restrict__ src, long count) {
   long q = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (q < count) dst[q] = src[q];

@@ -511,12 +511,14 @@ static void backsub_launch(hipStream_t st, const int* pt_ptr, const int* obs_cam
 
 template <int CNP>
 static void run_backsub(psba_ctx* c) {
+  probe_begin(c, PSBA_PROBE_BSOLVE);
   if (c->mixed())
     // (FP32 W rows: the id order's locality is worth more than the balance,
     // 33 against 64-71 us at the 88-camera scene)
     backsub_launch<CNP, float>(c->stream, c->pt_ptr, c->obs_cam, c->W32, c->Vinv, c->g + c->n, c->np, c->n, c->dp, c->eb, owner_of(c), gate_of(c, kGateActive), 256);
   else
     backsub_launch<CNP, double>(c->stream, c->pt_ptr, c->obs_cam, c->W, c->Vinv, c->g + c->n, c->np, c->n, c->dp, c->eb, owner_of(c), gate_of(c, kGateActive), 256, c->pt_order);
+  probe_end(c, PSBA_PROBE_BSOLVE);
 }
 
 int launch_backsub(psba_ctx* c) {

@@ -154,38 +154,79 @@ def test_levmar_with_outliers_first_iterations(P, O, gpu):
 # ~4e10 with all five intrinsics free; two correct FP64 factorizations give
 # steps that agree to about cond * eps, so the free-intrinsics scene is held
 # to 1e-7 instead of 1e-9.
+# At the stated size of the variable-intrinsics scene (54 cameras, 5,200
+# points) the dogleg's reduced systems are worse conditioned still: the
+# oracle run with every reduction in descending order (a second valid FP64
+# evaluation) differs from itself by 6e-8 to 3e-6 relative in accepted costs
+# and up to 3e-6 in final parameters, with identical decisions.  Those cases
+# (tol None) are held to ten times the oracle's own spread on the same scene.
 TR_SCENES = [(0, {}, 1e-9), (1, dict(n_pts=800, n_cams=20), 1e-7),
              (2, dict(n_pts=3000, n_cams=20, outlier_frac=0.0), 1e-9),
-             (1, {}, 1e-7), (1, dict(cnp=16), 1e-7)]
+             (1, {}, None), (1, dict(cnp=16), None)]
 TR_IDS = ["c0-cnp6", "c1-cnp11", "c2-cnp9", "c1-cnp11-full", "c1-cnp16-full"]
+
+
+def order_spread(O, sc, run):
+    """Relative spread of the oracle against itself with every reduction in
+    descending index order: (max over accepted trials of the cost difference,
+    final-parameter difference); the decisions must agree."""
+    out = []
+    try:
+        for desc in (False, True):
+            O.set_sum_order(desc)
+            o = O.Oracle.from_scene(sc, fix=True)
+            r = run(o)
+            out.append((r, np.concatenate(o.read_params())))
+    finally:
+        O.set_sum_order(False)
+    (ra, pa), (rb, pb) = out
+    assert [(a.itno, a.kind, a.factor_ok, a.accepted) for a in ra["trace"]] == \
+        [(b.itno, b.kind, b.factor_ok, b.accepted) for b in rb["trace"]]
+    dc = max([abs(a.cost - b.cost) / abs(b.cost) for a, b in zip(ra["trace"], rb["trace"]) if a.accepted] + [0.0])
+    return dc, rel(pa, pb)
+
+
+def dogleg_tolerances(O, sc, tol, run):
+    if tol is not None:
+        return tol, tol
+    dc, dp = order_spread(O, sc, run)
+    return max(1e-9, 10 * dc), max(1e-9, 10 * dp)
 
 
 @pytest.mark.parametrize("config,kw,tol", TR_SCENES, ids=TR_IDS)
 def test_dogleg_after_lm_with_gauge_fixed(P, O, gpu, config, kw, tol):
     O.set_threads(16)
     sc = fix_gauge(P.Scene(config, 3000 + config, **kw))
+
+    def lm_then_tr(o):
+        o.levmar()
+        return o.trust_region()
+
+    tc, tp = dogleg_tolerances(O, sc, tol, lm_then_tr)
     g, o = pair(P, O, sc)
     g.levmar()
     o.levmar()
     rg = g.trust_region()
     ro = o.trust_region()
     assert (rg["code"], rg["itno"]) == (ro["code"], ro["itno"])
-    compare_traces(rg["trace"], ro["trace"], tol=tol)
+    compare_traces(rg["trace"], ro["trace"], tol=tc)
     for a, b in zip(rg["trace"], ro["trace"]):
-        assert close(a.lambda_, b.lambda_, tol) or a.lambda_ == b.lambda_
-    assert rel(np.concatenate(g.read_params()), np.concatenate(o.read_params())) < tol
+        assert close(a.lambda_, b.lambda_, tc) or a.lambda_ == b.lambda_
+    assert rel(np.concatenate(g.read_params()), np.concatenate(o.read_params())) < tp
 
 
 @pytest.mark.parametrize("config,kw,tol", TR_SCENES, ids=TR_IDS)
 def test_hybrid_with_gauge_fixed(P, O, gpu, config, kw, tol):
+    O.set_threads(16)
     sc = fix_gauge(P.Scene(config, 4000 + config, **kw))
+    tc, tp = dogleg_tolerances(O, sc, tol, lambda o: o.hybrid())
     g, o = pair(P, O, sc)
     rg = g.hybrid()
     ro = o.hybrid()
     assert (rg["code"], rg["itno"]) == (ro["code"], ro["itno"])
-    compare_traces(rg["trace"], ro["trace"], tol=tol)
-    assert rel(rg["final_err"], ro["final_err"]) < tol
-    assert rel(np.concatenate(g.read_params()), np.concatenate(o.read_params())) < tol
+    compare_traces(rg["trace"], ro["trace"], tol=tc)
+    assert rel(rg["final_err"], ro["final_err"]) < tc
+    assert rel(np.concatenate(g.read_params()), np.concatenate(o.read_params())) < tp
 
 
 def test_dogleg_from_the_start(P, O, gpu):
