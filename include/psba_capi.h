@@ -128,6 +128,21 @@ int psba_generate_idxs(int nCams, int n3Dpts, int n2Dprojs, const char* vmask, i
 int psba_generate_idxs_device(int device, int nCams, int n3Dpts, int n2Dprojs, const char* vmask,
                               int* comm3DIdxCnt, int* iidx, int* jidx, int* blk_idx);
 
+/* generate_idxs with no allocation (no counterpart in the reference: for
+ * callers that cannot allocate, such as code under HIP stream capture or
+ * serving loops that forbid hipMalloc).  Device pointers in and out: d_vmask
+ * (nCams*n3Dpts bytes), d_iidx and d_jidx (n2Dprojs ints), d_blk_idx
+ * (nCams*n3Dpts ints), d_count (1 int, may be NULL: the number of visible
+ * entries; iidx/jidx entries beyond n2Dprojs are not written and their
+ * blk_idx is -1).  Runs on `stream` (a hipStream_t, NULL for the null
+ * stream) without synchronising; the results equal psba_generate_idxs_device's
+ * (comm3DIdxCnt is not produced).  Its only intermediate lives in a fixed
+ * device array of 65,537 ints per device, so nCams*n3Dpts is limited to
+ * 67,108,864 (PSBA_ERR_ARG beyond); calls that overlap on one device race. */
+int psba_generate_idxs_device_noalloc(int device, void* stream, int nCams, int n3Dpts, int n2Dprojs,
+                                      const char* d_vmask, int* d_iidx, int* d_jidx, int* d_blk_idx,
+                                      int* d_count);
+
 /* ---- drivers ---------------------------------------------------------------- */
 /* replaces levmar, PSBA/levmar.h:6-11.  *itno is the shared iteration counter
  * (the reference's global itno, PSBA/main.cpp:27); *finalErr receives the

@@ -24,7 +24,7 @@ ARCH = os.environ.get("PSBA_OFFLOAD_ARCH", "gfx950")
 HIPCC = os.environ.get("HIPCC", shutil.which("hipcc") or "/opt/rocm/bin/hipcc")
 
 SOURCES = ["ctx.cpp", "drivers.cpp", "dropin.cpp", "scene.cpp", "sba_io.cpp", "batch.cpp", "server.cpp",
-           "k_obs.hip", "k_schur.hip", "k_dense.hip", "k_tr.hip", "k_batch.hip", "k_index.hip", "k_lm.hip"]
+           "k_obs.hip", "k_schur.hip", "k_dense.hip", "k_tr.hip", "k_batch.hip", "k_index.hip", "k_lm.hip", "k_noalloc.hip"]
 
 HIP_FLAGS = [
     "-x", "hip", f"--offload-arch={ARCH}", "-O3", "-fPIC", "-std=c++17",
