@@ -462,6 +462,7 @@ def main():
         "vs_baseline": None,
         "dtype": "f64",
         "data": "synthetic (seeded scene generator, config 2)",
+        "device": P.device_info(0),
         "config": {"workload": "config 2: 88 cams (f,k1,k2+pose), %d pts, %d obs, LM to rel. decrease < 1e-8, "
                                "plain FP64" % (sc.n_pts, sc.n_obs),
                    "n_obs": sc.n_obs, "sum_t2": sc.sum_t2, "lm_iterations": r["itno"], "lm_trials": trials,

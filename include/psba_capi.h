@@ -115,6 +115,9 @@ int psba_restore_params(psba_ctx* ctx);
 const char* psba_strerror(int code);
 const char* psba_last_error(void);
 int psba_device_count(void);
+/* name (with the architecture), compute units, peak clock and memory of a
+ * device; any output may be NULL */
+int psba_device_info(int device, char* name, int name_cap, int* cus, int* clock_mhz, long long* mem_bytes);
 
 /* ---- host integer indexing ------------------------------------------------- */
 /* replaces generate_idxs, PSBA/misc.h:22-23 (PSBA/misc.cpp:178-218); bit-exact

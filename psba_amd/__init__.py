@@ -155,6 +155,8 @@ def lib():
         "psba_strerror": (ct.c_char_p, [ct.c_int]),
         "psba_last_error": (ct.c_char_p, []),
         "psba_device_count": (ct.c_int, []),
+        "psba_device_info": (ct.c_int, [ct.c_int, ct.c_char_p, ct.c_int, ct.POINTER(ct.c_int), ct.POINTER(ct.c_int),
+                                        ct.POINTER(ct.c_longlong)]),
         "psba_generate_idxs": (ct.c_int, [ct.c_int, ct.c_int, ct.c_int, ct.c_char_p, ip, ip, ip, ip, ip]),
         "psba_levmar": (ct.c_int, [vp, ct.POINTER(Opts), ip, dp, ct.POINTER(TraceRec), ct.c_int, ip]),
         "psba_trust_region": (ct.c_int, [vp, ct.POINTER(Opts), ip, dp, ct.POINTER(TraceRec), ct.c_int, ip]),
@@ -242,6 +244,14 @@ def _check(rc, what):
 
 def device_count():
     return lib().psba_device_count()
+
+
+def device_info(device=0):
+    """dict(name, cus, clock_mhz, mem_bytes) of a HIP device."""
+    name = ct.create_string_buffer(256)
+    cus, clk, mem = ct.c_int(), ct.c_int(), ct.c_longlong()
+    _check(lib().psba_device_info(device, name, 256, ct.byref(cus), ct.byref(clk), ct.byref(mem)), "psba_device_info")
+    return dict(name=name.value.decode(), cus=cus.value, clock_mhz=clk.value, mem_bytes=mem.value)
 
 
 def read_sba(cams_path, pts_path, cnp=11, pnp=3, mnp=2, quat_filter=True, filecnp=None):

@@ -197,7 +197,7 @@ int alloc_index(psba_ctx* c, long nitems, int npairs, int n_cchunk, int n_schunk
 // ---- device-side LM control (k_lm.hip) ----
 int launch_lm_init(psba_ctx* c, int itno, int max_iter, double rel_tol, int no_handoff);
 int launch_lm_mu0(psba_ctx* c);
-int launch_lm_control(psba_ctx* c);
+int launch_lm_control(psba_ctx* c, LmDev* out);  // out: the state after the decision (pinned host slot)
 // gates of the launchers in device-control mode (nullptr otherwise)
 inline const unsigned char* gate_of(const psba_ctx* c, int which) { return c->devctl ? c->gate + which : nullptr; }
 inline const int* owner_of(const psba_ctx* c) { return c->devctl ? c->zero_owner : nullptr; }

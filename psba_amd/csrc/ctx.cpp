@@ -145,6 +145,19 @@ int psba_device_count(void) {
   return n;
 }
 
+int psba_device_info(int device, char* name, int name_cap, int* cus, int* clock_mhz, long long* mem_bytes) {
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess || n == 0) return fail(PSBA_ERR_NODEVICE, "no HIP device visible");
+  if (device < 0 || device >= n) return fail(PSBA_ERR_ARG, "device out of range");
+  hipDeviceProp_t p;
+  PSBA_HIP_TRY(hipGetDeviceProperties(&p, device));
+  if (name && name_cap > 0) snprintf(name, (size_t)name_cap, "%s (%s)", p.name, p.gcnArchName);
+  if (cus) *cus = p.multiProcessorCount;
+  if (clock_mhz) *clock_mhz = p.clockRate / 1000;
+  if (mem_bytes) *mem_bytes = (long long)p.totalGlobalMem;
+  return PSBA_OK;
+}
+
 int psba_create(const psba_desc* d, psba_ctx** out) {
   if (!d || !out) return fail(PSBA_ERR_ARG, "null argument");
   *out = nullptr;

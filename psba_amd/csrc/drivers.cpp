@@ -119,14 +119,18 @@ int jacobian(psba_ctx* c, double coeff, double coeff_g) {
 // ---------------------------------------------------------------------------
 // psba_levmar with the decision on the device (k_lm.hip).  A trial is one
 // fixed sequence of launches -- Jacobian + normal equations (gated: only when
-// the trial opens an iteration), mu0 (first iteration), V^-1, Schur
+// the trial opens an iteration), mu0 (the solve's first trial), V^-1, Schur
 // complement, dense factor + solve, back substitution, step statistics,
-// trial cost, the control kernel, the commit (gated: accepted trials) -- and
-// an asynchronous copy of the LM state to one of two pinned slots.  Two
+// trial cost, the control kernel (which also writes the LM state to one of
+// two pinned host slots), the commit (gated: accepted trials).  Two
 // trials are kept queued: while the GPU runs trial t+1 the host waits for
 // trial t's event and reads its state, so the device never waits for the
-// host.  The trial behind the one that stopped the solve finds the gates off
-// and runs as no-ops.  The trace records are read back once, at the end.
+// host.  Only the kernels whose work depends on the decision are gated (the
+// Jacobian stage, mu0, the commit): a gate read in front of every workgroup
+// of the latency-bound Schur and dense-solve kernels cost more over a solve
+// (14 us a trial at the 88-camera scene) than the one trial queued behind the
+// stop, which runs those kernels to no effect.  The trace records are read
+// back once, at the end.
 // ---------------------------------------------------------------------------
 constexpr int kLmDepth = 2;
 
@@ -153,18 +157,17 @@ int lm_device_alloc(psba_ctx* c, int trace_cap) {
   return PSBA_OK;
 }
 
-int lm_enqueue_trial(psba_ctx* c, int slot) {
+int lm_enqueue_trial(psba_ctx* c, int slot, bool first) {
   PSBA_TRY(launch_jacobian_normal(c, 1.0, 1.0));
-  PSBA_TRY(launch_lm_mu0(c));
+  if (first) PSBA_TRY(launch_lm_mu0(c));
   PSBA_TRY(launch_vinv(c, 0.0));
   PSBA_TRY(launch_schur(c, 0.0));
   PSBA_TRY(launch_factor_solve(c));
   PSBA_TRY(launch_backsub(c));
   PSBA_TRY(launch_step_stats(c, 0.0));
   PSBA_TRY(launch_residual(c, true));
-  PSBA_TRY(launch_lm_control(c));
+  PSBA_TRY(launch_lm_control(c, c->h_lm + slot));
   PSBA_TRY(launch_commit(c));
-  PSBA_HIP_TRY(hipMemcpyAsync(c->h_lm + slot, c->lm, sizeof(LmDev), hipMemcpyDeviceToHost, c->stream));
   PSBA_HIP_TRY(hipEventRecord(c->ev_lm[slot], c->stream));
   return PSBA_OK;
 }
@@ -191,7 +194,7 @@ int lm_device(psba_ctx* c, const psba_opts* o, int* itno, double* finalErr, psba
     if (*itno < max_iter) {
       for (;;) {
         while (launched - checked < kLmDepth && launched < max_trials) {
-          PSBA_TRY(lm_enqueue_trial(c, (int)(launched % kLmDepth)));
+          PSBA_TRY(lm_enqueue_trial(c, (int)(launched % kLmDepth), launched == 0));
           launched++;
         }
         const int slot = (int)(checked % kLmDepth);

@@ -242,8 +242,8 @@ int launch_modchol(psba_ctx* c, double* A) {
 // The factor goes to Lf (S is overwritten by the trailing updates); e_a is
 // left intact.  Barriers: a 64-bit arrival counter that only grows; its value
 // before a launch (the epoch) is kept on the device beside the flags
-// (bar[2 + T]) and advanced by workgroup 0 at the end of the launch, so a
-// launch that a gate turns off leaves it unchanged.  Every wait is bounded.
+// (bar[2 + T]) and advanced by workgroup 0 at the end of the launch, so no
+// host bookkeeping has to follow the launches.  Every wait is bounded.
 // d_scal[SC_FACTOR_OK] becomes 1 on success, 0 when a pivot is not finite
 // and positive, and -1 when a wait timed out (an execution error: the
 // workgroups were not all resident).  The grid is at most one workgroup per
@@ -746,10 +746,8 @@ __global__ __launch_bounds__(256) void k_chol_solve(double* __restrict__ A, int 
                                                     const double* __restrict__ ea, double* __restrict__ bw,
                                                     double* __restrict__ yv, double* __restrict__ xs,
                                                     double* __restrict__ dp, int n, double* __restrict__ ok,
-                                                    unsigned long long* bar, int* __restrict__ fail,
-                                                    const unsigned char* __restrict__ gate) {
+                                                    unsigned long long* bar, int* __restrict__ fail) {
   __shared__ CholSmem sm;
-  if (gate && !*gate) return;
   const int G = gridDim.x;
   // read before this workgroup's first arrival, so before workgroup 0 (which
   // passes every barrier last) advances it at the end
@@ -1088,7 +1086,7 @@ int launch_factor_solve(psba_ctx* c) {
   int ld = c->n_pad, n = c->n;
   probe_begin(c, PSBA_PROBE_FACTOR);
   hipLaunchKernelGGL(k_chol_solve, dim3(G), dim3(256), 0, c->stream, c->S, ld, T, c->Lf, c->ea, c->bw, c->yv,
-                     c->xsol, c->dp, n, ok, c->bar, c->chol_fail, gate_of(c, kGateActive));
+                     c->xsol, c->dp, n, ok, c->bar, c->chol_fail);
   probe_end(c, PSBA_PROBE_FACTOR);
   c->L_cur = c->Lf;
   PSBA_LAUNCH_TRY();

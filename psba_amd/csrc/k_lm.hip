@@ -5,7 +5,7 @@
 // one thread after each trial, so that a solve's trials are queued back to
 // back with no host round trip between them.  The state lives in LmDev; three
 // one-byte gates steer the kernels of the next trial:
-//   gate[kGateActive] the solve is running (every trial kernel),
+//   gate[kGateActive] the solve is running (mu0, the control itself),
 //   gate[kGateJac]    the trial opens an iteration (Jacobian + normal equations),
 //   gate[kGateAccept] the trial was accepted (commit of p + dp).
 // The arithmetic and the order of the tests are those of psba_levmar
@@ -88,9 +88,20 @@ __global__ __launch_bounds__(256) void k_lm_mu0(LmDev* s, const unsigned char* _
   }
 }
 
-// the decision after one trial (PSBA/levmar.cpp:124-245)
+// the decision after one trial (PSBA/levmar.cpp:124-245); the state then
+// goes to `out`, a slot of pinned host memory that the host reads once the
+// trial's event has completed (no separate copy)
+__device__ void lm_decide(LmDev* s, unsigned char* gate, const double* __restrict__ scal,
+                          psba_trace_rec* __restrict__ trace);
+
 __global__ void k_lm_control(LmDev* s, unsigned char* gate, const double* __restrict__ scal,
-                             psba_trace_rec* __restrict__ trace) {
+                             psba_trace_rec* __restrict__ trace, LmDev* out) {
+  lm_decide(s, gate, scal, trace);
+  *out = *s;
+}
+
+__device__ void lm_decide(LmDev* s, unsigned char* gate, const double* __restrict__ scal,
+                          psba_trace_rec* __restrict__ trace) {
   gate[kGateAccept] = 0;
   if (!gate[kGateActive]) {
     gate[kGateJac] = 0;
@@ -182,8 +193,8 @@ int launch_lm_mu0(psba_ctx* c) {
   return PSBA_OK;
 }
 
-int launch_lm_control(psba_ctx* c) {
-  hipLaunchKernelGGL(k_lm_control, dim3(1), dim3(1), 0, c->stream, c->lm, c->gate, c->d_scal, c->lm_trace);
+int launch_lm_control(psba_ctx* c, LmDev* out) {
+  hipLaunchKernelGGL(k_lm_control, dim3(1), dim3(1), 0, c->stream, c->lm, c->gate, c->d_scal, c->lm_trace, out);
   PSBA_LAUNCH_TRY();
   return PSBA_OK;
 }

@@ -368,8 +368,8 @@ __global__ __launch_bounds__(256) void k_vinv(const double* __restrict__ V, cons
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= np) return;
   if (mu_p) {
-    const int p = ownr[i];
-    if (!run[p]) return;
+    const int p = ownr ? ownr[i] : 0;
+    if (run && !run[p]) return;
     mu = mu_p[p];
   }
   const double* v = V + (size_t)i * 6;
@@ -416,10 +416,9 @@ __global__ __launch_bounds__(256) void k_vinv(const double* __restrict__ V, cons
 }
 
 int launch_vinv(psba_ctx* c, double mu) {
-  // in device-control mode the damping comes from the LM state and the kernel
-  // is gated with the solve
+  // in device-control mode the damping comes from the LM state
   hipLaunchKernelGGL(k_vinv, dim3(ceil_div(c->np, 256)), dim3(256), 0, c->stream, c->V, c->g + c->n, c->np, mu,
-                     c->Vinv, c->z, mu_of(c), owner_of(c), gate_of(c, kGateActive));
+                     c->Vinv, c->z, mu_of(c), (const int*)nullptr, (const unsigned char*)nullptr);
   PSBA_LAUNCH_TRY();
   return PSBA_OK;
 }
@@ -515,9 +514,9 @@ static void run_backsub(psba_ctx* c) {
   if (c->mixed())
     // (FP32 W rows: the id order's locality is worth more than the balance,
     // 33 against 64-71 us at the 88-camera scene)
-    backsub_launch<CNP, float>(c->stream, c->pt_ptr, c->obs_cam, c->W32, c->Vinv, c->g + c->n, c->np, c->n, c->dp, c->eb, owner_of(c), gate_of(c, kGateActive), 256);
+    backsub_launch<CNP, float>(c->stream, c->pt_ptr, c->obs_cam, c->W32, c->Vinv, c->g + c->n, c->np, c->n, c->dp, c->eb, (const int*)nullptr, (const unsigned char*)nullptr, 256);
   else
-    backsub_launch<CNP, double>(c->stream, c->pt_ptr, c->obs_cam, c->W, c->Vinv, c->g + c->n, c->np, c->n, c->dp, c->eb, owner_of(c), gate_of(c, kGateActive), 256, c->pt_order);
+    backsub_launch<CNP, double>(c->stream, c->pt_ptr, c->obs_cam, c->W, c->Vinv, c->g + c->n, c->np, c->n, c->dp, c->eb, (const int*)nullptr, (const unsigned char*)nullptr, 256, c->pt_order);
   probe_end(c, PSBA_PROBE_BSOLVE);
 }
 

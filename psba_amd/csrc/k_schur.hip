@@ -230,7 +230,7 @@ static void run_schur_t(psba_ctx* c, double mu, const R* W, R* schpart, R* eapar
     probe_begin(c, PSBA_PROBE_SCHUR);
     hipLaunchKernelGGL((k_schur_chunks<CNP, R>), dim3(PSBA_SCHUR_XCD ? xcd_grid(c->n_schunk) : c->n_schunk), dim3(256),
                        0, c->stream, c->pair_items, c->schunk_begin, c->schunk_end, c->obs_pt, W, c->Vinv, schpart,
-                       owner_of(c), gate_of(c, kGateActive), c->n_schunk, (const int*)c->schunk_order);
+                       (const int*)nullptr, (const unsigned char*)nullptr, c->n_schunk, (const int*)c->schunk_order);
     probe_end(c, PSBA_PROBE_SCHUR);
   }
   hipLaunchKernelGGL((k_schur_finish<CNP, R>), dim3(c->n_lblocks + (c->n_pad - c->n)),
@@ -238,10 +238,11 @@ static void run_schur_t(psba_ctx* c, double mu, const R* W, R* schpart, R* eapar
                      c->fix, mu, mu_of(c), c->n_pad, c->n_lblocks, c->n, c->S);
   if (c->n_cchunk > 0)
     hipLaunchKernelGGL((k_ea_chunks<CNP, R>), dim3(c->n_cchunk), dim3(256), 0, c->stream, c->cam_obs,
-                       c->cchunk_begin, c->cchunk_end, c->obs_pt, W, c->z, eapart, owner_of(c),
-                       gate_of(c, kGateActive));
+                       c->cchunk_begin, c->cchunk_end, c->obs_pt, W, c->z, eapart, (const int*)nullptr,
+                       (const unsigned char*)nullptr);
   hipLaunchKernelGGL((k_ea_finish<CNP, R>), dim3(ceil_div((long)c->nc * CNP, 64)), dim3(64), 0, c->stream,
-                     c->cam_cchunk_ptr, eapart, c->g, c->nc, c->ea, owner_of(c), gate_of(c, kGateActive));
+                     c->cam_cchunk_ptr, eapart, c->g, c->nc, c->ea, (const int*)nullptr,
+                     (const unsigned char*)nullptr);
 }
 
 template <int CNP>
