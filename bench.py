@@ -396,6 +396,31 @@ def path_roofline(sc, info, r, solve_s):
             "per_trial_roof_us": 1e6 * t_trial, "per_jacobian_roof_us": 1e6 * t_jac}
 
 
+def copy_calibration(nbytes=512 << 20):
+    """A plain device-to-device copy of known size: the HBM bytes a
+    profiler's FETCH_SIZE and WRITE_SIZE counters report for it give the
+    factors that turn their readings for the solver's kernels into bytes."""
+    import ctypes as ct
+    H = None
+    for name in ("libamdhip64.so", "/opt/rocm/lib/libamdhip64.so"):
+        try:
+            H = ct.CDLL(name)
+            break
+        except OSError:
+            continue
+    if H is None:
+        raise RuntimeError("HIP runtime not found")
+    a, b = ct.c_void_p(), ct.c_void_p()
+    assert H.hipMalloc(ct.byref(a), ct.c_size_t(nbytes)) == 0 and H.hipMalloc(ct.byref(b), ct.c_size_t(nbytes)) == 0
+    assert H.hipMemset(a, 1, ct.c_size_t(nbytes)) == 0
+    assert H.hipDeviceSynchronize() == 0
+    assert H.hipMemcpy(b, a, ct.c_size_t(nbytes), 3) == 0  # hipMemcpyDeviceToDevice
+    assert H.hipDeviceSynchronize() == 0
+    H.hipFree(a)
+    H.hipFree(b)
+    log("copied %d bytes device to device" % nbytes)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -403,6 +428,10 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--seed", type=int, default=1002)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--headline-only", action="store_true",
+                    help="only the timed solves and the roofline line (for profiler runs)")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="one 512 MiB device-to-device copy and exit (scales the profiler's FETCH_SIZE/WRITE_SIZE)")
     ap.add_argument("--sections", default="mixed,index,batch,stream",
                     help="comma-separated sections after the headline solves: mixed, index, batch, stream")
     a = ap.parse_args()
@@ -411,6 +440,9 @@ def main():
         print("bench.py measures one MI355X only: multi-GPU runs are not part of this build "
               "(run with --gpus 1)", file=sys.stderr)
         sys.exit(2)
+    if a.copy_calibration:
+        copy_calibration()
+        return
     import psba_amd as P
 
     sc = P.Scene(2, a.seed)
@@ -482,6 +514,9 @@ def main():
                        "flops_per_launch": flops, "n": int(n), "traffic_source": traffic_src}
     log("dense factor+solve: %d launches, %.1f us avg, %.3f TFLOP/s (%.2f%% of FP64 peak)" % (
         fac_n, avg_s * 1e6, achieved, 100 * achieved / FP64_PEAK_TFLOPS))
+    if a.headline_only:
+        print(json.dumps(out), flush=True)
+        return
     out["stages"] = stage_rates(P, ctx, solve_on, sc, info, r, dt / a.steps)
     out["path_roofline"] = path_roofline(sc, info, r, dt / a.steps)
     log("path roofline: T_roof %.2f ms vs measured %.2f ms per solve (%.1f%%)" % (
