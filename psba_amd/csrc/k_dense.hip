@@ -230,11 +230,16 @@ int launch_modchol(psba_ctx* c, double* A) {
 // k+1 <= j <= i, the owning workgroup recomputes the panel rows it needs,
 //     L_ik = A_ik L_kk^-T,   L_jk = A_jk L_kk^-T        (row-parallel solves)
 //     A_ij -= L_ik L_jk^T                               (FP64 MFMA 16x16x4)
-// the task (i, k+1) stores L_ik and updates the right-hand side
-//     b_i -= L_ik y_k,
-// and the task (k+1, k+1) factors the new diagonal tile at once and forms
-// y_{k+1} = L_{k+1,k+1}^-1 b_{k+1}.  Recomputing L_ik per task trades a few
-// redundant triangular solves (cheap) for a single grid barrier per phase.
+// the task (i, k+1) stores L_ik, and the task (k+1, k+1) factors the new
+// diagonal tile at once.  Recomputing L_ik per task trades a few redundant
+// triangular solves (cheap) for a single grid barrier per phase.
+// The forward substitution L y = b runs one phase behind, off the diagonal
+// chain, on one more workgroup that has no tile tasks: in phase k it applies
+// y_{k-1} to the rows below (b_i -= L_{i,k-1} y_{k-1}, i >= k; the panel
+// L_{.,k-1} was stored in phase k-1) and forms y_k = L_kk^-1 b_k, so nothing
+// else waits for it.  Workgroup 0 forms y_0 before the first barrier and the
+// last y after the last one.  (With a single workgroup, the tasks (i, k+1)
+// update b_i and the diagonal task forms y_{k+1}.)
 // The back substitution L^T x = y runs in T more phases: workgroup 0 solves
 // block k (after applying x_{k+1} to it), the others apply x_{k+1} to the
 // blocks above.
@@ -742,6 +747,73 @@ __device__ __forceinline__ double tile_tmatvec(CholSmem& sm, const double* Lt, i
   return t;
 }
 
+// s = sum_c Lt[r][c] x[c] for row r = lane of a 64x64 tile, in a fixed order
+__device__ __forceinline__ double tile_matvec(CholSmem& sm, const double* Lt, int ld, const double* x) {
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  double s = 0.0;
+#pragma unroll
+  for (int c = 16 * w; c < 16 * w + 16; ++c) s += Lt[(size_t)lane * ld + c] * x[c];
+  sm.part[w][lane] = s;
+  __syncthreads();
+  const double t = ((sm.part[0][lane] + sm.part[1][lane]) + sm.part[2][lane]) + sm.part[3][lane];
+  __syncthreads();
+  return t;
+}
+
+// One forward step of k_chol_solve's forward workgroup in phase k >= 1:
+// b_i -= L_{i,k-1} y_{k-1} for i = k..T-1 (y_{k-1} in sm.w; from yv at k = 1,
+// where workgroup 0 formed it), then y_k = L_kk^-1 b_k into yv and sm.w.
+// Thread (lane, w) takes row `lane`, columns 16w..16w+15 of each tile; four
+// tiles' loads are in flight together, their partial sums meet in sm.X and
+// are added as (p0 + p1) + (p2 + p3) -- the single-workgroup path's order.
+__device__ void forward_step(CholSmem& sm, const double* __restrict__ Lf, int ld, int T, int k,
+                             double* __restrict__ bw, double* __restrict__ yv) {
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  if (k == 1) {
+    if (tid < NB) sm.w[tid] = yv[tid];
+    __syncthreads();
+  }
+  double* part = sm.X;  // [4 tiles][4 waves][NB]
+  for (int i = k; i < T; i += 4) {
+    const int nt = min(4, T - i);
+    double l[4][16];
+#pragma unroll
+    for (int u = 0; u < 4; ++u)
+      if (u < nt) {
+        const double* Lr = Lf + (size_t)(i + u) * NB * ld + (size_t)(k - 1) * NB + (size_t)lane * ld + 16 * w;
+#pragma unroll
+        for (int c = 0; c < 16; ++c) l[u][c] = Lr[c];
+      }
+    double b[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) b[u] = (u < nt && tid < NB) ? bw[(size_t)(i + u) * NB + tid] : 0.0;
+#pragma unroll
+    for (int u = 0; u < 4; ++u)
+      if (u < nt) {
+        double s = 0.0;
+#pragma unroll
+        for (int c = 0; c < 16; ++c) s += l[u][c] * sm.w[16 * w + c];
+        part[(4 * u + w) * NB + lane] = s;
+      }
+    __syncthreads();
+    if (tid < NB) {
+#pragma unroll
+      for (int u = 0; u < 4; ++u)
+        if (u < nt) {
+          const double* pu = part + 4 * u * NB;
+          const double bi = b[u] - ((pu[tid] + pu[NB + tid]) + (pu[2 * NB + tid] + pu[3 * NB + tid]));
+          bw[(size_t)(i + u) * NB + tid] = bi;
+          if (i + u == k) sm.v[tid] = bi;
+        }
+    }
+    __syncthreads();
+  }
+  load_lower(sm.P, Lf + (size_t)k * NB * ld + (size_t)k * NB, ld);
+  __syncthreads();
+  tile_forward(LDS(sm.P), LDS(sm.v), yv + (size_t)k * NB, LDS(sm.w));
+  __syncthreads();
+}
+
 __global__ __launch_bounds__(256) void k_chol_solve(double* __restrict__ A, int ld, int T, double* __restrict__ Lf,
                                                     const double* __restrict__ ea, double* __restrict__ bw,
                                                     double* __restrict__ yv, double* __restrict__ xs,
@@ -760,6 +832,8 @@ __global__ __launch_bounds__(256) void k_chol_solve(double* __restrict__ A, int 
   // task of the phase it factors straight into sm.P (with 1/diag in sm.rd) and
   // keeps y_k in sm.w, ready for the next phase
   bool carry = false;
+  // a workgroup beyond every phase's task count: the forward substitution
+  const bool fwd = T >= 3 && G > (T - 1) * T / 2;
 
   // ---- phase -1: factor tile (0,0) and y_0; copy the right-hand side
   if (blockIdx.x == 0) {
@@ -781,13 +855,14 @@ __global__ __launch_bounds__(256) void k_chol_solve(double* __restrict__ A, int 
   // ---- phases 0 .. T-2: factor and forward substitution
   for (int k = 0; k + 1 < T; ++k) {
     const int m = T - 1 - k, ntasks = m * (m + 1) / 2;
+    if (fwd && k >= 1 && (int)blockIdx.x == G - 1) forward_step(sm, Lf, ld, T, k, bw, yv);
     if ((int)blockIdx.x < ntasks) {
       if (carry) {
         if (tid < NB) sm.v[tid] = sm.w[tid];
         __syncthreads();
       } else {
         load_lower(sm.P, tile(Lf, k, k), ld);
-        if (tid < NB) sm.v[tid] = yv[(size_t)k * NB + tid];
+        if (!fwd && tid < NB) sm.v[tid] = yv[(size_t)k * NB + tid];
         __syncthreads();
         if (tid < NB) sm.rd[tid] = 1.0 / sm.P[tid * TP + tid];
         __syncthreads();
@@ -801,7 +876,7 @@ __global__ __launch_bounds__(256) void k_chol_solve(double* __restrict__ A, int 
       const int j = k + 1 + a, i = j + (t - cum);
       // b_i is read now so that its load overlaps the tile loads
       const int br_row = tid >> 2, br_p = tid & 3;
-      const double b_old = (j == k + 1 && br_p == 0) ? bw[(size_t)i * NB + br_row] : 0.0;
+      const double b_old = (!fwd && j == k + 1 && br_p == 0) ? bw[(size_t)i * NB + br_row] : 0.0;
       dvec4 acc[2][2];
       task_load(acc, sm, tile(A, i, j), tile(A, i, k), (i != j) ? tile(A, j, k) : nullptr, ld);
       trsm_rows(LDS(sm.X), LDS(sm.P), LDS(sm.rd));
@@ -812,20 +887,23 @@ __global__ __launch_bounds__(256) void k_chol_solve(double* __restrict__ A, int 
         // store L_ik (the diagonal task stores it later, beside its forward
         // solve); b_i -= L_ik y_k (4 threads per row, fixed order)
         if (i != j) store_tile(sm.X, tile(Lf, i, k), ld);
-        double s = 0.0;
+        if (!fwd) {
+          double s = 0.0;
 #pragma unroll
-        for (int c = 0; c < 16; ++c) s += sm.X[br_row * TP + 16 * br_p + c] * sm.v[16 * br_p + c];
-        s += __shfl_xor(s, 1, 64);
-        s += __shfl_xor(s, 2, 64);
-        if (br_p == 0) {
-          const double bi = b_old - s;
-          bw[(size_t)i * NB + br_row] = bi;
-          if (i == j) sm.part[0][br_row] = bi;
+          for (int c = 0; c < 16; ++c) s += sm.X[br_row * TP + 16 * br_p + c] * sm.v[16 * br_p + c];
+          s += __shfl_xor(s, 1, 64);
+          s += __shfl_xor(s, 2, 64);
+          if (br_p == 0) {
+            const double bi = b_old - s;
+            bw[(size_t)i * NB + br_row] = bi;
+            if (i == j) sm.part[0][br_row] = bi;
+          }
         }
       }
       if (i == j && j == k + 1) {
-        // the next diagonal tile: factor it and form y_{k+1}; as the
-        // workgroup's last task of the phase, into sm.P for the next phase
+        // the next diagonal tile: factor it (and form y_{k+1} without a
+        // forward workgroup); as the workgroup's last task of the phase,
+        // into sm.P for the next phase
         carry = (t + G >= ntasks);
         double* Dt = carry ? sm.P : sm.Y;
         __syncthreads();
@@ -835,7 +913,7 @@ __global__ __launch_bounds__(256) void k_chol_solve(double* __restrict__ A, int 
         if (bad && tid == 0) __hip_atomic_fetch_or(fail, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         store_tile_w123(Dt, tile(Lf, i, i), ld);
         store_tile_w123(sm.X, tile(Lf, i, k), ld);
-        tile_forward(LDS(Dt), LDS(sm.part[0]), yv + (size_t)i * NB, LDS(sm.w));
+        if (!fwd) tile_forward(LDS(Dt), LDS(sm.part[0]), yv + (size_t)i * NB, LDS(sm.w));
       } else {
         acc_store(acc, tile(A, i, j), ld);
       }
@@ -878,7 +956,30 @@ __global__ __launch_bounds__(256) void k_chol_solve(double* __restrict__ A, int 
     }
   } else if (blockIdx.x == 0) {
     load_lower(sm.P, tile(Lf, T - 1, T - 1), ld);
-    double r_next = yv[(size_t)(T - 1) * NB + (tid & 63)];
+    double r_next;
+    if (fwd) {
+      // the last forward step: b_{T-1} holds every term but L_{T-1,T-2} y_{T-2}
+      // (summed as in forward_step)
+      if (tid < NB) sm.w[tid] = yv[(size_t)(T - 2) * NB + tid];
+      __syncthreads();
+      {
+        const int lane = tid & 63, w = tid >> 6;
+        const double* Lr = tile(Lf, T - 1, T - 2) + (size_t)lane * ld + 16 * w;
+        double s = 0.0;
+#pragma unroll
+        for (int c = 0; c < 16; ++c) s += Lr[c] * sm.w[16 * w + c];
+        sm.part[w][lane] = s;
+      }
+      __syncthreads();
+      if (tid < NB)
+        sm.v[tid] = bw[(size_t)(T - 1) * NB + tid] - ((sm.part[0][tid] + sm.part[1][tid]) + (sm.part[2][tid] + sm.part[3][tid]));
+      __syncthreads();
+      tile_forward(LDS(sm.P), LDS(sm.v), yv + (size_t)(T - 1) * NB, LDS(sm.w));
+      __syncthreads();
+      r_next = sm.w[tid & 63];
+    } else {
+      r_next = yv[(size_t)(T - 1) * NB + (tid & 63)];
+    }
     __syncthreads();
     for (int k = T - 1; k >= 0; --k) {
       // r_{k-1} is complete two steps ahead: load it now, use it next step
@@ -959,19 +1060,6 @@ __global__ __launch_bounds__(256) void k_chol_solve(double* __restrict__ A, int 
 // S x = b with the factor L already in Lf (one workgroup; used by the mixed
 // path's FP64 refinement): forward L y = b, then back L^T x = y, tile by tile.
 // b has n entries (zero-padded to T tiles here); y and x are n_pad scratch.
-
-// s = sum_c Lt[r][c] x[c] for row r = lane of a 64x64 tile, in a fixed order
-__device__ __forceinline__ double tile_matvec(CholSmem& sm, const double* Lt, int ld, const double* x) {
-  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-  double s = 0.0;
-#pragma unroll
-  for (int c = 16 * w; c < 16 * w + 16; ++c) s += Lt[(size_t)lane * ld + c] * x[c];
-  sm.part[w][lane] = s;
-  __syncthreads();
-  const double t = ((sm.part[0][lane] + sm.part[1][lane]) + sm.part[2][lane]) + sm.part[3][lane];
-  __syncthreads();
-  return t;
-}
 
 // LDS of k_tri_solve: the diagonal tile, and the whole solution vector (y in
 // the forward pass, overwritten block by block by x in the backward pass)
@@ -1079,7 +1167,9 @@ int launch_factor_solve(psba_ctx* c) {
     // workgroup of a launch is resident at once on the stream's device
     c->chol_resident = std::min(per_cu, 1) * ncu;
     if (c->chol_resident < 1) return PSBA_ERR_UNSUPPORTED;
-    c->chol_grid = std::max(1, std::min(c->chol_resident, (T - 1) * T / 2));
+    // one workgroup per task of the first phase, and one more for the forward
+    // substitution when there are at least three tiles
+    c->chol_grid = std::max(1, std::min(c->chol_resident, (T - 1) * T / 2 + (T >= 3 ? 1 : 0)));
   }
   int G = c->chol_grid;
   double* ok = c->d_scal + SC_FACTOR_OK;

@@ -5,8 +5,11 @@
   has an order fixed by the problem alone, and stopped problems are skipped.
 * Against the single-problem psba_levmar with the same rules (no handoff,
   relative-decrease stop, 100 iterations) and against the oracle's LM, the
-  stop code and iteration count agree and the final cost agrees within 1e-9
-  relative (the reduction orders differ, the arithmetic does not).
+  stop code and iteration count agree exactly and the final cost and
+  parameters within 1e-9 relative (the reduction orders differ, the
+  arithmetic does not), or within ten times the spread of the oracle's own
+  solves from starts one ulp apart where that is wider (the free-scale,
+  16-parameter scenes).
 * Polling, autotuning and re-committing into a used arena change nothing.
 * The mixed-precision batch reaches the FP64 batch's final cost within 1e-6
   relative.
@@ -35,6 +38,28 @@ def solve_batch(P, scenes, precision=0, **opts):
     out = [b.result(s) for s in range(len(scenes))]
     b.close()
     return out
+
+
+def ulp_cloud(O, sc, starts=4):
+    """Spread of the oracle's converged LM (OPTS, no handoff) over starting
+    points one ulp away from the scene's (each parameter moved up or down at
+    random): the largest relative differences from the unperturbed solve in
+    the cost, the cameras and the points.  Every start must stop at the same
+    iteration with the same code."""
+    rng = np.random.default_rng(17)
+    out = []
+    for t in range(starts + 1):
+        cams, pts = sc.cams.copy(), sc.pts.copy()
+        if t:
+            cams = np.nextafter(cams, cams + rng.choice([-1.0, 1.0], cams.size))
+            pts = np.nextafter(pts, pts + rng.choice([-1.0, 1.0], pts.size))
+        o = O.Oracle(sc.cnp, sc.K, sc.q0, sc.meas, sc.iidx, sc.jidx, cams, pts, sc.fixmask)
+        r = o.levmar(max_iter=OPTS["max_iter"], flags=O.FLAG_NO_HANDOFF, rel_tol=OPTS["rel_tol"])
+        out.append((r, o.read_params()))
+    (r0, (c0, p0)) = out[0]
+    assert all((r["code"], r["itno"]) == (r0["code"], r0["itno"]) for r, _ in out)
+    return (max(abs(r["final_err"] - r0["final_err"]) / r0["final_err"] for r, _ in out[1:]),
+            max(rel(c, c0) for _, (c, _) in out[1:]), max(rel(p, p0) for _, (_, p) in out[1:]))
 
 
 def same(a, b):
@@ -75,19 +100,29 @@ def test_matches_levmar_and_oracle(P, O, gpu):
             assert r["itno"] == ref["itno"]
             assert abs(r["final_err"] - ref["final_err"]) <= 1e-9 * ref["final_err"]
         cams, pts = c.read_params()
-        assert rel(r["cams"], cams) < 1e-7
-        assert rel(r["pts"], pts) < 1e-7
+        assert rel(r["cams"], cams) < 1e-9
+        assert rel(r["pts"], pts) < 1e-9
         c.close()
 
 
 @pytest.mark.parametrize("config,kw", [(1, dict(n_pts=400, n_cams=12)), (1, dict(cnp=16, n_pts=400, n_cams=10)),
                                        (2, dict(n_pts=1500, n_cams=12, outlier_frac=0.0))],
                          ids=["cnp11", "cnp16", "cnp9"])
-def test_camera_models(P, gpu, config, kw):
+def test_camera_models(P, O, gpu, config, kw):
     """Fixed budget (8 iterations, before any stopping test can fire) and
     convergence by the relative-decrease rule.  The reference's own |dp| test
     (|dp| < 1e-12 |p|) is left out: at convergence it compares rounding noise,
-    so the iteration it fires on is not a property of the algorithm."""
+    so the iteration it fires on is not a property of the algorithm.
+
+    Stop code and iteration count must agree exactly, cost and parameters
+    within 1e-9 -- or within ten times the scene's own noise cloud where that
+    is wider: with only camera 0 held, the scale of the reconstruction is
+    free, and on the 16-parameter model the converged points move by up to
+    1.6e-5 (cameras 6.6e-8) when the oracle starts one ulp away, at the same
+    iteration count and cost (1e-15).  Any two correct FP64 solves land
+    inside that cloud, so closer agreement would not be a property of the
+    algorithm."""
+    O.set_threads(16)
     scenes = [P.Scene(config, 77 + k, **kw) for k in range(3)]
     short = solve_batch(P, scenes, max_iter=8, rel_tol=0.0)
     full = solve_batch(P, scenes, **OPTS)
@@ -98,14 +133,18 @@ def test_camera_models(P, gpu, config, kw):
         c.close()
         assert (r8["itno"], r8["code"]) == (g8["itno"], g8["code"]) == (8, P.ITER_CONTINUE)
         assert abs(r8["final_err"] - g8["final_err"]) <= 1e-9 * g8["final_err"]
-        assert rel(r8["cams"], cams) < 1e-7 and rel(r8["pts"], pts) < 1e-7
+        assert rel(r8["cams"], cams) < 1e-9 and rel(r8["pts"], pts) < 1e-9
         c = P.Context.from_scene(sc)
         g = c.levmar(flags=P.FLAG_NO_HANDOFF, **OPTS)
+        c_cams, c_pts = c.read_params()
         c.close()
-        info = (r["itno"], g["itno"], r["code"], g["code"], r["final_err"], g["final_err"])
+        dc, dcam, dpt = ulp_cloud(O, sc)
+        tc, tcam, tpt = (max(1e-9, 10 * x) for x in (dc, dcam, dpt))
+        info = (r["itno"], g["itno"], r["code"], g["code"], r["final_err"], g["final_err"], tc, tcam, tpt)
         assert r["code"] == g["code"], info
-        assert abs(r["itno"] - g["itno"]) <= 1, info
-        assert abs(r["final_err"] - g["final_err"]) <= 1e-9 * g["final_err"], info
+        assert r["itno"] == g["itno"], info
+        assert abs(r["final_err"] - g["final_err"]) <= tc * g["final_err"], info
+        assert rel(r["cams"], c_cams) < tcam and rel(r["pts"], c_pts) < tpt, info
 
 
 def test_poll_and_autotune(P, gpu):
