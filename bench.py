@@ -434,6 +434,8 @@ def main():
                     help="one 512 MiB device-to-device copy and exit (scales the profiler's FETCH_SIZE/WRITE_SIZE)")
     ap.add_argument("--sections", default="mixed,index,batch,stream",
                     help="comma-separated sections after the headline solves: mixed, index, batch, stream")
+    ap.add_argument("--stream-rates", default="200,1000,2000,4000",
+                    help="offered request rates (req/s) of the stream section")
     a = ap.parse_args()
     sections = set(x for x in a.sections.split(",") if x)
     if a.gpus != 1:
@@ -528,7 +530,7 @@ def main():
     if "batch" in sections:
         out["batch"] = batch_section(P)
     if "stream" in sections:
-        out["stream"] = stream_section(P)
+        out["stream"] = stream_section(P, rates=tuple(int(x) for x in a.stream_rates.split(",") if x))
     if not a.no_cpu:
         out["cpu_baseline"] = cpu_baseline(P, sc, trials)
     print(json.dumps(out), flush=True)

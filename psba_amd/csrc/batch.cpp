@@ -5,6 +5,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdarg>
+#include <cstdlib>
 #include <cstring>
 #include <thread>
 #include <vector>
@@ -264,6 +265,7 @@ int enqueue_round(psba_batch* b, int parity) {
 }
 
 int capture_round(psba_batch* b) {
+  if (b->eager) return PSBA_OK;
   if (b->graph_exec[0] && b->graph_iter == b->max_iter && b->graph_tol == b->rel_tol) return PSBA_OK;
   PSBA_TRY(free_graph(b));
   for (int k = 0; k < 2; ++k) {
@@ -285,7 +287,8 @@ int capture_round(psba_batch* b) {
 }
 
 int launch_round(psba_batch* b) {
-  PSBA_HIP_TRY(hipGraphLaunch(b->graph_exec[b->launched % 2], b->stream));
+  if (b->eager) PSBA_TRY(enqueue_round(b, b->launched % 2));
+  else PSBA_HIP_TRY(hipGraphLaunch(b->graph_exec[b->launched % 2], b->stream));
   PSBA_HIP_TRY(hipEventRecord(b->ev_round[b->launched % psba_batch::kDepth], b->stream));
   b->launched++;
   return PSBA_OK;
@@ -440,6 +443,8 @@ int psba_batch_create(const psba_batch_desc* d, psba_batch** out) {
   b->precision = d->precision;
   b->device = d->device;
   b->cap_p = d->max_problems;
+  const char* ev = std::getenv("PSBA_BATCH_EAGER");
+  b->eager = ev && ev[0] == '1';
   hipError_t e = hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking);
   for (int k = 0; k < psba_batch::kDepth && e == hipSuccess; ++k) e = hipEventCreate(&b->ev_round[k]);
   if (e == hipSuccess) e = hipEventCreate(&b->ev_start);

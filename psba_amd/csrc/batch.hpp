@@ -114,6 +114,10 @@ struct psba_batch {
   // round stays stable until the round two later is launched
   hipGraph_t graph[2] = {nullptr, nullptr};
   hipGraphExec_t graph_exec[2] = {nullptr, nullptr};
+  // rounds enqueued as plain launches instead (PSBA_BATCH_EAGER=1 at
+  // creation): the same kernels and arguments, for tools that do not follow
+  // graph launches
+  bool eager = false;
   static constexpr int kDepth = 2;  // rounds queued ahead of the host's check
   hipEvent_t ev_round[kDepth] = {nullptr, nullptr};
   int launched = 0, checked = 0;

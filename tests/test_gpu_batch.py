@@ -249,3 +249,15 @@ def test_config3_full_batch(P, O, gpu):
         r = res[k]
         assert (r["code"], r["itno"]) == (o["code"], o["itno"]), k
         assert abs(r["final_err"] - o["final_err"]) <= 1e-9 * o["final_err"], k
+
+
+def test_eager_rounds_equal_graph_rounds(P, gpu, monkeypatch):
+    """PSBA_BATCH_EAGER=1 (set when the batch is created) enqueues each round
+    as plain launches instead of replaying the captured graphs: the same
+    kernels and arguments, so the same results bit for bit."""
+    scenes = [window(P, k) for k in range(50, 54)]
+    graph = solve_batch(P, scenes)
+    monkeypatch.setenv("PSBA_BATCH_EAGER", "1")
+    eager = solve_batch(P, scenes)
+    for a, b in zip(graph, eager):
+        same(a, b)
