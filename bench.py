@@ -251,10 +251,11 @@ def index_section(P, sc, reps=5):
     return out
 
 
-def stream_section(P, seed=1004, rates=(200, 1000, 2000, 4000), duration=2.0, serial_n=100):
+def stream_section(P, seed=1004, rates=(200, 1000, 2000, 4000, 8000), duration=2.0, serial_n=100, capacity_n=2000):
     """Open-loop Poisson request stream (config 4 mix: 90% small, 9% medium,
     1% large) through the request loop, against one-at-a-time plain FP64
-    solves of the same mix."""
+    solves of the same mix; and the loop's capacity: capacity_n requests of
+    the mix all submitted at once, solves/s until the last completes."""
     rng = np.random.default_rng(seed)
     pool = {0: [], 1: [], 2: []}
     for c, count in ((0, 300), (1, 30), (2, 6)):
@@ -320,6 +321,13 @@ def stream_section(P, seed=1004, rates=(200, 1000, 2000, 4000), duration=2.0, se
     rel = [abs(done[k]["final_err"] - serial[k]) / serial[k] for k in range(len(sample))]
     out["max_rel_cost_vs_serial"] = float(max(rel))
     out["burst_solves_per_s"] = len(sample) / span
+    reqs, cls = draw(capacity_n)
+    done, lat, span, st = run(reqs, np.zeros(capacity_n))
+    out["capacity"] = {"requests": capacity_n, "solves_per_s": capacity_n / span,
+                       "vs_serial": capacity_n / span / serial_rate, "p50_ms": 1e3 * float(np.percentile(lat, 50)),
+                       "p99_ms": 1e3 * float(np.percentile(lat, 99)), "batches": st["batches"]}
+    log("stream capacity: %d requests at once, %.1f solves/s (%.1fx serial)" % (
+        capacity_n, capacity_n / span, capacity_n / span / serial_rate))
     return out
 
 
@@ -434,7 +442,7 @@ def main():
                     help="one 512 MiB device-to-device copy and exit (scales the profiler's FETCH_SIZE/WRITE_SIZE)")
     ap.add_argument("--sections", default="mixed,index,batch,stream",
                     help="comma-separated sections after the headline solves: mixed, index, batch, stream")
-    ap.add_argument("--stream-rates", default="200,1000,2000,4000",
+    ap.add_argument("--stream-rates", default="200,1000,2000,4000,8000",
                     help="offered request rates (req/s) of the stream section")
     a = ap.parse_args()
     sections = set(x for x in a.sections.split(",") if x)

@@ -32,6 +32,33 @@ __device__ inline void prob_ranges(const int* cam_off, const int* pt_off, int cn
   *b1 = N + 3L * pt_off[p + 1];
 }
 
+// The indices lo + threadIdx.x + s blockDim.x (s = 0, 1, ..) below hi, U at a
+// time in ascending order: fn(q, cnt) gets U indices of which the first cnt
+// are in range, so it can issue all of their loads before using any and
+// still accumulate in the plain strided loop's order.  Whole groups pass the
+// constant U (the range tests fold away); only the last group is partial.
+template <int U, typename F>
+__device__ inline void strided(long lo, long hi, F fn) {
+  const long step = blockDim.x;
+  long q0 = lo + threadIdx.x;
+  for (; q0 + (U - 1) * step < hi; q0 += U * step) {
+    long q[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) q[u] = q0 + u * step;
+    fn(q, U);
+  }
+  if (q0 < hi) {
+    long q[U];
+    int cnt = 0;
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      q[u] = q0 + u * step;
+      cnt += q[u] < hi;
+    }
+    fn(q, cnt);
+  }
+}
+
 }  // namespace
 
 // sum of squared residuals over one observation chunk; with dp the parameters
@@ -105,8 +132,20 @@ __global__ __launch_bounds__(256) void k_b_mu0(const double* __restrict__ uvdiag
   long a0, a1, b0, b1;
   prob_ranges(cam_off, pt_off, cnp, N, p, &a0, &a1, &b0, &b1);
   double m = 0.0;
-  for (long q = a0 + threadIdx.x; q < a1; q += blockDim.x) m = fmax(m, uvdiag[q]);
-  for (long q = b0 + threadIdx.x; q < b1; q += blockDim.x) m = fmax(m, uvdiag[q]);
+  strided<8>(a0, a1, [&](const long* q, int cnt) {
+    double v[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) v[u] = u < cnt ? uvdiag[q[u]] : 0.0;
+#pragma unroll
+    for (int u = 0; u < 8; ++u) m = fmax(m, v[u]);
+  });
+  strided<8>(b0, b1, [&](const long* q, int cnt) {
+    double v[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) v[u] = u < cnt ? uvdiag[q[u]] : 0.0;
+#pragma unroll
+    for (int u = 0; u < 8; ++u) m = fmax(m, v[u]);
+  });
 #pragma unroll
   for (int off = 32; off >= 1; off >>= 1) m = fmax(m, __shfl_xor(m, off, kWave));
   if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = m;
@@ -134,16 +173,22 @@ __global__ __launch_bounds__(256) void k_b_step_stats(const double* __restrict__
   prob_ranges(cam_off, pt_off, cnp, N, p, &a0, &a1, &b0, &b1);
   const double m = mu[p];
   double acc[2] = {0.0, 0.0};
-  for (long q = a0 + threadIdx.x; q < a1; q += blockDim.x) {
-    const double d = dp[q];
-    acc[0] += d * d;
-    acc[1] += d * (m * d + g[q]);
-  }
-  for (long q = b0 + threadIdx.x; q < b1; q += blockDim.x) {
-    const double d = dp[q];
-    acc[0] += d * d;
-    acc[1] += d * (m * d + g[q]);
-  }
+  auto part = [&](const long* q, int cnt) {
+    double d[8], gq[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      d[u] = u < cnt ? dp[q[u]] : 0.0;
+      gq[u] = u < cnt ? g[q[u]] : 0.0;
+    }
+#pragma unroll
+    for (int u = 0; u < 8; ++u)
+      if (u < cnt) {
+        acc[0] += d[u] * d[u];
+        acc[1] += d[u] * (m * d[u] + gq[u]);
+      }
+  };
+  strided<8>(a0, a1, part);
+  strided<8>(b0, b1, part);
   block_sum<2>(acc, sh);
   if (threadIdx.x == 0) {
     dp2[p] = acc[0];
@@ -233,16 +278,20 @@ __global__ __launch_bounds__(256) void k_b_commit(double* __restrict__ cams, dou
   long a0, a1, b0, b1;
   prob_ranges(cam_off, pt_off, cnp, N, p, &a0, &a1, &b0, &b1);
   double acc[1] = {0.0};
-  for (long q = a0 + threadIdx.x; q < a1; q += blockDim.x) {
-    const double v = cams[q] + dp[q];
-    cams[q] = v;
-    acc[0] += v * v;
-  }
-  for (long q = b0 + threadIdx.x; q < b1; q += blockDim.x) {
-    const double v = pts[q - N] + dp[q];
-    pts[q - N] = v;
-    acc[0] += v * v;
-  }
+  // camera parameters (q < N) live in cams, point coordinates in pts
+  auto part = [&](double* base, long shift, const long* q, int cnt) {
+    double v[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) v[u] = u < cnt ? base[q[u] - shift] + dp[q[u]] : 0.0;
+#pragma unroll
+    for (int u = 0; u < 8; ++u)
+      if (u < cnt) {
+        base[q[u] - shift] = v[u];
+        acc[0] += v[u] * v[u];
+      }
+  };
+  strided<8>(a0, a1, [&](const long* q, int cnt) { part(cams, 0, q, cnt); });
+  strided<8>(b0, b1, [&](const long* q, int cnt) { part(pts, N, q, cnt); });
   block_sum<1>(acc, sh);
   if (threadIdx.x == 0) p_L2[p] = acc[0];
 }

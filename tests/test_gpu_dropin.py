@@ -65,7 +65,11 @@ def test_reference_program_on_reference_data(P, O, gpu, tmp_path, cams_name, pts
     Trafalgar-21-11315) against the C-ABI hybrid on the same files: the same
     stop code, iteration count, final SSE and parameters bit for bit; initErr
     is the SSE at the entry of the last levmar call (PSBA/levmar.cpp:95).  The
-    oracle's hybrid solve reaches the same final error."""
+    oracle's hybrid solve stops with the same code after the same number of
+    iterations at the same final error (1e-12; measured <= 1.6e-14).  As in the
+    reference no camera is held, so the similarity gauge of the parameters is
+    free and they are compared through the cost, not one by one (the
+    oracle's and the GPU's parameters differ along it by up to 1.3e-3)."""
     d = run_dropin(os.path.join(GOLDEN, cams_name), os.path.join(GOLDEN, pts_name), tmp_path)
     pr = load_reference_problem(P, cams_name, pts_name)
     with P.Context(6, pr["n_cams"], pr["n_pts"], pr["n_obs"]) as g:
@@ -85,7 +89,8 @@ def test_reference_program_on_reference_data(P, O, gpu, tmp_path, cams_name, pts
     O.set_threads(16)
     o = O.Oracle(6, pr["K"], pr["q0"], pr["meas"], pr["iidx"], pr["jidx"], pr["cams"], pr["pts"], None)
     ro = o.hybrid()
-    assert abs(d["final"] - ro["final_err"]) <= 1e-6 * ro["final_err"]
+    assert (d["flag"], d["itno"]) == (ro["code"], ro["itno"])
+    assert abs(d["final"] - ro["final_err"]) <= 1e-12 * ro["final_err"]
 
 
 @pytest.mark.parametrize("cams_name,pts_name", REFERENCE_FILES)
