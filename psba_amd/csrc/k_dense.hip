@@ -884,9 +884,12 @@ __global__ __launch_bounds__(256) void k_chol_solve(double* __restrict__ A, int 
       __syncthreads();
       acc_update(acc, sm.X, (i != j) ? sm.Y : sm.X, i == j);
       if (j == k + 1) {
-        // store L_ik (the diagonal task stores it later, beside its forward
-        // solve); b_i -= L_ik y_k (4 threads per row, fixed order)
+        // store L_ik, and from the task (k+2, k+1) also L_{k+1,k} (its Y,
+        // the same bits as the diagonal task's X: the same tile, L_kk and
+        // reciprocals), which keeps that store off the diagonal chain;
+        // b_i -= L_ik y_k (4 threads per row, fixed order)
         if (i != j) store_tile(sm.X, tile(Lf, i, k), ld);
+        if (i == k + 2) store_tile(sm.Y, tile(Lf, k + 1, k), ld);
         if (!fwd) {
           double s = 0.0;
 #pragma unroll
@@ -911,9 +914,13 @@ __global__ __launch_bounds__(256) void k_chol_solve(double* __restrict__ A, int 
         __syncthreads();
         const int bad = tile_potrf(*(lds_smem*)&sm, LDS(Dt), carry);
         if (bad && tid == 0) __hip_atomic_fetch_or(fail, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        store_tile_w123(Dt, tile(Lf, i, i), ld);
-        store_tile_w123(sm.X, tile(Lf, i, k), ld);
-        if (!fwd) tile_forward(LDS(Dt), LDS(sm.part[0]), yv + (size_t)i * NB, LDS(sm.w));
+        if (fwd) {
+          store_tile(Dt, tile(Lf, i, i), ld);
+        } else {
+          store_tile_w123(Dt, tile(Lf, i, i), ld);
+          tile_forward(LDS(Dt), LDS(sm.part[0]), yv + (size_t)i * NB, LDS(sm.w));
+        }
+        if (i + 1 >= T) store_tile(sm.X, tile(Lf, i, k), ld);  // no task (k+2, k+1)
       } else {
         acc_store(acc, tile(A, i, j), ld);
       }
