@@ -392,6 +392,26 @@ __device__ __forceinline__ void load_full(double* dst, const double* src, int ld
   regs_to_lds(dst, v, false);
 }
 
+// A tile by waves 1-3 only (192 threads, element f = tid - 64 + 192 q,
+// q < 22), while wave 0 runs a tile solve: global -> registers, all loads
+// issued before the first is used, and registers -> LDS (stride TP; with
+// `lower`, zeros above the diagonal).
+__device__ __forceinline__ void tile_regs_w123(double (&v)[22], const double* src, int ld) {
+#pragma unroll
+  for (int q = 0; q < 22; ++q) {
+    const int f = (int)threadIdx.x - 64 + 192 * q;
+    v[q] = (f < NB * NB) ? src[(size_t)(f >> 6) * ld + (f & 63)] : 0.0;
+  }
+}
+
+__device__ __forceinline__ void regs_to_lds_w123(double* dst, const double (&v)[22], bool lower) {
+#pragma unroll
+  for (int q = 0; q < 22; ++q) {
+    const int f = (int)threadIdx.x - 64 + 192 * q, r = f >> 6, c = f & 63;
+    if (f < NB * NB) dst[r * TP + c] = (!lower || c <= r) ? v[q] : 0.0;
+  }
+}
+
 // store_tile by waves 1-3 only (wave 0 is free for a concurrent tile solve)
 __device__ __forceinline__ void store_tile_w123(const double* src, double* dst, int ld) {
   for (int f = (int)threadIdx.x - 64; f >= 0 && f < NB * NB; f += (int)blockDim.x - 64) {
@@ -1001,6 +1021,20 @@ __global__ __launch_bounds__(256) void k_chol_solve(double* __restrict__ A, int 
       if (k + 1 < T) br -= tile_tmatvec(sm, sm.X, TP, sm.w);
       __syncthreads();
       tile_backward(LDS(sm.P), br, xs + (size_t)k * NB, LDS(sm.v));
+      // Waves 1-3 return from tile_backward at once and load the next
+      // step's tiles while wave 0 solves: L_{k,k-1} and L_{k+1,k-1} straight
+      // into sm.X and sm.Y (the products above have read them), L_{k-1,k-1}
+      // into registers until wave 0 is done with sm.P.
+      double p0[22];
+      const bool loader = k >= 1 && tid >= 64;
+      if (loader) {
+        double p1[22], p2[22];
+        tile_regs_w123(p0, tile(Lf, k - 1, k - 1), ld);
+        tile_regs_w123(p1, tile(Lf, k, k - 1), ld);
+        if (k + 1 < T) tile_regs_w123(p2, tile(Lf, k + 1, k - 1), ld);
+        regs_to_lds_w123(sm.X, p1, false);
+        if (k + 1 < T) regs_to_lds_w123(sm.Y, p2, false);
+      }
       __syncthreads();
       if (tid < NB) {
         sm.rd[tid] = sm.w[tid];
@@ -1011,15 +1045,8 @@ __global__ __launch_bounds__(256) void k_chol_solve(double* __restrict__ A, int 
       if (k == 0) break;
       // x_k was stored by wave 0 (tile_backward): publish it
       if (tid == 0) flag_publish(xflag, epoch + (unsigned long long)(T - k));
-      double v0[16], v1[16], v2[16];
-      tile_regs(v0, tile(Lf, k - 1, k - 1), ld);
-      tile_regs(v1, tile(Lf, k, k - 1), ld);
-      if (k + 1 < T) tile_regs(v2, tile(Lf, k + 1, k - 1), ld);
+      if (loader) regs_to_lds_w123(sm.P, p0, true);
       r_next = r_pre;
-      __syncthreads();
-      regs_to_lds(sm.P, v0, true);
-      regs_to_lds(sm.X, v1, false);
-      if (k + 1 < T) regs_to_lds(sm.Y, v2, false);
       __syncthreads();
     }
   } else {
