@@ -559,6 +559,21 @@ struct Potrf16Scale<16> {
   static __device__ __forceinline__ void run(const double (&)[16], lds_dbl*, int, int, double, double) {}
 };
 
+// the factor's row r in place of the LDL^T row a, left of the diagonal (what
+// Potrf16Scale writes there); the entries from the diagonal on are left for
+// the caller (a[Q] * rq there is not the diagonal's d, nor zero above it)
+template <int Q>
+struct Potrf16Rows {
+  static __device__ __forceinline__ void run(double (&a)[16], double rdl) {
+    a[Q] = a[Q] * row_bcast<Q>(rdl);
+    Potrf16Rows<Q + 1>::run(a, rdl);
+  }
+};
+template <>
+struct Potrf16Rows<16> {
+  static __device__ __forceinline__ void run(double (&)[16], double) {}
+};
+
 // 16x16 Cholesky of B (LDS, stride TP) by one wave, lane = row (the four
 // 16-lane rows of the wave carry identical copies).  Outer-product LDL^T with
 // the pivot column broadcast by DPP row_newbcast (one reciprocal per pivot on
@@ -579,10 +594,82 @@ __device__ __forceinline__ void potrf16(lds_dbl* B, lds_dbl* rd16, __attribute__
   if (lane < 16) rd16[r] = rdl;
 }
 
+// The panel solve below the 16x16 block, in registers: x = A_r L_bb^-T for
+// this lane's panel row, right-looking (x_c *= 1/L_cc, then x_m -= x_c L_mc
+// for m > c).  1/L_cc and L_mc come from the lanes that hold row c / row m
+// of L_bb in the lane's own 16-lane row (DPP row_newbcast), not from LDS.
+template <int C, int M>
+struct PanelRow {
+  static __device__ __forceinline__ void update(double (&x)[16], const double (&Lr)[16]) {
+    x[M] = fma(-x[C], row_bcast<M>(Lr[C]), x[M]);
+    PanelRow<C, M + 1>::update(x, Lr);
+  }
+};
+template <int C>
+struct PanelRow<C, 16> {
+  static __device__ __forceinline__ void update(double (&)[16], const double (&)[16]) {}
+};
+template <int C>
+struct PanelStep {
+  static __device__ __forceinline__ void run(double (&x)[16], double (&Lr)[16], double rdl) {
+    // one step at a time (the empty asm pins Lr[C] here and x after the
+    // step): hoisting every step's broadcasts would hold 120 values in
+    // registers
+    __asm__ volatile("" : "+v"(Lr[C]));
+    x[C] = x[C] * row_bcast<C>(rdl);
+    PanelRow<C, C + 1>::update(x, Lr);
+#pragma unroll
+    for (int m = C; m < 16; ++m) __asm__ volatile("" : "+v"(x[m]));
+    PanelStep<C + 1>::run(x, Lr, rdl);
+  }
+};
+template <>
+struct PanelStep<16> {
+  static __device__ __forceinline__ void run(double (&)[16], double (&)[16], double) {}
+};
+
+// potrf16 on the diagonal block B (LDS, stride TP; its upper part is left
+// unspecified, tile_potrf clears it) and the panel solve of the R rows below
+// it, by one wave: lane l takes panel row 16 + l (l < R), whose
+// entries are loaded before the factor starts; every 16-lane row of the wave
+// keeps its copy of the factor (row l % 16 of L_bb) for the panel's
+// broadcasts.  The same operations in the same order as potrf16 followed by
+// the per-row panel solve from LDS, so the same results bit for bit.
+__device__ __forceinline__ void potrf16_panel(lds_dbl* B, int R, lds_dbl* rd16,
+                                              __attribute__((address_space(3))) int* bad) {
+  const int lane = threadIdx.x & 63;
+  const int r = lane & 15;
+  double x[16];
+#pragma unroll
+  for (int c = 0; c < 16; ++c) x[c] = (lane < R) ? B[(16 + lane) * TP + c] : 0.0;
+  double a[16];
+#pragma unroll
+  for (int q = 0; q < 16; ++q) a[q] = B[r * TP + q];
+  Potrf16Step<0>::run(a, lane, bad);
+  double dd = a[0];
+#pragma unroll
+  for (int q = 1; q < 16; ++q) dd = (r == q) ? a[q] : dd;
+  const double d = sqrt(dd), rdl = 1.0 / d;
+  Potrf16Rows<0>::run(a, rdl);  // a = row r of L_bb left of the diagonal
+  if (lane < 16) {
+    // the diagonal is d; what lands above it is cleared at the end of
+    // tile_potrf, and nothing reads it before
+#pragma unroll
+    for (int q = 0; q < 16; ++q) B[r * TP + q] = a[q];
+    B[r * TP + r] = d;
+    rd16[r] = rdl;
+  }
+  PanelStep<0>::run(x, a, rdl);
+  if (lane < R) {
+#pragma unroll
+    for (int c = 0; c < 16; ++c) B[(16 + lane) * TP + c] = x[c];
+  }
+}
+
 // Cholesky of the 64x64 LDS tile Yt (lower part), in place, zeros above; with
 // write_rd also 1 / diag(L) into sm.rd.
-// Right-looking over 16-column blocks: the diagonal block (one wave), the
-// panel below (one lane per row), the trailing lower blocks (MFMA, K = 16).
+// Right-looking over 16-column blocks: the diagonal block and the panel below
+// it (one wave, potrf16_panel), the trailing lower blocks (MFMA, K = 16).
 PSBA_OUTLINE int tile_potrf(lds_smem& sm, lds_dbl* Yt, bool write_rd) {
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   const int li = lane & 15, lk = lane >> 4;
@@ -591,41 +678,9 @@ PSBA_OUTLINE int tile_potrf(lds_smem& sm, lds_dbl* Yt, bool write_rd) {
 #pragma unroll 1
   for (int kb = 0; kb < 4; ++kb) {
     const int b0 = 16 * kb;
-    if (w == 0) potrf16(Yt + b0 * TP + b0, sm.rd16, &sm.bad);
+    if (w == 0) potrf16_panel(Yt + b0 * TP + b0, 48 - b0, sm.rd16, &sm.bad);
     __syncthreads();
     if (write_rd && threadIdx.x < 16) sm.rd[b0 + threadIdx.x] = sm.rd16[threadIdx.x];
-    const int R = 48 - b0;
-    if ((int)threadIdx.x < R) {
-      const int row = b0 + 16 + threadIdx.x;
-      // right-looking: column c is scaled, then every later column of the row
-      // takes its term (a_m - x_0 L[m][0] - x_1 L[m][1] - ..., ascending)
-      // column c + 1 of the diagonal block is read from LDS before column c's
-      // updates, so the read latency overlaps them
-      double x[16], Lc[16], Ln[16], rdv[16];
-#pragma unroll
-      for (int c = 0; c < 16; ++c) {
-        x[c] = Yt[row * TP + b0 + c];
-        rdv[c] = sm.rd16[c];
-        Lc[c] = Yt[(b0 + c) * TP + b0];
-      }
-#pragma unroll
-      for (int c = 0; c < 16; ++c) {
-        if (c + 1 < 16) {
-#pragma unroll
-          for (int m = c + 2; m < 16; ++m) Ln[m] = Yt[(b0 + m) * TP + b0 + c + 1];
-        }
-        x[c] = x[c] * rdv[c];
-#pragma unroll
-        for (int m = c + 1; m < 16; ++m) x[m] = fma(-x[c], Lc[m], x[m]);
-#pragma unroll
-        for (int m = c + 2; m < 16; ++m) Lc[m] = Ln[m];
-#pragma unroll
-        for (int m = c; m < 16; ++m) __asm__ volatile("" : "+v"(x[m]));
-      }
-#pragma unroll
-      for (int c = 0; c < 16; ++c) Yt[row * TP + b0 + c] = x[c];
-    }
-    __syncthreads();
     const int m = 3 - kb, nblk = m * (m + 1) / 2;
     for (int bi = w; bi < nblk; bi += 4) {
       int jb = 0, cum = 0;
