@@ -670,18 +670,24 @@ __device__ __forceinline__ void potrf16_panel(lds_dbl* B, int R, lds_dbl* rd16,
 // write_rd also 1 / diag(L) into sm.rd.
 // Right-looking over 16-column blocks: the diagonal block and the panel below
 // it (one wave, potrf16_panel), the trailing lower blocks (MFMA, K = 16).
-PSBA_OUTLINE int tile_potrf(lds_smem& sm, lds_dbl* Yt, bool write_rd) {
+// Only the first nkb blocks are eliminated: the rows and columns from 16 * nkb
+// on must be padding (unit diagonal, zeros elsewhere in the lower part), which
+// the first blocks' panels and updates leave as they are (+0 products added to
+// +0 and 1.0), so their factor is the identity already and 1/diag is 1.
+PSBA_OUTLINE int tile_potrf(lds_smem& sm, lds_dbl* Yt, bool write_rd, int nkb) {
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   const int li = lane & 15, lk = lane >> 4;
   if (threadIdx.x == 0) sm.bad = 0;
+  if (write_rd)
+    for (int q = 16 * nkb + (int)threadIdx.x; q < NB; q += blockDim.x) sm.rd[q] = 1.0;
   __syncthreads();
 #pragma unroll 1
-  for (int kb = 0; kb < 4; ++kb) {
+  for (int kb = 0; kb < nkb; ++kb) {
     const int b0 = 16 * kb;
     if (w == 0) potrf16_panel(Yt + b0 * TP + b0, 48 - b0, sm.rd16, &sm.bad);
     __syncthreads();
     if (write_rd && threadIdx.x < 16) sm.rd[b0 + threadIdx.x] = sm.rd16[threadIdx.x];
-    const int m = 3 - kb, nblk = m * (m + 1) / 2;
+    const int m = nkb - 1 - kb, nblk = m * (m + 1) / 2;
     for (int bi = w; bi < nblk; bi += 4) {
       int jb = 0, cum = 0;
       while (cum + (m - jb) <= bi) { cum += m - jb; ++jb; }
@@ -771,7 +777,11 @@ __device__ __forceinline__ void task_load(dvec4 (&acc)[2][2], CholSmem& sm, cons
 // L is loaded first, the solved value is broadcast by readlane, and every
 // lane updates unconditionally (a lane past its own row no longer matters,
 // entries above the diagonal are zero).  y -> yg (global) and ys (LDS).
-PSBA_OUTLINE void tile_forward(const lds_dbl* L, const lds_dbl* b, double* yg, lds_dbl* ys) {
+// The steps run in 16-row blocks, the blocks from row nr on only if nr does
+// not cover them: rows from nr on must be padding (unit diagonal, zero
+// coupling, zero right-hand side), whose steps give +0 and change no other
+// row, so skipping them gives the same values.
+PSBA_OUTLINE void tile_forward(const lds_dbl* L, const lds_dbl* b, double* yg, lds_dbl* ys, int nr) {
   if (threadIdx.x >= 64) return;
   const int lane = threadIdx.x;
   double Lr[NB];
@@ -780,17 +790,22 @@ PSBA_OUTLINE void tile_forward(const lds_dbl* L, const lds_dbl* b, double* yg, l
   double br = b[lane], yv = 0.0;
   const double rdl = 1.0 / L[lane * TP + lane];
 #pragma unroll
-  for (int m = 0; m < NB; ++m) {
-    const double ym = bcast(br * rdl, m);
-    yv = (lane == m) ? ym : yv;
-    br = fma(-Lr[m], ym, br);
+  for (int mb = 0; mb < 4; ++mb) {
+    if (16 * mb >= nr) break;
+#pragma unroll
+    for (int m = 16 * mb; m < 16 * mb + 16; ++m) {
+      const double ym = bcast(br * rdl, m);
+      yv = (lane == m) ? ym : yv;
+      br = fma(-Lr[m], ym, br);
+    }
   }
   yg[lane] = yv;
   ys[lane] = yv;
 }
 
-// x = L^-T r (L lower tile in LDS), wave 0, same scheme with the lane's column
-PSBA_OUTLINE void tile_backward(const lds_dbl* L, double br, double* xg, lds_dbl* xsm) {
+// x = L^-T r (L lower tile in LDS), wave 0, same scheme with the lane's column;
+// the padding rows from nr on (as for tile_forward) come first and are skipped
+PSBA_OUTLINE void tile_backward(const lds_dbl* L, double br, double* xg, lds_dbl* xsm, int nr) {
   if (threadIdx.x >= 64) return;
   const int lane = threadIdx.x;
   double Lc[NB];
@@ -799,10 +814,14 @@ PSBA_OUTLINE void tile_backward(const lds_dbl* L, double br, double* xg, lds_dbl
   double xv = 0.0;
   const double rdl = 1.0 / L[lane * TP + lane];
 #pragma unroll
-  for (int m = NB - 1; m >= 0; --m) {
-    const double xm = bcast(br * rdl, m);
-    xv = (lane == m) ? xm : xv;
-    br = fma(-Lc[m], xm, br);
+  for (int mb = 3; mb >= 0; --mb) {
+    if (16 * mb >= nr) continue;
+#pragma unroll
+    for (int m = 16 * mb + 15; m >= 16 * mb; --m) {
+      const double xm = bcast(br * rdl, m);
+      xv = (lane == m) ? xm : xv;
+      br = fma(-Lc[m], xm, br);
+    }
   }
   xg[lane] = xv;
   xsm[lane] = xv;
@@ -885,7 +904,7 @@ __device__ void forward_step(CholSmem& sm, const double* __restrict__ Lf, int ld
   }
   load_lower(sm.P, Lf + (size_t)k * NB * ld + (size_t)k * NB, ld);
   __syncthreads();
-  tile_forward(LDS(sm.P), LDS(sm.v), yv + (size_t)k * NB, LDS(sm.w));
+  tile_forward(LDS(sm.P), LDS(sm.v), yv + (size_t)k * NB, LDS(sm.w), NB);  // k < T - 1: no padding
   __syncthreads();
 }
 
@@ -903,6 +922,9 @@ __global__ __launch_bounds__(256) void k_chol_solve(double* __restrict__ A, int 
   const int tid = threadIdx.x;
   unsigned long long nbar = 0;
   auto tile = [&](double* base, int i, int j) { return base + (size_t)i * NB * ld + (size_t)j * NB; };
+  // rows of tile i below n (the rest, in the last tile, are identity padding)
+  auto rows_of = [&](int i) { return min(NB, n - i * NB); };
+  auto nkb_of = [&](int i) { return (rows_of(i) + 15) >> 4; };
   // workgroup 0 carries the diagonal chain: when the diagonal task is its last
   // task of the phase it factors straight into sm.P (with 1/diag in sm.rd) and
   // keeps y_k in sm.w, ready for the next phase
@@ -918,10 +940,10 @@ __global__ __launch_bounds__(256) void k_chol_solve(double* __restrict__ A, int 
     }
     if (tid < NB) sm.part[0][tid] = ea[tid];
     __syncthreads();
-    const int bad = tile_potrf(*(lds_smem*)&sm, LDS(sm.P), true);
+    const int bad = tile_potrf(*(lds_smem*)&sm, LDS(sm.P), true, nkb_of(0));
     if (bad && tid == 0) __hip_atomic_fetch_or(fail, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     store_tile_w123(sm.P, Lf, ld);
-    tile_forward(LDS(sm.P), LDS(sm.part[0]), yv, LDS(sm.w));
+    tile_forward(LDS(sm.P), LDS(sm.part[0]), yv, LDS(sm.w), rows_of(0));
     carry = true;
   }
   for (int q = NB + (int)blockIdx.x * (int)blockDim.x + tid; q < T * NB; q += G * (int)blockDim.x) bw[q] = ea[q];
@@ -987,13 +1009,13 @@ __global__ __launch_bounds__(256) void k_chol_solve(double* __restrict__ A, int 
         __syncthreads();
         acc_store(acc, Dt, TP);
         __syncthreads();
-        const int bad = tile_potrf(*(lds_smem*)&sm, LDS(Dt), carry);
+        const int bad = tile_potrf(*(lds_smem*)&sm, LDS(Dt), carry, nkb_of(i));
         if (bad && tid == 0) __hip_atomic_fetch_or(fail, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         if (fwd) {
           store_tile(Dt, tile(Lf, i, i), ld);
         } else {
           store_tile_w123(Dt, tile(Lf, i, i), ld);
-          tile_forward(LDS(Dt), LDS(sm.part[0]), yv + (size_t)i * NB, LDS(sm.w));
+          tile_forward(LDS(Dt), LDS(sm.part[0]), yv + (size_t)i * NB, LDS(sm.w), rows_of(i));
         }
         if (i + 1 >= T) store_tile(sm.X, tile(Lf, i, k), ld);  // no task (k+2, k+1)
       } else {
@@ -1028,7 +1050,7 @@ __global__ __launch_bounds__(256) void k_chol_solve(double* __restrict__ A, int 
       }
       load_lower(sm.P, tile(Lf, k, k), ld);
       __syncthreads();
-      tile_backward(LDS(sm.P), br, xs + (size_t)k * NB, LDS(sm.v));
+      tile_backward(LDS(sm.P), br, xs + (size_t)k * NB, LDS(sm.v), rows_of(k));
       __syncthreads();
       if (tid < NB) {
         const int q = k * NB + tid;
@@ -1056,7 +1078,7 @@ __global__ __launch_bounds__(256) void k_chol_solve(double* __restrict__ A, int 
       if (tid < NB)
         sm.v[tid] = bw[(size_t)(T - 1) * NB + tid] - ((sm.part[0][tid] + sm.part[1][tid]) + (sm.part[2][tid] + sm.part[3][tid]));
       __syncthreads();
-      tile_forward(LDS(sm.P), LDS(sm.v), yv + (size_t)(T - 1) * NB, LDS(sm.w));
+      tile_forward(LDS(sm.P), LDS(sm.v), yv + (size_t)(T - 1) * NB, LDS(sm.w), rows_of(T - 1));
       __syncthreads();
       r_next = sm.w[tid & 63];
     } else {
@@ -1075,7 +1097,7 @@ __global__ __launch_bounds__(256) void k_chol_solve(double* __restrict__ A, int 
       if (k + 2 < T) br -= tile_tmatvec(sm, sm.Y, TP, sm.rd);
       if (k + 1 < T) br -= tile_tmatvec(sm, sm.X, TP, sm.w);
       __syncthreads();
-      tile_backward(LDS(sm.P), br, xs + (size_t)k * NB, LDS(sm.v));
+      tile_backward(LDS(sm.P), br, xs + (size_t)k * NB, LDS(sm.v), rows_of(k));
       // Waves 1-3 return from tile_backward at once and load the next
       // step's tiles while wave 0 solves: L_{k,k-1} and L_{k+1,k-1} straight
       // into sm.X and sm.Y (the products above have read them), L_{k-1,k-1}
@@ -1203,7 +1225,7 @@ __global__ __launch_bounds__(256) void k_tri_solve(const double* __restrict__ Lf
     load_lower(sm.P, tile(i, i), ld);
     if (tid < NB) sm.v[tid] = ((q < n) ? b[q] : 0.0) - t;
     __syncthreads();
-    tile_forward(LDS(sm.P), LDS(sm.v), y + (size_t)i * NB, LDS(sm.vec) + (size_t)i * NB);
+    tile_forward(LDS(sm.P), LDS(sm.v), y + (size_t)i * NB, LDS(sm.vec) + (size_t)i * NB, NB);
     __syncthreads();
   }
   // backward: x_i = L_ii^-T (y_i - sum_{k>i} L_ki^T x_k); thread (lane, w) takes
@@ -1233,7 +1255,7 @@ __global__ __launch_bounds__(256) void k_tri_solve(const double* __restrict__ Lf
     const double br = sm.vec[i * NB + lane] - t;
     load_lower(sm.P, tile(i, i), ld);
     __syncthreads();
-    tile_backward(LDS(sm.P), br, x + (size_t)i * NB, LDS(sm.vec) + (size_t)i * NB);
+    tile_backward(LDS(sm.P), br, x + (size_t)i * NB, LDS(sm.vec) + (size_t)i * NB, NB);
     __syncthreads();
     if (tid < NB && i * NB + tid < n) out[i * NB + tid] = sm.vec[i * NB + tid];
   }
@@ -1312,9 +1334,9 @@ __global__ __launch_bounds__(256) void k_b_chol_solve(double* __restrict__ S, do
   }
   if (tid < NB) sm.part[0][tid] = (tid < n) ? b[tid] : 0.0;
   __syncthreads();
-  bad |= tile_potrf(*(lds_smem*)&sm, LDS(sm.Y), false);
+  bad |= tile_potrf(*(lds_smem*)&sm, LDS(sm.Y), false, 4);
   store_tile(sm.Y, L, ld);
-  tile_forward(LDS(sm.Y), LDS(sm.part[0]), yvp, LDS(sm.w));
+  tile_forward(LDS(sm.Y), LDS(sm.part[0]), yvp, LDS(sm.w), NB);
   __syncthreads();
 
   for (int k = 0; k + 1 < T; ++k) {
@@ -1357,9 +1379,9 @@ __global__ __launch_bounds__(256) void k_b_chol_solve(double* __restrict__ S, do
         __syncthreads();
         acc_store(acc, sm.Y, TP);
         __syncthreads();
-        bad |= tile_potrf(*(lds_smem*)&sm, LDS(sm.Y), false);
+        bad |= tile_potrf(*(lds_smem*)&sm, LDS(sm.Y), false, 4);
         store_tile(sm.Y, tile(L, i, i), ld);
-        tile_forward(LDS(sm.Y), LDS(sm.part[0]), yvp + (size_t)i * NB, LDS(sm.w));
+        tile_forward(LDS(sm.Y), LDS(sm.part[0]), yvp + (size_t)i * NB, LDS(sm.w), NB);
       } else {
         acc_store(acc, tile(A, i, j), ld);
       }
@@ -1374,7 +1396,7 @@ __global__ __launch_bounds__(256) void k_b_chol_solve(double* __restrict__ S, do
     double br = yvp[(size_t)k * NB + (tid & 63)];
     if (k + 1 < T) br -= tile_tmatvec(sm, tile(L, k + 1, k), ld, sm.w);
     __syncthreads();
-    tile_backward(LDS(sm.P), br, xsp + (size_t)k * NB, LDS(sm.v));
+    tile_backward(LDS(sm.P), br, xsp + (size_t)k * NB, LDS(sm.v), NB);
     __syncthreads();
     if (tid < NB) {
       const int q = k * NB + tid;
@@ -1438,14 +1460,14 @@ __global__ __launch_bounds__(256) void k_b_chol_init(double* __restrict__ S, dou
   }
   if (tid < NB) sm.part[0][tid] = (tid < n) ? b[tid] : 0.0;
   __syncthreads();
-  const int bad = tile_potrf(*(lds_smem*)&sm, LDS(sm.Y), false);
+  const int bad = tile_potrf(*(lds_smem*)&sm, LDS(sm.Y), false, 4);
   store_tile(sm.Y, L, ld);
-  tile_forward(LDS(sm.Y), LDS(sm.part[0]), yv + vec_off[p], LDS(sm.w));
+  tile_forward(LDS(sm.Y), LDS(sm.part[0]), yv + vec_off[p], LDS(sm.w), NB);
   __syncthreads();
   if (T == 1) {
     const double br = sm.w[tid & 63];
     __syncthreads();
-    tile_backward(LDS(sm.Y), br, xs + vec_off[p], LDS(sm.v));
+    tile_backward(LDS(sm.Y), br, xs + vec_off[p], LDS(sm.v), NB);
     __syncthreads();
     if (tid < n) dp[(size_t)cam_off[p] * cnp + tid] = sm.v[tid];
   }
@@ -1502,9 +1524,9 @@ __global__ __launch_bounds__(256) void k_b_chol_phase(int k, const int* __restri
     __syncthreads();
     acc_store(acc, sm.Y, TP);
     __syncthreads();
-    const int bad = tile_potrf(*(lds_smem*)&sm, LDS(sm.Y), false);
+    const int bad = tile_potrf(*(lds_smem*)&sm, LDS(sm.Y), false, 4);
     store_tile(sm.Y, tile(L, i, i), ld);
-    tile_forward(LDS(sm.Y), LDS(sm.part[0]), yvp + (size_t)i * NB, LDS(sm.w));
+    tile_forward(LDS(sm.Y), LDS(sm.part[0]), yvp + (size_t)i * NB, LDS(sm.w), NB);
     if (bad && tid == 0) fok[p] = 0.0;
   } else {
     acc_store(acc, tile(A, i, j), ld);
@@ -1537,7 +1559,7 @@ __global__ __launch_bounds__(256) void k_b_chol_back(int s, const int* __restric
     double br = yvp[(size_t)k * NB + (tid & 63)];
     if (s > 0) br -= tile_tmatvec(sm, tile(L, k + 1, k), ld, sm.w);
     __syncthreads();
-    tile_backward(LDS(sm.P), br, xsp + (size_t)k * NB, LDS(sm.v));
+    tile_backward(LDS(sm.P), br, xsp + (size_t)k * NB, LDS(sm.v), NB);
     __syncthreads();
     if (tid < NB) {
       const int q = k * NB + tid;
