@@ -669,7 +669,8 @@ __device__ __forceinline__ void potrf16_panel(lds_dbl* B, int R, lds_dbl* rd16,
 // Cholesky of the 64x64 LDS tile Yt (lower part), in place, zeros above; with
 // write_rd also 1 / diag(L) into sm.rd.
 // Right-looking over 16-column blocks: the diagonal block and the panel below
-// it (one wave, potrf16_panel), the trailing lower blocks (MFMA, K = 16).
+// it (one wave, potrf16_panel), the trailing lower blocks (MFMA, K = 16): the
+// next panel's at once, the others while wave 0 factors the next block.
 // Only the first nkb blocks are eliminated: the rows and columns from 16 * nkb
 // on must be padding (unit diagonal, zeros elsewhere in the lower part), which
 // the first blocks' panels and updates leave as they are (+0 products added to
@@ -681,29 +682,42 @@ PSBA_OUTLINE int tile_potrf(lds_smem& sm, lds_dbl* Yt, bool write_rd, int nkb) {
   if (write_rd)
     for (int q = 16 * nkb + (int)threadIdx.x; q < NB; q += blockDim.x) sm.rd[q] = 1.0;
   __syncthreads();
+  // block (ib, jc) -= L_{ib,cb} L_{jc,cb}^T, block column cb = b0 / 16 (MFMA, K = 16)
+  auto update = [&](int ib, int jc, int b0) {
+    dvec4 acc;
+#pragma unroll
+    for (int rr = 0; rr < 4; ++rr) acc[rr] = Yt[(16 * ib + 4 * rr + lk) * TP + 16 * jc + li];
+#pragma unroll
+    for (int k0 = 0; k0 < 16; k0 += 4) {
+      const double av = -Yt[(16 * ib + li) * TP + b0 + k0 + lk];
+      const double bv = Yt[(16 * jc + li) * TP + b0 + k0 + lk];
+      acc = __builtin_amdgcn_mfma_f64_16x16x4f64(av, bv, acc, 0, 0, 0);
+    }
+#pragma unroll
+    for (int rr = 0; rr < 4; ++rr) Yt[(16 * ib + 4 * rr + lk) * TP + 16 * jc + li] = acc[rr];
+  };
 #pragma unroll 1
   for (int kb = 0; kb < nkb; ++kb) {
     const int b0 = 16 * kb;
-    if (w == 0) potrf16_panel(Yt + b0 * TP + b0, 48 - b0, sm.rd16, &sm.bad);
+    if (w == 0) {
+      potrf16_panel(Yt + b0 * TP + b0, 48 - b0, sm.rd16, &sm.bad);
+    } else if (kb >= 1) {
+      // while wave 0 factors, waves 1-3 apply block column kb - 1 to the
+      // blocks right of this panel (columns kb + 1 on), which potrf16_panel
+      // neither reads nor writes; each block still takes block column kb - 1
+      // before block column kb
+      const int m = nkb - 1 - kb, nblk = m * (m + 1) / 2;
+      for (int bi = w - 1; bi < nblk; bi += 3) {
+        int jb = 0, cum = 0;
+        while (cum + (m - jb) <= bi) { cum += m - jb; ++jb; }
+        update(kb + 1 + jb + (bi - cum), kb + 1 + jb, b0 - 16);
+      }
+    }
     __syncthreads();
     if (write_rd && threadIdx.x < 16) sm.rd[b0 + threadIdx.x] = sm.rd16[threadIdx.x];
-    const int m = nkb - 1 - kb, nblk = m * (m + 1) / 2;
-    for (int bi = w; bi < nblk; bi += 4) {
-      int jb = 0, cum = 0;
-      while (cum + (m - jb) <= bi) { cum += m - jb; ++jb; }
-      const int ib = kb + 1 + jb + (bi - cum), jc = kb + 1 + jb;
-      dvec4 acc;
-#pragma unroll
-      for (int rr = 0; rr < 4; ++rr) acc[rr] = Yt[(16 * ib + 4 * rr + lk) * TP + 16 * jc + li];
-#pragma unroll
-      for (int k0 = 0; k0 < 16; k0 += 4) {
-        const double av = -Yt[(16 * ib + li) * TP + b0 + k0 + lk];
-        const double bv = Yt[(16 * jc + li) * TP + b0 + k0 + lk];
-        acc = __builtin_amdgcn_mfma_f64_16x16x4f64(av, bv, acc, 0, 0, 0);
-      }
-#pragma unroll
-      for (int rr = 0; rr < 4; ++rr) Yt[(16 * ib + 4 * rr + lk) * TP + 16 * jc + li] = acc[rr];
-    }
+    // the next panel (block column kb + 1) now; the blocks right of it wait
+    // for the next factor step
+    for (int ib = kb + 1 + w; ib < nkb; ib += 4) update(ib, kb + 1, b0);
     __syncthreads();
   }
   for (int f = threadIdx.x; f < NB * NB; f += blockDim.x) {
