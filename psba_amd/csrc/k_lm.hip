@@ -69,18 +69,30 @@ __global__ void k_lm_init(LmDev* s, unsigned char* gate, const double* __restric
 }
 
 // mu0 = tau * max(UVdiag) on the solve's first iteration (PSBA/levmar.cpp:113-120)
-__global__ __launch_bounds__(256) void k_lm_mu0(LmDev* s, const unsigned char* __restrict__ gate,
-                                                const double* __restrict__ uvdiag, long ntot) {
-  __shared__ double sh[4];
+// One workgroup of 1024 threads, eight loads in flight per thread (a maximum
+// does not depend on the order in which it is taken).
+__global__ __launch_bounds__(1024) void k_lm_mu0(LmDev* s, const unsigned char* __restrict__ gate,
+                                                 const double* __restrict__ uvdiag, long ntot) {
+  __shared__ double sh[16];
   if (!gate[kGateActive] || !s->first) return;
   double m = 0.0;
-  for (long q = threadIdx.x; q < ntot; q += blockDim.x) m = fmax(m, uvdiag[q]);
+  for (long q0 = threadIdx.x; q0 < ntot; q0 += 8L * blockDim.x) {
+    double v[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      const long q = q0 + (long)u * blockDim.x;
+      v[u] = q < ntot ? uvdiag[q] : 0.0;
+    }
+#pragma unroll
+    for (int u = 0; u < 8; ++u) m = fmax(m, v[u]);
+  }
 #pragma unroll
   for (int off = 32; off >= 1; off >>= 1) m = fmax(m, __shfl_xor(m, off, 64));
   if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = m;
   __syncthreads();
   if (threadIdx.x == 0) {
-    m = fmax(fmax(sh[0], sh[1]), fmax(sh[2], sh[3]));
+    m = sh[0];
+    for (int w = 1; w < (int)(blockDim.x >> 6); ++w) m = fmax(m, sh[w]);
     s->mu = kInitMu * m;
     s->first = 0;
     s->p_L2 = 1e+3;
@@ -188,7 +200,7 @@ int launch_lm_init(psba_ctx* c, int itno, int max_iter, double rel_tol, int no_h
 }
 
 int launch_lm_mu0(psba_ctx* c) {
-  hipLaunchKernelGGL(k_lm_mu0, dim3(1), dim3(256), 0, c->stream, c->lm, c->gate, c->uvdiag, (long)c->ntot);
+  hipLaunchKernelGGL(k_lm_mu0, dim3(1), dim3(1024), 0, c->stream, c->lm, c->gate, c->uvdiag, (long)c->ntot);
   PSBA_LAUNCH_TRY();
   return PSBA_OK;
 }
