@@ -61,7 +61,9 @@ def cpu_baseline(P, scene, gpu_trials, reps=5):
     iteration (Jacobian, normal equations, explicit-inverse step, trial cost)
     from the starting point of each scene, median of `reps` runs, on one
     thread and on all cores used here; configs 0, 1 (cnp 11), 2 and the
-    three request classes of config 4.  The headline value scales the
+    three request classes of config 4.  For configs 0, 1 and 4 also the
+    time to converge, median of `reps` full solves, beside the GPU's
+    (`gpu_ms_to_converge`, same stop rule).  The headline value scales the
     all-core iteration time of config 2 by the GPU solve's trial count."""
     from oracle import binding as O
     nproc = os.cpu_count() or 1
@@ -70,6 +72,14 @@ def cpu_baseline(P, scene, gpu_trials, reps=5):
              ("config4_small", P.Scene(4, 1004, problem=0, size_class=0)),
              ("config4_medium", P.Scene(4, 1004, problem=10000, size_class=1)),
              ("config4_large", P.Scene(4, 1004, problem=20000, size_class=2))]
+    # full solves to convergence beside the GPU's, where the CPU finishes in
+    # seconds: the configs' own stop rules (config 0 and 1: at most 50
+    # iterations; config 4: relative decrease < 1e-8 or 100 iterations), LM
+    # without the dogleg hand-off on both sides; 1-thread runs where cheap
+    converge = {"config0": (dict(max_iter=50), True), "config1": (dict(max_iter=50), False),
+                "config4_small": (dict(max_iter=100, rel_tol=1e-8), True),
+                "config4_medium": (dict(max_iter=100, rel_tol=1e-8), False),
+                "config4_large": (dict(max_iter=100, rel_tol=1e-8), False)}
     per = {}
     for name, sc in cases:
         row = {"n_cams": sc.n_cams, "n_pts": sc.n_pts, "n_obs": sc.n_obs}
@@ -82,13 +92,38 @@ def cpu_baseline(P, scene, gpu_trials, reps=5):
                 o.levmar(max_iter=1, solve_mode=O.SOLVE_INVERSE, flags=O.FLAG_NO_HANDOFF)
                 ts.append(time.perf_counter() - t0)
             row["ms_per_iteration_threads_%d" % th] = 1e3 * float(np.median(ts))
+        if name in converge:
+            kw, one_thread = converge[name]
+            g = P.Context.from_scene(sc)
+            g.snapshot()
+            gt = []
+            for _ in range(reps + 1):
+                g.restore()
+                g.itno = 0
+                t0 = time.perf_counter()
+                r = g.levmar(flags=P.FLAG_NO_HANDOFF, **kw)
+                gt.append(time.perf_counter() - t0)
+            g.close()
+            row["gpu_ms_to_converge"] = 1e3 * float(np.median(gt[1:]))
+            row["gpu_iterations"] = r["itno"]
+            for th in sorted({1, allc} if one_thread else {allc}):
+                O.set_threads(th)
+                ts = []
+                for _ in range(reps):
+                    o = O.Oracle.from_scene(sc)
+                    t0 = time.perf_counter()
+                    ro = o.levmar(solve_mode=O.SOLVE_INVERSE, flags=O.FLAG_NO_HANDOFF, **kw)
+                    ts.append(time.perf_counter() - t0)
+                row["ms_to_converge_threads_%d" % th] = 1e3 * float(np.median(ts))
+            row["cpu_iterations"] = ro["itno"]
         per[name] = row
-        log("cpu baseline %s: %s" % (name, ", ".join("%s %.1f" % kv for kv in row.items() if kv[0].startswith("ms"))))
+        log("cpu baseline %s: %s" % (name, ", ".join(("%s %.1f" % kv) if isinstance(kv[1], float) else "%s %s" % kv
+                                                     for kv in row.items() if kv[0].startswith(("ms", "gpu", "cpu")))))
     it_s = per["config2"]["ms_per_iteration_threads_%d" % allc] * 1e-3
     return {"value": 1.0 / (it_s * gpu_trials), "unit": "solves/s", "cores": allc, "kind": "port",
             "sample": "median of %d runs of one LM iteration (explicit-inverse step) of the config-2 scene on %d "
-                      "threads, %.3f s, scaled by the %d trials of the GPU solve; 1-thread and other configs in "
-                      "per_config" % (reps, allc, it_s, gpu_trials),
+                      "threads, %.3f s, scaled by the %d trials of the GPU solve; 1-thread, other configs and "
+                      "times to converge (CPU and GPU) in per_config" % (reps, allc, it_s, gpu_trials),
             "cpu_model": cpu_model(), "nproc": nproc, "per_config": per}
 
 
