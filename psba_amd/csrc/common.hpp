@@ -91,10 +91,11 @@ __device__ inline void grid_sum_partials(double (&v)[K], double* partials, doubl
   }
 }
 
+// One block adds nb blocks' K partials in block order (thread t takes blocks
+// t, t + blockDim, .., then block_sum) and writes out[0..K) from thread 0.
 template <int K>
-__global__ __launch_bounds__(256) void k_sum_partials(const double* __restrict__ partials, int nb,
-                                                      double* __restrict__ out) {
-  __shared__ double sh[K * 4];
+__device__ inline void sum_partials_block(const double* __restrict__ partials, int nb, double* __restrict__ out,
+                                          double* sh) {
   double acc[K];
 #pragma unroll
   for (int k = 0; k < K; ++k) acc[k] = 0;
@@ -107,6 +108,13 @@ __global__ __launch_bounds__(256) void k_sum_partials(const double* __restrict__
 #pragma unroll
     for (int k = 0; k < K; ++k) out[k] = acc[k];
   }
+}
+
+template <int K>
+__global__ __launch_bounds__(256) void k_sum_partials(const double* __restrict__ partials, int nb,
+                                                      double* __restrict__ out) {
+  __shared__ double sh[K * 4];
+  sum_partials_block<K>(partials, nb, out, sh);
 }
 
 // k_sum_partials that leaves `out` untouched unless *gate is set

@@ -201,6 +201,10 @@ int launch_lm_control(psba_ctx* c, LmDev* out);  // out: the state after the dec
 // gates of the launchers in device-control mode (nullptr otherwise)
 inline const unsigned char* gate_of(const psba_ctx* c, int which) { return c->devctl ? c->gate + which : nullptr; }
 inline const int* owner_of(const psba_ctx* c) { return c->devctl ? c->zero_owner : nullptr; }
+// Under device control a trial's cost partials (one per 256 observations) sit
+// in red_part after the step statistics' (two per 256 parameters), so that the
+// LM control kernel can sum both: red_part holds 8 per block of the larger.
+inline long trial_cost_part_off(const psba_ctx* c) { return 2L * ((c->ntot + 255) / 256); }
 inline const double* mu_of(const psba_ctx* c) { return c->devctl ? &c->lm->mu : nullptr; }
 
 // ---- launchers (k_tr.hip) ----

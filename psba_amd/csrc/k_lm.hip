@@ -106,10 +106,20 @@ __global__ __launch_bounds__(1024) void k_lm_mu0(LmDev* s, const unsigned char* 
 __device__ void lm_decide(LmDev* s, unsigned char* gate, const double* __restrict__ scal,
                           psba_trace_rec* __restrict__ trace);
 
-__global__ void k_lm_control(LmDev* s, unsigned char* gate, const double* __restrict__ scal,
-                             psba_trace_rec* __restrict__ trace, LmDev* out) {
-  lm_decide(s, gate, scal, trace);
-  *out = *s;
+// One block: first the trial's step statistics (|dp|^2 and the gain-ratio
+// denominator, from k_step_stats' partials) and cost (from k_residual's) are
+// summed into scal, as k_sum_partials<2> and <1> would, then thread 0 decides.
+__global__ __launch_bounds__(256) void k_lm_control(LmDev* s, unsigned char* gate, double* __restrict__ scal,
+                                                    psba_trace_rec* __restrict__ trace, LmDev* out,
+                                                    const double* __restrict__ step_part, int nb_step,
+                                                    const double* __restrict__ cost_part, int nb_cost) {
+  __shared__ double sh[8];
+  sum_partials_block<2>(step_part, nb_step, scal + SC_DP2, sh);
+  sum_partials_block<1>(cost_part, nb_cost, scal + SC_COST, sh);
+  if (threadIdx.x == 0) {
+    lm_decide(s, gate, scal, trace);
+    *out = *s;
+  }
 }
 
 __device__ void lm_decide(LmDev* s, unsigned char* gate, const double* __restrict__ scal,
@@ -206,7 +216,9 @@ int launch_lm_mu0(psba_ctx* c) {
 }
 
 int launch_lm_control(psba_ctx* c, LmDev* out) {
-  hipLaunchKernelGGL(k_lm_control, dim3(1), dim3(1), 0, c->stream, c->lm, c->gate, c->d_scal, c->lm_trace, out);
+  hipLaunchKernelGGL(k_lm_control, dim3(1), dim3(256), 0, c->stream, c->lm, c->gate, c->d_scal, c->lm_trace, out,
+                     (const double*)c->red_part, (int)ceil_div(c->ntot, 256),
+                     (const double*)(c->red_part + trial_cost_part_off(c)), (int)ceil_div(c->no, 256));
   PSBA_LAUNCH_TRY();
   return PSBA_OK;
 }

@@ -57,10 +57,14 @@ __global__ __launch_bounds__(256) void k_residual(const double* __restrict__ K, 
 template <int CNP>
 static void run_residual(psba_ctx* c, bool trial) {
   unsigned nb = ceil_div(c->no, 256);
+  // a trial under device control leaves its partials beside the step
+  // statistics' for the LM control kernel, which sums both
+  const bool to_control = trial && c->devctl;
+  double* part = to_control ? c->red_part + trial_cost_part_off(c) : c->red_part;
   hipLaunchKernelGGL(k_residual<CNP>, dim3(nb), dim3(256), 0, c->stream, c->K, c->q0, c->cams, c->pts,
                      trial ? c->dp : nullptr, c->n, c->meas, c->obs_cam, c->obs_pt, c->no,
-                     trial ? nullptr : c->e, c->red_part, c->loss, c->loss_scale);
-  launch_sum_partials<1>(c->stream, c->red_part, (int)nb, c->d_scal + SC_COST);
+                     trial ? nullptr : c->e, part, c->loss, c->loss_scale);
+  if (!to_control) launch_sum_partials<1>(c->stream, c->red_part, (int)nb, c->d_scal + SC_COST);
 }
 
 int launch_residual(psba_ctx* c, bool trial) {
@@ -548,7 +552,8 @@ int launch_step_stats(psba_ctx* c, double mu) {
   const unsigned nb = ceil_div(c->ntot, 256);
   hipLaunchKernelGGL(k_step_stats, dim3(nb), dim3(256), 0, c->stream, c->dp, c->g, (long)c->ntot, mu, mu_of(c),
                      c->red_part);
-  launch_sum_partials<2>(c->stream, c->red_part, (int)nb, c->d_scal + SC_DP2);
+  // under device control the LM control kernel sums these partials
+  if (!c->devctl) launch_sum_partials<2>(c->stream, c->red_part, (int)nb, c->d_scal + SC_DP2);
   PSBA_LAUNCH_TRY();
   return PSBA_OK;
 }
