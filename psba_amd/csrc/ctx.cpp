@@ -67,6 +67,8 @@ static void free_index(psba_ctx* c) {
   dfree(c->schunk_end);
   dfree(c->pair_schunk_ptr);
   dfree(c->schunk_order);
+  dfree(c->schunk_slot);
+  dfree(c->pair_pt);
   dfree(c->pt_order);
   dfree(c->lblock_pair);
   dfree(c->ucpart);
@@ -450,6 +452,8 @@ int alloc_index(psba_ctx* c, long nitems, int npairs, int n_cchunk, int n_schunk
   PSBA_TRY(dalloc(&c->schunk_end, n_schunk));
   PSBA_TRY(dalloc(&c->pair_schunk_ptr, npairs + 1));
   PSBA_TRY(dalloc(&c->schunk_order, n_schunk));
+  PSBA_TRY(dalloc(&c->schunk_slot, n_schunk));
+  PSBA_TRY(dalloc(&c->pair_pt, (size_t)nitems));
   PSBA_TRY(dalloc(&c->pt_order, np));
   PSBA_TRY(dalloc(&c->lblock_pair, nlb));
   if (c->mixed()) {
@@ -576,6 +580,7 @@ int psba_set_index(psba_ctx* c, const int* iidx, const int* jidx) {
   PSBA_TRY(up(c->lblock_pair, lbp.data(), sizeof(int) * nlb));
   PSBA_TRY(up(c->schunk_order, sord.data(), sizeof(int) * sord.size()));
   PSBA_TRY(build_point_order(c));
+  PSBA_TRY(build_schur_slots(c));
   PSBA_HIP_TRY(hipStreamSynchronize(c->stream));
   c->indexed = true;
   return PSBA_OK;

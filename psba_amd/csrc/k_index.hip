@@ -366,6 +366,7 @@ int psba_set_index_device(psba_ctx* c, const int* iidx, const int* jidx) {
   IX_TRY(hipGetLastError());
   IX_TRY(hipStreamSynchronize(st));  // the temporaries are freed on return
   PSBA_TRY(build_point_order(c));
+  PSBA_TRY(build_schur_slots(c));
   c->indexed = true;
   return PSBA_OK;
 }
@@ -380,6 +381,35 @@ __global__ void k_point_order_keys(const int* __restrict__ pt_ptr, int np, unsig
   if (i >= np) return;
   key[i] = ~(unsigned)(pt_ptr[i + 1] - pt_ptr[i]);  // ascending key = descending length
   val[i] = i;
+}
+
+__global__ void k_schur_slots(const int* __restrict__ order, const int* __restrict__ sb,
+                              const int* __restrict__ se, int n, int4* __restrict__ slot) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= n) return;
+  const int b = order[s];
+  slot[s] = make_int4(sb[b], se[b], b, 0);
+}
+
+__global__ void k_item_points(const int2* __restrict__ items, const int* __restrict__ obs_pt, long n,
+                              int* __restrict__ ipt) {
+  const long q = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q < n) ipt[q] = obs_pt[items[q].x];
+}
+
+// The launch slots of the Schur chunks and the points of the pair items, from
+// the finished index (host or device build alike).
+int build_schur_slots(psba_ctx* c) {
+  hipStream_t st = c->stream;
+  if (c->n_schunk > 0)
+    hipLaunchKernelGGL(k_schur_slots, dim3((c->n_schunk + 255) / 256), dim3(256), 0, st, c->schunk_order,
+                       c->schunk_begin, c->schunk_end, c->n_schunk, c->schunk_slot);
+  if (c->n_items > 0)
+    hipLaunchKernelGGL(k_item_points, dim3((unsigned)((c->n_items + 255) / 256)), dim3(256), 0, st, c->pair_items,
+                       c->obs_pt, (long)c->n_items, c->pair_pt);
+  PSBA_HIP_TRY(hipGetLastError());
+  PSBA_HIP_TRY(hipStreamSynchronize(st));
+  return PSBA_OK;
 }
 
 // Points by descending track length, ties in ascending id (a stable radix
