@@ -90,11 +90,11 @@ struct psba_ctx {
   int n_lblocks = 0;            // nc*(nc+1)/2 lower block positions
   int* lblock_pair = nullptr;   // pair index of each lower block position or -1
   int* schunk_order = nullptr;  // launch order of the pair chunks (schunk_launch_key)
-  // the pair chunks in launch order as (begin, end, chunk, 0), and each pair
-  // item's point: what k_schur_chunks reads first, without the lookups through
-  // schunk_order and obs_pt (built from the arrays above by build_schur_slots)
-  int4* schunk_slot = nullptr;
-  int* pair_pt = nullptr;
+  // the pair chunks' items in launch order, kPairChunk records per launch
+  // slot: (obs in camera k, obs in camera l, point, chunk), then (-1, -1, -1,
+  // chunk) past the chunk's items; what k_schur_chunks reads first, at an
+  // address its workgroup index gives (built by build_schur_slots)
+  int4* slot_items = nullptr;
   int* pt_order = nullptr;      // points by descending track length (ties: ascending id)
   double sum_t2 = 0;            // sum over points of track length squared
   double entry_cost = 0;        // SSE at the entry of the last psba_levmar call
@@ -181,7 +181,7 @@ int launch_axpby(psba_ctx* c, double* out, double alpha, const double* a, double
 // ---- launchers (k_schur.hip) ----
 int launch_schur(psba_ctx* c, double mu);                 // S (lower), ea
 int build_point_order(psba_ctx* c);                        // pt_order (index builds)
-int build_schur_slots(psba_ctx* c);                        // schunk_slot, pair_pt (index builds)
+int build_schur_slots(psba_ctx* c);                        // slot_items (index builds)
 int launch_ea_rhs(psba_ctx* c, const double* ga, const double* z, double* out);  // ga - sum W z
 
 // ---- launchers (k_dense.hip) ----

@@ -383,30 +383,29 @@ __global__ void k_point_order_keys(const int* __restrict__ pt_ptr, int np, unsig
   val[i] = i;
 }
 
-__global__ void k_schur_slots(const int* __restrict__ order, const int* __restrict__ sb,
-                              const int* __restrict__ se, int n, int4* __restrict__ slot) {
-  const int s = blockIdx.x * blockDim.x + threadIdx.x;
-  if (s >= n) return;
-  const int b = order[s];
-  slot[s] = make_int4(sb[b], se[b], b, 0);
+__global__ void k_slot_items(const int* __restrict__ order, const int* __restrict__ sb, const int* __restrict__ se,
+                             const int2* __restrict__ items, const int* __restrict__ obs_pt, int n,
+                             int4* __restrict__ rec) {
+  const long g = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= (long)n * kPairChunk) return;
+  const int s = (int)(g / kPairChunk), t = (int)(g - (long)s * kPairChunk);
+  const int b = order[s], beg = sb[b];
+  if (t < se[b] - beg) {
+    const int2 pr = items[beg + t];
+    rec[g] = make_int4(pr.x, pr.y, obs_pt[pr.x], b);
+  } else {
+    rec[g] = make_int4(-1, -1, -1, b);
+  }
 }
 
-__global__ void k_item_points(const int2* __restrict__ items, const int* __restrict__ obs_pt, long n,
-                              int* __restrict__ ipt) {
-  const long q = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (q < n) ipt[q] = obs_pt[items[q].x];
-}
-
-// The launch slots of the Schur chunks and the points of the pair items, from
-// the finished index (host or device build alike).
+// The Schur chunks' item records in launch order, from the finished index
+// (host or device build alike).
 int build_schur_slots(psba_ctx* c) {
   hipStream_t st = c->stream;
   if (c->n_schunk > 0)
-    hipLaunchKernelGGL(k_schur_slots, dim3((c->n_schunk + 255) / 256), dim3(256), 0, st, c->schunk_order,
-                       c->schunk_begin, c->schunk_end, c->n_schunk, c->schunk_slot);
-  if (c->n_items > 0)
-    hipLaunchKernelGGL(k_item_points, dim3((unsigned)((c->n_items + 255) / 256)), dim3(256), 0, st, c->pair_items,
-                       c->obs_pt, (long)c->n_items, c->pair_pt);
+    hipLaunchKernelGGL(k_slot_items, dim3((unsigned)(((long)c->n_schunk * kPairChunk + 255) / 256)), dim3(256), 0,
+                       st, c->schunk_order, c->schunk_begin, c->schunk_end, c->pair_items, c->obs_pt, c->n_schunk,
+                       c->slot_items);
   PSBA_HIP_TRY(hipGetLastError());
   PSBA_HIP_TRY(hipStreamSynchronize(st));
   return PSBA_OK;

@@ -27,17 +27,17 @@ namespace psba {
 //   4. the G group partials are summed in group order into part[b].
 // The order of every sum is fixed by the chunk, so results do not depend on
 // scheduling.
-// With SLOTS (one problem): workgroup slot s takes slot[s] = (begin, end,
-// chunk) and the items' points from item_pt, so its first loads are one slot
-// and then the items, with no lookups through order, sb/se and obs_pt between.
+// With SLOTS (one problem): the workgroup of launch slot s reads its chunk's
+// item records at rec[s * kPairChunk ..] (observations, point, chunk; -1 past
+// the chunk's items), an address its index gives, so the W rows and V^-1 are
+// its second round of loads.
 template <int CNP, typename R, bool SLOTS>
 __global__ __launch_bounds__(256) void k_schur_chunks(const int2* __restrict__ items, const int* __restrict__ sb,
                                                       const int* __restrict__ se, const int* __restrict__ obs_pt,
                                                       const R* __restrict__ W, const double* __restrict__ Vinv,
                                                       R* __restrict__ part, const int* __restrict__ ownr,
                                                       const unsigned char* __restrict__ run, int nchunk,
-                                                      const int* __restrict__ order, const int4* __restrict__ slot,
-                                                      const int* __restrict__ item_pt) {
+                                                      const int* __restrict__ order, const int4* __restrict__ rec) {
   constexpr int C2 = CNP * CNP, REC = 3 * CNP;
   constexpr int NBK = (CNP + 2) / 3;  // 3-wide row / column blocks
   constexpr int NB2 = NBK * NBK;
@@ -45,6 +45,7 @@ __global__ __launch_bounds__(256) void k_schur_chunks(const int2* __restrict__ i
   static_assert(G * C2 <= 2 * kPairChunk * REC, "group partials must fit the staging buffer");
   __shared__ R sBuf[2 * kPairChunk * REC];
   __shared__ int sIdx[3 * kPairChunk];
+  __shared__ int sMeta[2];
   R* const sY = sBuf;
   R* const sW = sBuf + kPairChunk * REC;
 #if PSBA_SCHUR_XCD
@@ -53,26 +54,37 @@ __global__ __launch_bounds__(256) void k_schur_chunks(const int2* __restrict__ i
 #else
   int b = blockIdx.x;
 #endif
-  int beg, cnt;
+  const int t = threadIdx.x;
+  int cnt;
   if constexpr (SLOTS) {
-    const int4 si = slot[b];
-    b = si.z;
-    beg = si.x;
-    cnt = si.y - si.x;
+    if (t < kPairChunk) {
+      const int4 r = rec[(size_t)b * kPairChunk + t];
+      sIdx[t] = r.x;
+      sIdx[kPairChunk + t] = r.y;
+      sIdx[2 * kPairChunk + t] = r.z;
+      // the chunk's items are the records before the first -1
+      const unsigned long long m = __ballot(r.x >= 0);
+      if (t == 0) {
+        sMeta[0] = r.w;
+        sMeta[1] = __popcll(m);
+      }
+    }
+    __syncthreads();
+    b = sMeta[0];
+    cnt = sMeta[1];
   } else {
     if (order) b = order[b];
     if (run && !run[ownr[b]]) return;
-    beg = sb[b];
+    const int beg = sb[b];
     cnt = se[b] - beg;
+    if (t < cnt) {
+      const int2 pr = items[beg + t];
+      sIdx[t] = pr.x;
+      sIdx[kPairChunk + t] = pr.y;
+      sIdx[2 * kPairChunk + t] = obs_pt[pr.x];
+    }
+    __syncthreads();
   }
-  const int t = threadIdx.x;
-  if (t < cnt) {
-    const int2 pr = items[beg + t];
-    sIdx[t] = pr.x;
-    sIdx[kPairChunk + t] = pr.y;
-    sIdx[2 * kPairChunk + t] = SLOTS ? item_pt[beg + t] : obs_pt[pr.x];
-  }
-  __syncthreads();
   {
     const int it = t >> 2, q = t & 3;
     if (it < cnt) {
@@ -244,7 +256,7 @@ static void run_schur_t(psba_ctx* c, double mu, const R* W, R* schpart, R* eapar
     hipLaunchKernelGGL((k_schur_chunks<CNP, R, true>), dim3(PSBA_SCHUR_XCD ? xcd_grid(c->n_schunk) : c->n_schunk),
                        dim3(256), 0, c->stream, c->pair_items, c->schunk_begin, c->schunk_end, c->obs_pt, W, c->Vinv,
                        schpart, (const int*)nullptr, (const unsigned char*)nullptr, c->n_schunk,
-                       (const int*)c->schunk_order, (const int4*)c->schunk_slot, (const int*)c->pair_pt);
+                       (const int*)c->schunk_order, (const int4*)c->slot_items);
     probe_end(c, PSBA_PROBE_SCHUR);
   }
   hipLaunchKernelGGL((k_schur_finish<CNP, R>), dim3(c->n_lblocks + (c->n_pad - c->n)),
@@ -374,8 +386,7 @@ static void run_b_schur_t(psba_batch* b) {
   if (b->n_schunk > 0)
     hipLaunchKernelGGL((k_schur_chunks<CNP, R, false>), dim3(PSBA_SCHUR_XCD ? xcd_grid(b->n_schunk) : b->n_schunk),
                        dim3(256), 0, b->stream, b->items, b->schunk_b, b->schunk_e, b->obs_pt, W, b->Vinv, schpart,
-                       b->schunk_prob, b->active, b->n_schunk, (const int*)nullptr, (const int4*)nullptr,
-                       (const int*)nullptr);
+                       b->schunk_prob, b->active, b->n_schunk, (const int*)nullptr, (const int4*)nullptr);
   if (b->nlb > 0)
     hipLaunchKernelGGL((k_b_schur_finish<CNP, R>), dim3(b->nlb), dim3(CNP * CNP > 128 ? 256 : 128), 0, b->stream,
                        b->lb_prob, b->lb_k, b->lb_l, b->lb_pair, b->psp, schpart, b->U, b->fix, b->mu, b->cam_off,
