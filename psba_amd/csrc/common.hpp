@@ -203,6 +203,12 @@ __device__ __forceinline__ void loss_eval(int kind, double d, double s, double& 
   }
 }
 
+// A problem's gate (batch: per problem through ownr; one problem: ownr may be
+// null, the gate is run[0]).
+__device__ inline bool gated_off(const unsigned char* run, const int* ownr, int i) {
+  return run && !run[ownr ? ownr[i] : 0];
+}
+
 // Workgroups of a launch go to the 8 XCDs round-robin (workgroup b to XCD
 // b % 8), each XCD with its own L2.  A grid of xcd_grid(n) workgroups with
 // item = xcd_item(b, n) gives every XCD one contiguous eighth of an n-item

@@ -95,6 +95,10 @@ struct psba_ctx {
   // chunk) past the chunk's items; what k_schur_chunks reads first, at an
   // address its workgroup index gives (built by build_schur_slots)
   int4* slot_items = nullptr;
+  // the camera chunks' observations and their points, cam_chunk_size(cnp)
+  // records per chunk ((-1, -1) past the chunk's end), read at an address the
+  // workgroup index gives (built by build_schur_slots)
+  int2* cchunk_rec = nullptr;
   int* pt_order = nullptr;      // points by descending track length (ties: ascending id)
   double sum_t2 = 0;            // sum over points of track length squared
   double entry_cost = 0;        // SSE at the entry of the last psba_levmar call
@@ -181,7 +185,7 @@ int launch_axpby(psba_ctx* c, double* out, double alpha, const double* a, double
 // ---- launchers (k_schur.hip) ----
 int launch_schur(psba_ctx* c, double mu);                 // S (lower), ea
 int build_point_order(psba_ctx* c);                        // pt_order (index builds)
-int build_schur_slots(psba_ctx* c);                        // slot_items (index builds)
+int build_schur_slots(psba_ctx* c);                        // slot_items, cchunk_rec (index builds)
 int launch_ea_rhs(psba_ctx* c, const double* ga, const double* z, double* out);  // ga - sum W z
 
 // ---- launchers (k_dense.hip) ----

@@ -68,6 +68,7 @@ static void free_index(psba_ctx* c) {
   dfree(c->pair_schunk_ptr);
   dfree(c->schunk_order);
   dfree(c->slot_items);
+  dfree(c->cchunk_rec);
   dfree(c->pt_order);
   dfree(c->lblock_pair);
   dfree(c->ucpart);
@@ -452,6 +453,7 @@ int alloc_index(psba_ctx* c, long nitems, int npairs, int n_cchunk, int n_schunk
   PSBA_TRY(dalloc(&c->pair_schunk_ptr, npairs + 1));
   PSBA_TRY(dalloc(&c->schunk_order, n_schunk));
   PSBA_TRY(dalloc(&c->slot_items, (size_t)n_schunk * kPairChunk));
+  PSBA_TRY(dalloc(&c->cchunk_rec, (size_t)n_cchunk * cam_chunk_size(C)));
   PSBA_TRY(dalloc(&c->pt_order, np));
   PSBA_TRY(dalloc(&c->lblock_pair, nlb));
   if (c->mixed()) {

@@ -398,10 +398,28 @@ __global__ void k_slot_items(const int* __restrict__ order, const int* __restric
   }
 }
 
-// The Schur chunks' item records in launch order, from the finished index
-// (host or device build alike).
+__global__ void k_cchunk_recs(const int* __restrict__ cam_obs, const int* __restrict__ cb, const int* __restrict__ ce,
+                              const int* __restrict__ obs_pt, int n, int ch, int2* __restrict__ rec) {
+  const long g = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= (long)n * ch) return;
+  const int b = (int)(g / ch), s = (int)(g - (long)b * ch);
+  const int beg = cb[b];
+  if (s < ce[b] - beg) {
+    const int o = cam_obs[beg + s];
+    rec[g] = make_int2(o, obs_pt[o]);
+  } else {
+    rec[g] = make_int2(-1, -1);
+  }
+}
+
+// The Schur chunks' item records in launch order and the camera chunks'
+// records, from the finished index (host or device build alike).
 int build_schur_slots(psba_ctx* c) {
   hipStream_t st = c->stream;
+  const int ch = cam_chunk_size(c->cnp);
+  if (c->n_cchunk > 0)
+    hipLaunchKernelGGL(k_cchunk_recs, dim3((unsigned)(((long)c->n_cchunk * ch + 255) / 256)), dim3(256), 0, st,
+                       c->cam_obs, c->cchunk_begin, c->cchunk_end, c->obs_pt, c->n_cchunk, ch, c->cchunk_rec);
   if (c->n_schunk > 0)
     hipLaunchKernelGGL(k_slot_items, dim3((unsigned)(((long)c->n_schunk * kPairChunk + 255) / 256)), dim3(256), 0,
                        st, c->schunk_order, c->schunk_begin, c->schunk_end, c->pair_items, c->obs_pt, c->n_schunk,

@@ -223,18 +223,24 @@ __global__ __launch_bounds__(256) void k_cam_chunks(const int* __restrict__ cam_
                                                     const int* __restrict__ ce, const R* __restrict__ JA,
                                                     const R* __restrict__ e, R* __restrict__ part,
                                                     const int* __restrict__ ownr,
-                                                    const unsigned char* __restrict__ run) {
+                                                    const unsigned char* __restrict__ run,
+                                                    const int2* __restrict__ rec) {
   constexpr int NU = CNP * (CNP + 1) / 2, NE = NU + CNP, REC = 2 * CNP + 2;
   constexpr int CH = (CNP > 11) ? 128 : kCamChunk;
+  static_assert(CH <= 256, "one observation per thread");
   __shared__ R sJ[CH * REC];
   const int b = blockIdx.x;
-  if (run && !run[ownr[b]]) return;
+  // with rec (one problem) the observation comes from the chunk's record at
+  // an address the workgroup index gives, loaded beside the chunk's extent
+  // and the gate
+  const int o_rec = (rec && (int)threadIdx.x < CH) ? rec[(size_t)b * CH + threadIdx.x].x : -1;
   const int beg = cb[b], cnt = ce[b] - beg;
+  if (gated_off(run, ownr, b)) return;
   // one thread per observation stages its Jacobian rows and residual (one
   // round of dependent loads for the whole chunk, all of a record's loads in
   // flight together)
   for (int s = threadIdx.x; s < cnt; s += blockDim.x) {
-    const int o = cam_obs[beg + s];
+    const int o = rec ? o_rec : cam_obs[beg + s];
     const R* ja = JA + (size_t)o * 2 * CNP;
     R* dst = sJ + s * REC;
 #pragma unroll
@@ -317,7 +323,8 @@ static void run_jac_normal_t(psba_ctx* c, double coeff, double coeff_g, R* e, R*
                      gate_of(c, kGateJac));
   if (c->n_cchunk > 0)
     hipLaunchKernelGGL((k_cam_chunks<CNP, R>), dim3(c->n_cchunk), dim3(256), 0, c->stream, c->cam_obs,
-                       c->cchunk_begin, c->cchunk_end, JA, e, ucpart, owner_of(c), gate_of(c, kGateJac));
+                       c->cchunk_begin, c->cchunk_end, JA, e, ucpart, (const int*)nullptr, gate_of(c, kGateJac),
+                       (const int2*)c->cchunk_rec);
   hipLaunchKernelGGL((k_cam_finish<CNP, R>), dim3(c->nc), dim3(256), 0, c->stream, c->cam_cchunk_ptr, ucpart, c->nc,
                      coeff, coeff_g, c->U, c->g, c->uvdiag, owner_of(c), gate_of(c, kGateJac));
 }
@@ -972,7 +979,7 @@ static void run_b_jac_t(psba_batch* b) {
                      (int)b->NP, 1.0, 1.0, b->V, b->g + b->N, b->uvdiag + b->N, b->pt_prob, b->need_jac);
   if (b->n_cchunk > 0)
     hipLaunchKernelGGL((k_cam_chunks<CNP, R>), dim3(b->n_cchunk), dim3(256), 0, b->stream, b->cam_obs, b->cchunk_b,
-                       b->cchunk_e, JA, e, ucpart, b->cchunk_prob, b->need_jac);
+                       b->cchunk_e, JA, e, ucpart, b->cchunk_prob, b->need_jac, (const int2*)nullptr);
   hipLaunchKernelGGL((k_cam_finish<CNP, R>), dim3(b->NC), dim3(256), 0, b->stream, b->ccp, ucpart, (int)b->NC, 1.0,
                      1.0, b->U, b->g, b->uvdiag, b->cam_prob, b->need_jac);
 }

@@ -200,19 +200,29 @@ __global__ __launch_bounds__(256) void k_ea_chunks(const int* __restrict__ cam_o
                                                    const int* __restrict__ ce, const int* __restrict__ obs_pt,
                                                    const R* __restrict__ W, const double* __restrict__ z,
                                                    R* __restrict__ part, const int* __restrict__ ownr,
-                                                   const unsigned char* __restrict__ run) {
+                                                   const unsigned char* __restrict__ run,
+                                                   const int2* __restrict__ rec) {
   constexpr int CH = (CNP > 11) ? 128 : kCamChunk;
+  static_assert(CH <= 256, "one observation per thread");
   __shared__ R sP[CH * CNP];
   const int b = blockIdx.x;
-  if (run && !run[ownr[b]]) return;
+  // with rec (one problem) the observation and its point come from the
+  // chunk's record at an address the workgroup index gives, beside the extent
+  const int2 r_rec = (rec && (int)threadIdx.x < CH) ? rec[(size_t)b * CH + threadIdx.x] : make_int2(-1, -1);
   const int beg = cb[b], cnt = ce[b] - beg;
+  if (gated_off(run, ownr, b)) return;
   // the chunk's observation and point indices first (the dependent loads once
   // per observation), then the products with independent, coalesced loads
   __shared__ int sO[CH], sI[CH];
   for (int s = threadIdx.x; s < cnt; s += blockDim.x) {
-    const int o = cam_obs[beg + s];
-    sO[s] = o;
-    sI[s] = obs_pt[o];
+    if (rec) {
+      sO[s] = r_rec.x;
+      sI[s] = r_rec.y;
+    } else {
+      const int o = cam_obs[beg + s];
+      sO[s] = o;
+      sI[s] = obs_pt[o];
+    }
   }
   __syncthreads();
 #pragma unroll 4
@@ -265,7 +275,7 @@ static void run_schur_t(psba_ctx* c, double mu, const R* W, R* schpart, R* eapar
   if (c->n_cchunk > 0)
     hipLaunchKernelGGL((k_ea_chunks<CNP, R>), dim3(c->n_cchunk), dim3(256), 0, c->stream, c->cam_obs,
                        c->cchunk_begin, c->cchunk_end, c->obs_pt, W, c->z, eapart, (const int*)nullptr,
-                       (const unsigned char*)nullptr);
+                       (const unsigned char*)nullptr, (const int2*)c->cchunk_rec);
   hipLaunchKernelGGL((k_ea_finish<CNP, R>), dim3(ceil_div((long)c->nc * CNP, 64)), dim3(64), 0, c->stream,
                      c->cam_cchunk_ptr, eapart, c->g, c->nc, c->ea, (const int*)nullptr,
                      (const unsigned char*)nullptr);
@@ -286,7 +296,7 @@ static void run_ea_rhs(psba_ctx* c, const double* ga, const double* z, double* o
     if (c->n_cchunk > 0)
       hipLaunchKernelGGL((k_ea_chunks<CNP, float>), dim3(c->n_cchunk), dim3(256), 0, c->stream, c->cam_obs,
                          c->cchunk_begin, c->cchunk_end, c->obs_pt, c->W32, z, c->eapart32, (const int*)nullptr,
-                         (const unsigned char*)nullptr);
+                         (const unsigned char*)nullptr, (const int2*)c->cchunk_rec);
     hipLaunchKernelGGL((k_ea_finish<CNP, float>), dim3(ceil_div((long)c->nc * CNP, 64)), dim3(64), 0, c->stream,
                        c->cam_cchunk_ptr, c->eapart32, ga, c->nc, out, (const int*)nullptr,
                        (const unsigned char*)nullptr);
@@ -294,7 +304,7 @@ static void run_ea_rhs(psba_ctx* c, const double* ga, const double* z, double* o
     if (c->n_cchunk > 0)
       hipLaunchKernelGGL((k_ea_chunks<CNP, double>), dim3(c->n_cchunk), dim3(256), 0, c->stream, c->cam_obs,
                          c->cchunk_begin, c->cchunk_end, c->obs_pt, c->W, z, c->eapart, (const int*)nullptr,
-                         (const unsigned char*)nullptr);
+                         (const unsigned char*)nullptr, (const int2*)c->cchunk_rec);
     hipLaunchKernelGGL((k_ea_finish<CNP, double>), dim3(ceil_div((long)c->nc * CNP, 64)), dim3(64), 0, c->stream,
                        c->cam_cchunk_ptr, c->eapart, ga, c->nc, out, (const int*)nullptr,
                        (const unsigned char*)nullptr);
@@ -395,7 +405,7 @@ static void run_b_schur_t(psba_batch* b) {
                      b->S);
   if (b->n_cchunk > 0)
     hipLaunchKernelGGL((k_ea_chunks<CNP, R>), dim3(b->n_cchunk), dim3(256), 0, b->stream, b->cam_obs, b->cchunk_b,
-                       b->cchunk_e, b->obs_pt, W, b->z, eapart, b->cchunk_prob, b->active);
+                       b->cchunk_e, b->obs_pt, W, b->z, eapart, b->cchunk_prob, b->active, (const int2*)nullptr);
   hipLaunchKernelGGL((k_ea_finish<CNP, R>), dim3(ceil_div(b->NC * CNP, 64)), dim3(64), 0, b->stream, b->ccp, eapart,
                      b->g, (int)b->NC, b->ea, b->cam_prob, b->active);
 }
