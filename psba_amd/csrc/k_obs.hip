@@ -471,27 +471,43 @@ __global__ __launch_bounds__(256) void k_backsub(const int* __restrict__ pt_ptr,
   const int i = (in && order) ? order[slot] : slot;
   const bool ok = in && (!run || run[ownr[i]]);
   R s = 0;
-  if (ok) {
-    // camera indices are fetched two observations ahead, so each step waits
-    // for one round of loads (W and dpa) instead of two dependent ones
-    const int beg = pt_ptr[i], end = pt_ptr[i + 1];
-    int jc = beg < end ? obs_cam[beg] : 0;
-    int jn = beg + 1 < end ? obs_cam[beg + 1] : 0;
-#pragma unroll 2
-    for (int o = beg; o < end; ++o) {
-      const int jnn = o + 2 < end ? obs_cam[o + 2] : 0;
+  {
+    // Uniform control flow: every lane of the wave steps through the wave's
+    // longest track; a lane past its own track (or without a point) loads a
+    // valid row and keeps its sum.  The W values and camera index of the next
+    // PF steps are in registers before they are used (unconditional loads, so
+    // the waits count exactly), and the running sum keeps its order.
+    constexpr int PF = 2;
+    const int beg = ok ? pt_ptr[i] : 0, len = ok ? pt_ptr[i + 1] - beg : 0;
+    int lmax = len;
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) lmax = max(lmax, __shfl_xor(lmax, off, 64));
+    R wb[PF][CNP];
+    int cb[PF];
+    auto fetch = [&](int u, int st) {
+      const int o = beg + max(0, min(st, len - 1));
+      cb[u] = obs_cam[o];
       const R* w = W + (size_t)o * 3 * CNP + cc;
-      if constexpr (LDS_DPA) {
-        const double* da = sDa + jc * CNP;
 #pragma unroll
-        for (int k = 0; k < CNP; ++k) s += w[3 * k] * (R)da[k];
-      } else {
-        const double* da = dp + (size_t)jc * CNP;
+      for (int k = 0; k < CNP; ++k) wb[u][k] = w[3 * k];
+    };
 #pragma unroll
-        for (int k = 0; k < CNP; ++k) s += w[3 * k] * (R)da[k];
+    for (int u = 0; u < PF; ++u) fetch(u, u);
+    for (int st0 = 0; st0 < lmax; st0 += PF) {
+#pragma unroll
+      for (int u = 0; u < PF; ++u) {
+        const int st = st0 + u;
+        const double* da;
+        if constexpr (LDS_DPA) da = sDa + cb[u] * CNP;
+        else da = dp + (size_t)cb[u] * CNP;
+        const bool live = st < len;
+#pragma unroll
+        for (int k = 0; k < CNP; ++k) {
+          const R t = s + wb[u][k] * (R)da[k];
+          s = live ? t : s;
+        }
+        fetch(u, st + PF);
       }
-      jc = jn;
-      jn = jnn;
     }
   }
   const double e = ok ? gb[3 * (size_t)i + cc] - (double)s : 0.0;
