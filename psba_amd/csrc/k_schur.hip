@@ -91,17 +91,44 @@ __global__ __launch_bounds__(256) void k_schur_chunks(const int2* __restrict__ i
       const R* wa = W + (size_t)sIdx[it] * REC;
       const R* wb = W + (size_t)sIdx[kPairChunk + it] * REC;
       const double* vi = Vinv + (size_t)sIdx[2 * kPairChunk + it] * 6;
-      const R v00 = (R)vi[0], v01 = (R)vi[1], v02 = (R)vi[2], v11 = (R)vi[3], v12 = (R)vi[4], v22 = (R)vi[5];
+      // every load first and unconditional (a lane's rows and elements past
+      // the record are clamped to the last one and not stored), so that one
+      // wait covers them all: per-lane loop bounds made each remainder a
+      // round of its own
+      constexpr int NR = (CNP + 3) / 4, NW = (REC + 3) / 4;
+      double v[6];
+      R wv[NR][3], bv[NW];
 #pragma unroll
-      for (int r = q; r < CNP; r += 4) {
-        const R w0 = wa[3 * r], w1 = wa[3 * r + 1], w2 = wa[3 * r + 2];
-        R* y = sY + it * REC + 3 * r;
-        y[0] = w0 * v00 + w1 * v01 + w2 * v02;
-        y[1] = w0 * v01 + w1 * v11 + w2 * v12;
-        y[2] = w0 * v02 + w1 * v12 + w2 * v22;
+      for (int k = 0; k < 6; ++k) v[k] = vi[k];
+#pragma unroll
+      for (int j = 0; j < NR; ++j) {
+        const int r = min(q + 4 * j, CNP - 1);
+#pragma unroll
+        for (int c = 0; c < 3; ++c) wv[j][c] = wa[3 * r + c];
       }
 #pragma unroll
-      for (int el = q; el < REC; el += 4) sW[it * REC + el] = wb[el];
+      for (int j = 0; j < NW; ++j) bv[j] = wb[min(q + 4 * j, REC - 1)];
+      // (the empty asm keeps the compiler from sinking the loads of a row
+      // used under a condition into that condition's block)
+#pragma unroll
+      for (int j = 0; j < NR; ++j)
+#pragma unroll
+        for (int c = 0; c < 3; ++c) __asm__ volatile("" : "+v"(wv[j][c]));
+      const R v00 = (R)v[0], v01 = (R)v[1], v02 = (R)v[2], v11 = (R)v[3], v12 = (R)v[4], v22 = (R)v[5];
+#pragma unroll
+      for (int j = 0; j < NR; ++j) {
+        const int r = q + 4 * j;
+        if (r < CNP) {
+          const R w0 = wv[j][0], w1 = wv[j][1], w2 = wv[j][2];
+          R* y = sY + it * REC + 3 * r;
+          y[0] = w0 * v00 + w1 * v01 + w2 * v02;
+          y[1] = w0 * v01 + w1 * v11 + w2 * v12;
+          y[2] = w0 * v02 + w1 * v12 + w2 * v22;
+        }
+      }
+#pragma unroll
+      for (int j = 0; j < NW; ++j)
+        if (q + 4 * j < REC) sW[it * REC + q + 4 * j] = bv[j];
     }
   }
   __syncthreads();
