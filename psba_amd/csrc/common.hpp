@@ -203,6 +203,26 @@ __device__ __forceinline__ void loss_eval(int kind, double d, double s, double& 
   }
 }
 
+// acc = 0 + load(q0) + load(q0 + 1) + .. + load(q1 - 1), added in that order,
+// with the loads issued B at a time and unconditionally (indices past the end
+// clamped to the last one, their additions skipped), so that each batch of B
+// costs one wait rather than one per unrolled group of a runtime-bounded loop.
+template <int B, typename F>
+__device__ inline double ordered_sum(int q0, int q1, F load) {
+  double acc = 0.0;
+  for (int qb = q0; qb < q1; qb += B) {
+    double v[B];
+#pragma unroll
+    for (int u = 0; u < B; ++u) v[u] = load(min(qb + u, q1 - 1));
+#pragma unroll
+    for (int u = 0; u < B; ++u) {
+      const double t = acc + v[u];
+      acc = (qb + u < q1) ? t : acc;
+    }
+  }
+  return acc;
+}
+
 // A problem's gate (batch: per problem through ownr; one problem: ownr may be
 // null, the gate is run[0]).
 __device__ inline bool gated_off(const unsigned char* run, const int* ownr, int i) {

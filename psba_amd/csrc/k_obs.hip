@@ -290,9 +290,7 @@ __global__ __launch_bounds__(256) void k_cam_finish(const int* __restrict__ ccp,
   const int j = blockIdx.x;
   if (run && !run[ownr[j]]) return;
   for (int t = threadIdx.x; t < NE; t += blockDim.x) {
-    double acc = 0.0;
-#pragma unroll 8
-    for (int q = ccp[j]; q < ccp[j + 1]; ++q) acc += (double)part[(size_t)q * NE + t];
+    const double acc = ordered_sum<16>(ccp[j], ccp[j + 1], [&](int q) { return (double)part[(size_t)q * NE + t]; });
     if (t < NU) {
       int r = 0, rem = t;
       while (rem >= CNP - r) {

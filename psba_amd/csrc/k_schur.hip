@@ -205,9 +205,7 @@ __global__ __launch_bounds__(256) void k_schur_finish(const int* __restrict__ lb
   for (int t = threadIdx.x; t < C2; t += blockDim.x) {
     const int r = t / CNP, cc = t - r * CNP;
     double acc = 0.0;
-    if (q >= 0)
-#pragma unroll 8
-      for (int h = psp[q]; h < psp[q + 1]; ++h) acc += (double)part[(size_t)h * C2 + t];
+    if (q >= 0) acc = ordered_sum<16>(psp[q], psp[q + 1], [&](int h) { return (double)part[(size_t)h * C2 + t]; });
     double val;
     if (k == l) {
       double u = U[(size_t)k * C2 + t];
@@ -280,9 +278,7 @@ __global__ __launch_bounds__(64) void k_ea_finish(const int* __restrict__ ccp, c
   if (t >= nc * CNP) return;
   int j = t / CNP, r = t - j * CNP;
   if (run && !run[ownr[j]]) return;
-  double acc = 0.0;
-#pragma unroll 8
-  for (int q = ccp[j]; q < ccp[j + 1]; ++q) acc += (double)part[(size_t)q * CNP + r];
+  const double acc = ordered_sum<16>(ccp[j], ccp[j + 1], [&](int q) { return (double)part[(size_t)q * CNP + r]; });
   ea[t] = ga[t] - acc;
 }
 
@@ -386,9 +382,7 @@ __global__ __launch_bounds__(256) void k_b_schur_finish(const int* __restrict__ 
   for (int t = threadIdx.x; t < C2; t += blockDim.x) {
     const int r = t / CNP, cc = t - r * CNP;
     double acc = 0.0;
-    if (q >= 0)
-#pragma unroll 8
-      for (int h = psp[q]; h < psp[q + 1]; ++h) acc += (double)part[(size_t)h * C2 + t];
+    if (q >= 0) acc = ordered_sum<16>(psp[q], psp[q + 1], [&](int h) { return (double)part[(size_t)h * C2 + t]; });
     double val;
     if (k == l) {
       double u = U[(size_t)kg * C2 + t];
