@@ -301,34 +301,31 @@ __device__ __forceinline__ double quad_perm(double v) {
 __device__ __forceinline__ double quad_xor1(double v) { return quad_perm<0xB1>(v); }  // [1,0,3,2]
 __device__ __forceinline__ double quad_xor2(double v) { return quad_perm<0x4E>(v); }  // [2,3,0,1]
 
-// Grid barrier in two halves: between grid_arrive and grid_wait a workgroup
-// may read data that was published before the previous barrier (prefetch).
-__device__ __forceinline__ void grid_arrive(unsigned long long* bar) {
+// Grid barrier over a counter that only grows: every workgroup adds one and
+// waits until the counter reaches `target`.  The add returns the count, so the
+// workgroup that arrives last (workgroup 0, which carries the diagonal chain)
+// sees at once that the others are in and does not poll; the others poll
+// with a bounded wait (~1 s, then fail bit 2; never reached when every
+// workgroup of the launch is resident).
+__device__ __forceinline__ void grid_barrier(unsigned long long* bar, unsigned long long target, int* fail) {
   __syncthreads();
   if (threadIdx.x == 0) {
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
-    __hip_atomic_fetch_add(bar, 1ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-  }
-}
-
-__device__ __forceinline__ void grid_wait(unsigned long long* bar, unsigned long long target, int* fail) {
-  if (threadIdx.x == 0) {
-    unsigned spins = 0;
-    while (__hip_atomic_load(bar, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target) {
-      __builtin_amdgcn_s_sleep(2);
-      if (++spins > (1u << 24)) {  // ~1 s: never reached when every workgroup is resident
-        __hip_atomic_fetch_or(fail, 2, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        break;
+    const unsigned long long arrived =
+        __hip_atomic_fetch_add(bar, 1ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) + 1;
+    if (arrived < target) {
+      unsigned spins = 0;
+      while (__hip_atomic_load(bar, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target) {
+        __builtin_amdgcn_s_sleep(2);
+        if (++spins > (1u << 24)) {
+          __hip_atomic_fetch_or(fail, 2, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+          break;
+        }
       }
     }
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
   }
   __syncthreads();
-}
-
-__device__ __forceinline__ void grid_barrier(unsigned long long* bar, unsigned long long target, int* fail) {
-  grid_arrive(bar);
-  grid_wait(bar, target, fail);
 }
 
 // Point-to-point flags (back substitution).  A flag only grows, across
@@ -341,7 +338,7 @@ __device__ __forceinline__ void flag_publish(unsigned long long* f, unsigned lon
   __hip_atomic_store(f, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
-// Wait until *f >= v (thread 0 polls, bounded like grid_wait), then the whole
+// Wait until *f >= v (thread 0 polls, bounded like the grid barrier), then the whole
 // workgroup may read what was published before.
 __device__ __forceinline__ void flag_wait(unsigned long long* f, unsigned long long v, int* fail) {
   if (threadIdx.x == 0) {
