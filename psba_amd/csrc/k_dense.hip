@@ -339,15 +339,20 @@ __device__ __forceinline__ void flag_publish(unsigned long long* f, unsigned lon
 }
 
 // Wait until *f >= v (thread 0 polls, bounded like the grid barrier), then the whole
-// workgroup may read what was published before.
-__device__ __forceinline__ void flag_wait(unsigned long long* f, unsigned long long v, int* fail) {
+// workgroup may read what was published before.  `seen` is a value of *f that
+// thread 0 read earlier (0 if none): when it already reaches v, thread 0 does
+// not read the flag again and only takes the acquire fence.
+__device__ __forceinline__ void flag_wait(unsigned long long* f, unsigned long long v, int* fail,
+                                          unsigned long long seen = 0) {
   if (threadIdx.x == 0) {
-    unsigned spins = 0;
-    while (__hip_atomic_load(f, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < v) {
-      __builtin_amdgcn_s_sleep(2);
-      if (++spins > (1u << 24)) {
-        __hip_atomic_fetch_or(fail, 2, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        break;
+    if (seen < v) {
+      unsigned spins = 0;
+      while (__hip_atomic_load(f, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < v) {
+        __builtin_amdgcn_s_sleep(2);
+        if (++spins > (1u << 24)) {
+          __hip_atomic_fetch_or(fail, 2, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+          break;
+        }
       }
     }
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
@@ -1097,18 +1102,22 @@ __global__ __launch_bounds__(256) void k_chol_solve(double* __restrict__ A, int 
     }
     __syncthreads();
     for (int k = T - 1; k >= 0; --k) {
-      // r_{k-1} is complete two steps ahead: load it now, use it next step
-      double r_pre = 0.0;
-      if (k >= 1) {
-        if (k - 1 + 3 <= T - 1) flag_wait(rflag + (k - 1), epoch + 1, fail);
-        r_pre = yv[(size_t)(k - 1) * NB + (tid & 63)];
-      }
+      // r_{k-1} (complete two steps ahead when the helpers have rows in it):
+      // its flag is read now and checked after this step's solve, so that
+      // the round trip overlaps the products and the solve
+      const bool need_flag = k >= 1 && k - 1 + 3 <= T - 1;
+      unsigned long long fseen = 0;
+      if (need_flag && tid == 0)
+        fseen = __hip_atomic_load(rflag + (k - 1), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
       // sm.w = x_{k+1}, sm.rd = x_{k+2}; sm.X = L_{k+1,k}, sm.Y = L_{k+2,k}
       double br = r_next;
       if (k + 2 < T) br -= tile_tmatvec(sm, sm.Y, TP, sm.rd);
       if (k + 1 < T) br -= tile_tmatvec(sm, sm.X, TP, sm.w);
       __syncthreads();
       tile_backward(LDS(sm.P), br, xs + (size_t)k * NB, LDS(sm.v), rows_of(k));
+      // x_k was stored by wave 0 (tile_backward): wave 0 publishes it at
+      // once, while waves 1-3 load
+      if (k >= 1 && tid == 0) flag_publish(xflag, epoch + (unsigned long long)(T - k));
       // Waves 1-3 return from tile_backward at once and load the next
       // step's tiles while wave 0 solves: L_{k,k-1} and L_{k+1,k-1} straight
       // into sm.X and sm.Y (the products above have read them), L_{k-1,k-1}
@@ -1131,10 +1140,9 @@ __global__ __launch_bounds__(256) void k_chol_solve(double* __restrict__ A, int 
         if (q < n) dp[q] = sm.v[tid];
       }
       if (k == 0) break;
-      // x_k was stored by wave 0 (tile_backward): publish it
-      if (tid == 0) flag_publish(xflag, epoch + (unsigned long long)(T - k));
+      if (need_flag) flag_wait(rflag + (k - 1), epoch + 1, fail, fseen);
+      r_next = yv[(size_t)(k - 1) * NB + (tid & 63)];
       if (loader) regs_to_lds_w123(sm.P, p0, true);
-      r_next = r_pre;
       __syncthreads();
     }
   } else {
