@@ -446,7 +446,8 @@ constexpr int kBacksubLdsDpa = 4096;  // camera parameters that go through LDS (
 
 // With LDS_DPA the camera step dp[0..n) is first copied into LDS (dynamic
 // shared memory of n doubles), so that a long track waits for the W rows only.
-template <int CNP, typename R, bool LDS_DPA>
+// SORTED: `order` holds the points by descending track length.
+template <int CNP, typename R, bool LDS_DPA, bool SORTED>
 __global__ __launch_bounds__(256) void k_backsub(const int* __restrict__ pt_ptr, const int* __restrict__ obs_cam,
                                                  const R* __restrict__ W, const double* __restrict__ Vinv,
                                                  const double* __restrict__ gb, int np, int n,
@@ -466,15 +467,38 @@ __global__ __launch_bounds__(256) void k_backsub(const int* __restrict__ pt_ptr,
   // tracks of similar length
   const int slot = (blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6)) * kBacksubPtsPerWave + grp;
   const bool in = grp < kBacksubPtsPerWave && slot < np;
-  const int i = (in && order) ? order[slot] : slot;
+  const int i = (in && SORTED) ? order[slot] : slot;
   const bool ok = in && (!run || run[ownr[i]]);
   R s = 0;
-  {
-    // Uniform control flow: every lane of the wave steps through the wave's
-    // longest track; a lane past its own track (or without a point) loads a
-    // valid row and keeps its sum.  The W values and camera index of the next
-    // PF steps are in registers before they are used (unconditional loads, so
-    // the waits count exactly), and the running sum keeps its order.
+  if constexpr (!SORTED) {
+    // points in id order (tracks of any length side by side): each lane
+    // walks its own track; camera indices are fetched two observations
+    // ahead, so each step waits for one round of loads (W and dpa)
+    if (ok) {
+      const int beg = pt_ptr[i], end = pt_ptr[i + 1];
+      int jc = beg < end ? obs_cam[beg] : 0;
+      int jn = beg + 1 < end ? obs_cam[beg + 1] : 0;
+#pragma unroll 2
+      for (int o = beg; o < end; ++o) {
+        const int jnn = o + 2 < end ? obs_cam[o + 2] : 0;
+        const R* w = W + (size_t)o * 3 * CNP + cc;
+        const double* da;
+        if constexpr (LDS_DPA) da = sDa + jc * CNP;
+        else da = dp + (size_t)jc * CNP;
+#pragma unroll
+        for (int k = 0; k < CNP; ++k) s += w[3 * k] * (R)da[k];
+        jc = jn;
+        jn = jnn;
+      }
+    }
+  } else {
+    // points by track length (the lanes of a wave have tracks of similar
+    // length): uniform control flow, every lane of the wave steps through the
+    // wave's longest track, a lane past its own track (or without a point)
+    // loading a valid row and keeping its sum.  The W values and camera index
+    // of the next PF steps are in registers before they are used
+    // (unconditional loads, so the waits count exactly), and the running sum
+    // keeps its order.
     constexpr int PF = 2;
     const int beg = ok ? pt_ptr[i] : 0, len = ok ? pt_ptr[i + 1] - beg : 0;
     int lmax = len;
@@ -524,13 +548,11 @@ static void backsub_launch(hipStream_t st, const int* pt_ptr, const int* obs_cam
                            const double* gb, int np, int n, double* dp, double* eb, const int* pt_ownr,
                            const unsigned char* run, int block, const int* order = nullptr) {
   const int per_block = (block / 64) * kBacksubPtsPerWave;
-  if (n <= kBacksubLdsDpa)
-    hipLaunchKernelGGL((k_backsub<CNP, R, true>), dim3(ceil_div(np, per_block)), dim3(block),
-                       sizeof(double) * (size_t)n, st, pt_ptr, obs_cam, W, Vinv, gb, np, n, dp, eb, pt_ownr, run,
-                       order);
-  else
-    hipLaunchKernelGGL((k_backsub<CNP, R, false>), dim3(ceil_div(np, per_block)), dim3(block), 0, st, pt_ptr,
-                       obs_cam, W, Vinv, gb, np, n, dp, eb, pt_ownr, run, order);
+  const size_t lds = n <= kBacksubLdsDpa ? sizeof(double) * (size_t)n : 0;
+  auto kern = n <= kBacksubLdsDpa ? (order ? k_backsub<CNP, R, true, true> : k_backsub<CNP, R, true, false>)
+                                  : (order ? k_backsub<CNP, R, false, true> : k_backsub<CNP, R, false, false>);
+  hipLaunchKernelGGL(kern, dim3(ceil_div(np, per_block)), dim3(block), lds, st, pt_ptr, obs_cam, W, Vinv, gb, np, n,
+                     dp, eb, pt_ownr, run, order);
 }
 
 template <int CNP>
