@@ -928,9 +928,15 @@ __global__ __launch_bounds__(256) void k_chol_solve(double* __restrict__ A, int 
                                                     const double* __restrict__ ea, double* __restrict__ bw,
                                                     double* __restrict__ yv, double* __restrict__ xs,
                                                     double* __restrict__ dp, int n, double* __restrict__ ok,
-                                                    unsigned long long* bar, int* __restrict__ fail) {
+                                                    unsigned long long* bar, int* __restrict__ fail,
+                                                    const unsigned char* __restrict__ gate) {
   __shared__ CholSmem sm;
   const int G = gridDim.x;
+  // with gate (device-side LM control): a trial queued behind the one that
+  // stopped the solve ends before anything is factored or any workgroup
+  // arrives at a barrier (the counters, epoch and fail word untouched); the
+  // gate is read now and tested once workgroup 0's first tile is loaded
+  const bool run = !gate || *gate;
   // read before this workgroup's first arrival, so before workgroup 0 (which
   // passes every barrier last) advances it at the end
   const unsigned long long bar_base =
@@ -956,12 +962,14 @@ __global__ __launch_bounds__(256) void k_chol_solve(double* __restrict__ A, int 
     }
     if (tid < NB) sm.part[0][tid] = ea[tid];
     __syncthreads();
+    if (!run) return;
     const int bad = tile_potrf(*(lds_smem*)&sm, LDS(sm.P), true, nkb_of(0));
     if (bad && tid == 0) __hip_atomic_fetch_or(fail, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     store_tile_w123(sm.P, Lf, ld);
     tile_forward(LDS(sm.P), LDS(sm.part[0]), yv, LDS(sm.w), rows_of(0));
     carry = true;
   }
+  if (!run) return;
   for (int q = NB + (int)blockIdx.x * (int)blockDim.x + tid; q < T * NB; q += G * (int)blockDim.x) bw[q] = ea[q];
   grid_barrier(bar, bar_base + (unsigned long long)G * (++nbar), fail);
 
@@ -1306,7 +1314,7 @@ int launch_factor_solve(psba_ctx* c) {
   int ld = c->n_pad, n = c->n;
   probe_begin(c, PSBA_PROBE_FACTOR);
   hipLaunchKernelGGL(k_chol_solve, dim3(G), dim3(256), 0, c->stream, c->S, ld, T, c->Lf, c->ea, c->bw, c->yv,
-                     c->xsol, c->dp, n, ok, c->bar, c->chol_fail);
+                     c->xsol, c->dp, n, ok, c->bar, c->chol_fail, gate_of(c, kGateActive));
   probe_end(c, PSBA_PROBE_FACTOR);
   c->L_cur = c->Lf;
   PSBA_LAUNCH_TRY();
