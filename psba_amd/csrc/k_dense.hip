@@ -503,6 +503,97 @@ PSBA_OUTLINE void trsm_rows(lds_dbl* out, const lds_dbl* L, const lds_dbl* rd) {
   }
 }
 
+// The diagonal task's pair of steps in one routine: out = A_ik L^-T as
+// trsm_rows, and D = A_ii - out out^T (A_ii from global, ld; only the lower
+// quadrants updated, as acc_update with lower_only), with the product's
+// k-steps for a 16-column block of out taken as soon as every wave has
+// finished that block (one workgroup barrier per block), so that they run
+// beside the next block's column chain.  Per accumulator the MFMA sequence is
+// acc_update's, in the same order, so D is the same bits.
+PSBA_OUTLINE void trsm_syrk(lds_dbl* out, const lds_dbl* L, const lds_dbl* rd, const double* A, int ld,
+                            lds_dbl* D) {
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int r = 16 * w + (lane >> 2), g = lane & 3;
+  const int li = lane & 15, lk = lane >> 4;
+  const int R0 = 32 * (w >> 1), C0 = 32 * (w & 1);
+  const bool upd = C0 <= R0;
+  dvec4 acc[2][2];
+#pragma unroll
+  for (int a = 0; a < 2; ++a)
+#pragma unroll
+    for (int bq = 0; bq < 2; ++bq)
+#pragma unroll
+      for (int rr = 0; rr < 4; ++rr)
+        acc[a][bq][rr] = A[(size_t)(R0 + 16 * a + 4 * rr + lk) * ld + C0 + 16 * bq + li];
+  double x[16];
+#pragma unroll
+  for (int q = 0; q < 16; ++q) x[q] = out[r * TP + 4 * q + g];
+#pragma unroll
+  for (int b = 0; b < 4; ++b) {
+    double rdv[4], Lb[4][16];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      rdv[q] = rd[16 * b + 4 * q + g];
+#pragma unroll
+      for (int cb = 0; cb < 16; ++cb) Lb[q][cb] = (4 * q + g > cb) ? L[(16 * b + 4 * q + g) * TP + 16 * b + cb] : 0.0;
+    }
+#pragma unroll
+    for (int cb = 0; cb < 16; ++cb) {
+      const int q0 = cb >> 2, g0 = cb & 3;
+      const double xo = x[4 * b + q0] * rdv[q0];
+      double xc;
+      if (g0 == 0) xc = quad_bcast<0>(xo);
+      else if (g0 == 1) xc = quad_bcast<1>(xo);
+      else if (g0 == 2) xc = quad_bcast<2>(xo);
+      else xc = quad_bcast<3>(xo);
+      const double fm = fma(-xc, Lb[q0][cb], x[4 * b + q0]);
+      x[4 * b + q0] = (g == g0) ? xc : ((g > g0) ? fm : x[4 * b + q0]);
+#pragma unroll
+      for (int q = q0 + 1; q < 4; ++q) x[4 * b + q] = fma(-xc, Lb[q][cb], x[4 * b + q]);
+#pragma unroll
+      for (int q = q0; q < 4; ++q) __asm__ volatile("" : "+v"(x[4 * b + q]));
+    }
+#pragma unroll
+    for (int q = 0; q < 4; ++q) out[r * TP + 16 * b + 4 * q + g] = x[4 * b + q];
+    __syncthreads();  // block b of out complete for every row
+    if (upd) {
+#pragma unroll
+      for (int k0 = 16 * b; k0 < 16 * b + 16; k0 += 4) {
+        const double a0 = -out[(R0 + li) * TP + k0 + lk];
+        const double a1 = -out[(R0 + 16 + li) * TP + k0 + lk];
+        const double b0 = out[(C0 + li) * TP + k0 + lk];
+        const double b1 = out[(C0 + 16 + li) * TP + k0 + lk];
+        acc[0][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b0, acc[0][0], 0, 0, 0);
+        acc[0][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b1, acc[0][1], 0, 0, 0);
+        acc[1][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b0, acc[1][0], 0, 0, 0);
+        acc[1][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b1, acc[1][1], 0, 0, 0);
+      }
+    }
+    if (b < 3) {
+#pragma unroll
+      for (int bp = b + 1; bp < 4; ++bp) {
+        dvec4 t = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+        for (int k0 = 0; k0 < 16; k0 += 4) {
+          const double av = out[(16 * w + li) * TP + 16 * b + k0 + lk];
+          const double bv = L[(16 * bp + li) * TP + 16 * b + k0 + lk];
+          t = __builtin_amdgcn_mfma_f64_16x16x4f64(av, bv, t, 0, 0, 0);
+        }
+#pragma unroll
+        for (int k = 0; k < 4; ++k) out[(16 * w + 4 * k + lk) * TP + 16 * bp + li] = t[k];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) x[4 * bp + q] -= out[r * TP + 16 * bp + 4 * q + g];
+      }
+    }
+  }
+#pragma unroll
+  for (int a = 0; a < 2; ++a)
+#pragma unroll
+    for (int bq = 0; bq < 2; ++bq)
+#pragma unroll
+      for (int rr = 0; rr < 4; ++rr) D[(R0 + 16 * a + 4 * rr + lk) * TP + C0 + 16 * bq + li] = acc[a][bq][rr];
+}
+
 // value of v on lane N of each 16-lane row (DPP row_newbcast; CDNA's 64-bit
 // DPP takes this control, so the move is one v_mov_b64_dpp, not two halves)
 template <int N>
@@ -998,12 +1089,25 @@ __global__ __launch_bounds__(256) void k_chol_solve(double* __restrict__ A, int 
       // b_i is read now so that its load overlaps the tile loads
       const int br_row = tid >> 2, br_p = tid & 3;
       const double b_old = (!fwd && j == k + 1 && br_p == 0) ? bw[(size_t)i * NB + br_row] : 0.0;
+      const bool diag_next = i == j && j == k + 1;
+      // the next diagonal tile: factored by this workgroup; as its last task
+      // of the phase, into sm.P for the next phase
+      if (diag_next) carry = (t + G >= ntasks);
+      double* Dt = carry ? sm.P : sm.Y;
       dvec4 acc[2][2];
-      task_load(acc, sm, tile(A, i, j), tile(A, i, k), (i != j) ? tile(A, j, k) : nullptr, ld);
-      trsm_rows(LDS(sm.X), LDS(sm.P), LDS(sm.rd));
-      if (i != j) trsm_rows(LDS(sm.Y), LDS(sm.P), LDS(sm.rd));
-      __syncthreads();
-      acc_update(acc, sm.X, (i != j) ? sm.Y : sm.X, i == j);
+      if (diag_next) {
+        // L_{k+1,k} and the update of A_{k+1,k+1} by it, in one routine
+        load_full(sm.X, tile(A, i, k), ld);
+        __syncthreads();
+        trsm_syrk(LDS(sm.X), LDS(sm.P), LDS(sm.rd), tile(A, i, j), ld, LDS(Dt));
+        __syncthreads();
+      } else {
+        task_load(acc, sm, tile(A, i, j), tile(A, i, k), (i != j) ? tile(A, j, k) : nullptr, ld);
+        trsm_rows(LDS(sm.X), LDS(sm.P), LDS(sm.rd));
+        if (i != j) trsm_rows(LDS(sm.Y), LDS(sm.P), LDS(sm.rd));
+        __syncthreads();
+        acc_update(acc, sm.X, (i != j) ? sm.Y : sm.X, i == j);
+      }
       if (j == k + 1) {
         // store L_ik, and from the task (k+2, k+1) also L_{k+1,k} (its Y,
         // the same bits as the diagonal task's X: the same tile, L_kk and
@@ -1024,14 +1128,9 @@ __global__ __launch_bounds__(256) void k_chol_solve(double* __restrict__ A, int 
           }
         }
       }
-      if (i == j && j == k + 1) {
-        // the next diagonal tile: factor it (and form y_{k+1} without a
-        // forward workgroup); as the workgroup's last task of the phase,
-        // into sm.P for the next phase
-        carry = (t + G >= ntasks);
-        double* Dt = carry ? sm.P : sm.Y;
-        __syncthreads();
-        acc_store(acc, Dt, TP);
+      if (diag_next) {
+        // factor the next diagonal tile (updated into Dt above; and form
+        // y_{k+1} without a forward workgroup)
         __syncthreads();
         const int bad = tile_potrf(*(lds_smem*)&sm, LDS(Dt), carry, nkb_of(i));
         if (bad && tid == 0) __hip_atomic_fetch_or(fail, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
