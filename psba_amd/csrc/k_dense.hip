@@ -1047,10 +1047,8 @@ __global__ __launch_bounds__(256) void k_chol_solve(double* __restrict__ A, int 
 
   // ---- phase -1: factor tile (0,0) and y_0; copy the right-hand side
   if (blockIdx.x == 0) {
-    for (int f = tid; f < NB * NB; f += blockDim.x) {
-      const int r = f >> 6, c = f & 63;
-      sm.P[r * TP + c] = A[(size_t)r * ld + c];
-    }
+    // (all sixteen loads of a thread in flight together)
+    load_full(sm.P, A, ld);
     if (tid < NB) sm.part[0][tid] = ea[tid];
     __syncthreads();
     if (!run) return;
@@ -1454,10 +1452,7 @@ __global__ __launch_bounds__(256) void k_b_chol_solve(double* __restrict__ S, do
   auto tile = [&](double* base, int i, int j) { return base + (size_t)i * NB * ld + (size_t)j * NB; };
   int bad = 0;
   for (int q = tid; q < T * NB; q += blockDim.x) bwp[q] = (q < n) ? b[q] : 0.0;
-  for (int f = tid; f < NB * NB; f += blockDim.x) {
-    const int r = f >> 6, c = f & 63;
-    sm.Y[r * TP + c] = A[(size_t)r * ld + c];
-  }
+  load_full(sm.Y, A, ld);  // (all sixteen loads of a thread in flight together)
   if (tid < NB) sm.part[0][tid] = (tid < n) ? b[tid] : 0.0;
   __syncthreads();
   bad |= tile_potrf(*(lds_smem*)&sm, LDS(sm.Y), false, 4);
@@ -1580,10 +1575,7 @@ __global__ __launch_bounds__(256) void k_b_chol_init(double* __restrict__ S, dou
   const double* b = ea + (size_t)cam_off[p] * cnp;
   double* bwp = bw + vec_off[p];
   for (int q = tid; q < T * NB; q += blockDim.x) bwp[q] = (q < n) ? b[q] : 0.0;
-  for (int f = tid; f < NB * NB; f += blockDim.x) {
-    const int r = f >> 6, c = f & 63;
-    sm.Y[r * TP + c] = A[(size_t)r * ld + c];
-  }
+  load_full(sm.Y, A, ld);  // (all sixteen loads of a thread in flight together)
   if (tid < NB) sm.part[0][tid] = (tid < n) ? b[tid] : 0.0;
   __syncthreads();
   const int bad = tile_potrf(*(lds_smem*)&sm, LDS(sm.Y), false, 4);
