@@ -517,6 +517,10 @@ PSBA_OUTLINE void trsm_syrk(lds_dbl* out, const lds_dbl* L, const lds_dbl* rd, c
   const int li = lane & 15, lk = lane >> 4;
   const int R0 = 32 * (w >> 1), C0 = 32 * (w & 1);
   const bool upd = C0 <= R0;
+  // A read as global memory: through a generic pointer these would be flat
+  // loads, which count against the LDS wait counter too, so the first LDS
+  // wait of the column chain would wait for them
+  const __attribute__((address_space(1))) double* Ag = (const __attribute__((address_space(1))) double*)A;
   dvec4 acc[2][2];
 #pragma unroll
   for (int a = 0; a < 2; ++a)
@@ -524,7 +528,7 @@ PSBA_OUTLINE void trsm_syrk(lds_dbl* out, const lds_dbl* L, const lds_dbl* rd, c
     for (int bq = 0; bq < 2; ++bq)
 #pragma unroll
       for (int rr = 0; rr < 4; ++rr)
-        acc[a][bq][rr] = A[(size_t)(R0 + 16 * a + 4 * rr + lk) * ld + C0 + 16 * bq + li];
+        acc[a][bq][rr] = Ag[(size_t)(R0 + 16 * a + 4 * rr + lk) * ld + C0 + 16 * bq + li];
   double x[16];
 #pragma unroll
   for (int q = 0; q < 16; ++q) x[q] = out[r * TP + 4 * q + g];
