@@ -83,25 +83,48 @@ __device__ inline void block_sum(double (&v)[K], double* sh) {
 // an L2 write-back, which made the per-block ticket scheme cost tens of
 // microseconds per launch.
 template <int K>
-__device__ inline void grid_sum_partials(double (&v)[K], double* partials, double* sh) {
+__device__ inline void grid_sum_partials(double (&v)[K], double* partials, double* sh, long blk) {
   block_sum<K>(v, sh);
   if (threadIdx.x == 0) {
 #pragma unroll
-    for (int k = 0; k < K; ++k) partials[(size_t)blockIdx.x * K + k] = v[k];
+    for (int k = 0; k < K; ++k) partials[(size_t)blk * K + k] = v[k];
   }
+}
+template <int K>
+__device__ inline void grid_sum_partials(double (&v)[K], double* partials, double* sh) {
+  grid_sum_partials<K>(v, partials, sh, blockIdx.x);
 }
 
 // One block adds nb blocks' K partials in block order (thread t takes blocks
 // t, t + blockDim, .., then block_sum) and writes out[0..K) from thread 0.
+// Eight of a thread's blocks are loaded at a time (clamped to the last block,
+// the additions past the end skipped), so a batch waits for one load latency;
+// the additions keep the order b = t, t + blockDim, ...
 template <int K>
 __device__ inline void sum_partials_block(const double* __restrict__ partials, int nb, double* __restrict__ out,
                                           double* sh) {
+  constexpr int B = 8;
   double acc[K];
 #pragma unroll
   for (int k = 0; k < K; ++k) acc[k] = 0;
-  for (int b = threadIdx.x; b < nb; b += blockDim.x) {
+  const int step = blockDim.x;
+  for (int b0 = threadIdx.x; b0 < nb; b0 += B * step) {
+    double v[B][K];
 #pragma unroll
-    for (int k = 0; k < K; ++k) acc[k] += partials[(size_t)b * K + k];
+    for (int u = 0; u < B; ++u) {
+      const int b = min(b0 + u * step, nb - 1);
+#pragma unroll
+      for (int k = 0; k < K; ++k) v[u][k] = partials[(size_t)b * K + k];
+    }
+#pragma unroll
+    for (int u = 0; u < B; ++u) {
+      const bool live = b0 + u * step < nb;
+#pragma unroll
+      for (int k = 0; k < K; ++k) {
+        const double t = acc[k] + v[u][k];
+        acc[k] = live ? t : acc[k];
+      }
+    }
   }
   block_sum<K>(acc, sh);
   if (threadIdx.x == 0) {
@@ -124,18 +147,7 @@ __global__ __launch_bounds__(256) void k_sum_partials_gated(const double* __rest
                                                             const unsigned char* __restrict__ gate) {
   if (!*gate) return;
   __shared__ double sh[K * 4];
-  double acc[K];
-#pragma unroll
-  for (int k = 0; k < K; ++k) acc[k] = 0;
-  for (int b = threadIdx.x; b < nb; b += blockDim.x) {
-#pragma unroll
-    for (int k = 0; k < K; ++k) acc[k] += partials[(size_t)b * K + k];
-  }
-  block_sum<K>(acc, sh);
-  if (threadIdx.x == 0) {
-#pragma unroll
-    for (int k = 0; k < K; ++k) out[k] = acc[k];
-  }
+  sum_partials_block<K>(partials, nb, out, sh);
 }
 
 template <int K>

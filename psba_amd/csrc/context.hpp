@@ -174,6 +174,7 @@ int launch_maxdiag(psba_ctx* c);                          // -> d_scal[SC_MAXDIA
 int launch_vinv(psba_ctx* c, double mu);                  // Vinv, z
 int launch_backsub(psba_ctx* c);                          // eb, dp[n..)
 int launch_step_stats(psba_ctx* c, double mu);            // SC_DP2, SC_RHODEN
+int launch_trial_eval(psba_ctx* c);  // device control: step statistics + trial cost partials, one launch
 int launch_commit(psba_ctx* c);                           // p += dp, SC_P2
 // out[SC_D0..] = dots of (J x1).(J x1), (J x1).(J x2), (J x2).(J x2); x2 may be null
 int launch_jmul_dots(psba_ctx* c, const double* x1, const double* x2);
@@ -215,6 +216,9 @@ inline const int* owner_of(const psba_ctx* c) { return c->devctl ? c->zero_owner
 // in red_part after the step statistics' (two per 256 parameters), so that the
 // LM control kernel can sum both: red_part holds 8 per block of the larger.
 inline long trial_cost_part_off(const psba_ctx* c) { return 2L * ((c->ntot + 255) / 256); }
+// and the commit's |p|^2 partials (one per 256 parameters) after those, summed
+// by the next trial's LM control kernel
+inline long commit_part_off(const psba_ctx* c) { return trial_cost_part_off(c) + (c->no + 255) / 256; }
 inline const double* mu_of(const psba_ctx* c) { return c->devctl ? &c->lm->mu : nullptr; }
 
 // ---- launchers (k_tr.hip) ----

@@ -164,8 +164,7 @@ int lm_enqueue_trial(psba_ctx* c, int slot, bool first) {
   PSBA_TRY(launch_schur(c, 0.0));
   PSBA_TRY(launch_factor_solve(c));
   PSBA_TRY(launch_backsub(c));
-  PSBA_TRY(launch_step_stats(c, 0.0));
-  PSBA_TRY(launch_residual(c, true));
+  PSBA_TRY(launch_trial_eval(c));
   PSBA_TRY(launch_lm_control(c, c->h_lm + slot));
   PSBA_TRY(launch_commit(c));
   PSBA_HIP_TRY(hipEventRecord(c->ev_lm[slot], c->stream));

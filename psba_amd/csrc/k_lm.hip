@@ -107,15 +107,19 @@ __device__ void lm_decide(LmDev* s, unsigned char* gate, const double* __restric
                           psba_trace_rec* __restrict__ trace);
 
 // One block: first the trial's step statistics (|dp|^2 and the gain-ratio
-// denominator, from k_step_stats' partials) and cost (from k_residual's) are
-// summed into scal, as k_sum_partials<2> and <1> would, then thread 0 decides.
+// denominator, from k_trial_eval's step partials) and cost (its cost
+// partials) are summed into scal, as k_sum_partials<2> and <1> would, and,
+// after an accepted trial (p_pending), |p|^2 from the commit's partials; then
+// thread 0 decides.
 __global__ __launch_bounds__(256) void k_lm_control(LmDev* s, unsigned char* gate, double* __restrict__ scal,
                                                     psba_trace_rec* __restrict__ trace, LmDev* out,
                                                     const double* __restrict__ step_part, int nb_step,
-                                                    const double* __restrict__ cost_part, int nb_cost) {
+                                                    const double* __restrict__ cost_part, int nb_cost,
+                                                    const double* __restrict__ p2_part, int nb_p2) {
   __shared__ double sh[8];
   sum_partials_block<2>(step_part, nb_step, scal + SC_DP2, sh);
   sum_partials_block<1>(cost_part, nb_cost, scal + SC_COST, sh);
+  if (gate[kGateActive] && s->p_pending) sum_partials_block<1>(p2_part, nb_p2, scal + SC_P2, sh);
   if (threadIdx.x == 0) {
     lm_decide(s, gate, scal, trace);
     *out = *s;
@@ -218,7 +222,8 @@ int launch_lm_mu0(psba_ctx* c) {
 int launch_lm_control(psba_ctx* c, LmDev* out) {
   hipLaunchKernelGGL(k_lm_control, dim3(1), dim3(256), 0, c->stream, c->lm, c->gate, c->d_scal, c->lm_trace, out,
                      (const double*)c->red_part, (int)ceil_div(c->ntot, 256),
-                     (const double*)(c->red_part + trial_cost_part_off(c)), (int)ceil_div(c->no, 256));
+                     (const double*)(c->red_part + trial_cost_part_off(c)), (int)ceil_div(c->no, 256),
+                     (const double*)(c->red_part + commit_part_off(c)), (int)ceil_div(c->ntot, 256));
   PSBA_LAUNCH_TRY();
   return PSBA_OK;
 }

@@ -22,16 +22,17 @@ constexpr int kJacBlock = 128;  // threads per Jacobian workgroup (context launc
 // ---------------------------------------------------------------------------
 // One thread per observation.  With dp != nullptr the parameters are p + dp
 // (the trial point), computed with the same additions the commit performs.
+// The block's work as a routine of its block index `blk` (k_residual, and
+// k_trial_eval below, which runs it beside the step statistics).
 template <int CNP>
-__global__ __launch_bounds__(256) void k_residual(const double* __restrict__ K, const double* __restrict__ q0,
-                                                  const double* __restrict__ cams, const double* __restrict__ pts,
-                                                  const double* __restrict__ dp, int n,
-                                                  const double* __restrict__ meas, const int* __restrict__ obs_cam,
-                                                  const int* __restrict__ obs_pt, int no, double* __restrict__ e_out,
-                                                  double* partials, int loss, double loss_d) {
-  __shared__ double sh[4];
+__device__ __forceinline__ void residual_block(const double* __restrict__ K, const double* __restrict__ q0,
+                                               const double* __restrict__ cams, const double* __restrict__ pts,
+                                               const double* __restrict__ dp, int n, const double* __restrict__ meas,
+                                               const int* __restrict__ obs_cam, const int* __restrict__ obs_pt,
+                                               int no, double* __restrict__ e_out, double* partials, int loss,
+                                               double loss_d, int blk, double* sh) {
   double acc[1] = {0.0};
-  int o = blockIdx.x * blockDim.x + threadIdx.x;
+  int o = blk * blockDim.x + threadIdx.x;
   if (o < no) {
     int j = obs_cam[o], i = obs_pt[o];
     double cam[CNP], M[3], uv[2];
@@ -51,20 +52,82 @@ __global__ __launch_bounds__(256) void k_residual(const double* __restrict__ K, 
       loss_eval(loss, loss_d, acc[0], acc[0], drho);
     }
   }
-  grid_sum_partials<1>(acc, partials, sh);
+  grid_sum_partials<1>(acc, partials, sh, blk);
+}
+
+template <int CNP>
+__global__ __launch_bounds__(256) void k_residual(const double* __restrict__ K, const double* __restrict__ q0,
+                                                  const double* __restrict__ cams, const double* __restrict__ pts,
+                                                  const double* __restrict__ dp, int n,
+                                                  const double* __restrict__ meas, const int* __restrict__ obs_cam,
+                                                  const int* __restrict__ obs_pt, int no, double* __restrict__ e_out,
+                                                  double* partials, int loss, double loss_d) {
+  __shared__ double sh[4];
+  residual_block<CNP>(K, q0, cams, pts, dp, n, meas, obs_cam, obs_pt, no, e_out, partials, loss, loss_d,
+                      blockIdx.x, sh);
+}
+
+// |dp|^2 and dp.(mu dp + g) over 256 parameters per block (block `blk`)
+__device__ __forceinline__ void step_stats_block(const double* __restrict__ dp, const double* __restrict__ g,
+                                                 long ntot, double mu, const double* __restrict__ mu_p,
+                                                 double* partials, long blk, double* sh) {
+  double acc[2] = {0.0, 0.0};
+  if (mu_p) mu = *mu_p;
+  long q = blk * blockDim.x + threadIdx.x;
+  if (q < ntot) {
+    double d = dp[q];
+    acc[0] = d * d;
+    acc[1] = d * (mu * d + g[q]);
+  }
+  grid_sum_partials<2>(acc, partials, sh, blk);
+}
+
+// A device-controlled trial's step statistics and trial cost in one launch:
+// blocks [0, nb_step) are k_step_stats' blocks, the rest k_residual's (the
+// same per-block arithmetic, so the same partials), each set of partials
+// where the LM control kernel sums it.
+template <int CNP>
+__global__ __launch_bounds__(256) void k_trial_eval(const double* __restrict__ g, long ntot,
+                                                    const double* __restrict__ mu_p, double* step_part, int nb_step,
+                                                    const double* __restrict__ K, const double* __restrict__ q0,
+                                                    const double* __restrict__ cams, const double* __restrict__ pts,
+                                                    const double* __restrict__ dp, int n,
+                                                    const double* __restrict__ meas, const int* __restrict__ obs_cam,
+                                                    const int* __restrict__ obs_pt, int no, double* cost_part,
+                                                    int loss, double loss_d) {
+  __shared__ double sh[8];
+  const int b = blockIdx.x;
+  if (b < nb_step) {
+    step_stats_block(dp, g, ntot, 0.0, mu_p, step_part, b, sh);
+  } else {
+    residual_block<CNP>(K, q0, cams, pts, dp, n, meas, obs_cam, obs_pt, no, nullptr, cost_part, loss, loss_d,
+                        b - nb_step, sh);
+  }
+}
+
+template <int CNP>
+static void run_trial_eval(psba_ctx* c) {
+  const unsigned nb_step = ceil_div(c->ntot, 256), nb_cost = ceil_div(c->no, 256);
+  hipLaunchKernelGGL(k_trial_eval<CNP>, dim3(nb_step + nb_cost), dim3(256), 0, c->stream, c->g, (long)c->ntot,
+                     mu_of(c), c->red_part, (int)nb_step, c->K, c->q0, c->cams, c->pts, c->dp, c->n, c->meas,
+                     c->obs_cam, c->obs_pt, c->no, c->red_part + trial_cost_part_off(c), c->loss, c->loss_scale);
+}
+
+int launch_trial_eval(psba_ctx* c) {
+  if (!c->devctl) return PSBA_ERR_ARG;
+  PSBA_DISPATCH_CNP(c->cnp, run_trial_eval, c);
+  PSBA_LAUNCH_TRY();
+  return PSBA_OK;
 }
 
 template <int CNP>
 static void run_residual(psba_ctx* c, bool trial) {
   unsigned nb = ceil_div(c->no, 256);
-  // a trial under device control leaves its partials beside the step
-  // statistics' for the LM control kernel, which sums both
-  const bool to_control = trial && c->devctl;
-  double* part = to_control ? c->red_part + trial_cost_part_off(c) : c->red_part;
+  // (a trial under device control takes k_trial_eval instead)
   hipLaunchKernelGGL(k_residual<CNP>, dim3(nb), dim3(256), 0, c->stream, c->K, c->q0, c->cams, c->pts,
                      trial ? c->dp : nullptr, c->n, c->meas, c->obs_cam, c->obs_pt, c->no,
-                     trial ? nullptr : c->e, part, c->loss, c->loss_scale);
-  if (!to_control) launch_sum_partials<1>(c->stream, c->red_part, (int)nb, c->d_scal + SC_COST);
+                     trial ? nullptr : c->e, c->red_part, c->loss, c->loss_scale);
+  launch_sum_partials<1>(c->stream, c->red_part, (int)nb, c->d_scal + SC_COST);
 }
 
 int launch_residual(psba_ctx* c, bool trial) {
@@ -580,23 +643,14 @@ __global__ __launch_bounds__(256) void k_step_stats(const double* __restrict__ d
                                                     long ntot, double mu, const double* __restrict__ mu_p,
                                                     double* partials) {
   __shared__ double sh[8];
-  double acc[2] = {0.0, 0.0};
-  if (mu_p) mu = *mu_p;
-  long q = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (q < ntot) {
-    double d = dp[q];
-    acc[0] = d * d;
-    acc[1] = d * (mu * d + g[q]);
-  }
-  grid_sum_partials<2>(acc, partials, sh);
+  step_stats_block(dp, g, ntot, mu, mu_p, partials, blockIdx.x, sh);
 }
 
 int launch_step_stats(psba_ctx* c, double mu) {
   const unsigned nb = ceil_div(c->ntot, 256);
   hipLaunchKernelGGL(k_step_stats, dim3(nb), dim3(256), 0, c->stream, c->dp, c->g, (long)c->ntot, mu, mu_of(c),
                      c->red_part);
-  // under device control the LM control kernel sums these partials
-  if (!c->devctl) launch_sum_partials<2>(c->stream, c->red_part, (int)nb, c->d_scal + SC_DP2);
+  launch_sum_partials<2>(c->stream, c->red_part, (int)nb, c->d_scal + SC_DP2);
   PSBA_LAUNCH_TRY();
   return PSBA_OK;
 }
@@ -626,15 +680,12 @@ __global__ __launch_bounds__(256) void k_commit(double* __restrict__ cams, doubl
 
 int launch_commit(psba_ctx* c) {
   const unsigned nb = ceil_div(c->ntot, 256);
-  hipLaunchKernelGGL(k_commit, dim3(nb), dim3(256), 0, c->stream, c->cams, c->pts, c->dp, c->n, (long)c->ntot,
-                     c->red_part, gate_of(c, kGateAccept));
-  if (c->devctl) {
-    // the partials of a gated-off commit are stale: sum them only after an accept
-    hipLaunchKernelGGL(k_sum_partials_gated<1>, dim3(1), dim3(256), 0, c->stream, c->red_part, (int)nb,
-                       c->d_scal + SC_P2, gate_of(c, kGateAccept));
-  } else {
-    launch_sum_partials<1>(c->stream, c->red_part, (int)nb, c->d_scal + SC_P2);
-  }
+  // under device control the partials stay where the next trial's LM control
+  // kernel sums them, when this commit ran (an accept leaves p_pending set)
+  double* part = c->devctl ? c->red_part + commit_part_off(c) : c->red_part;
+  hipLaunchKernelGGL(k_commit, dim3(nb), dim3(256), 0, c->stream, c->cams, c->pts, c->dp, c->n, (long)c->ntot, part,
+                     gate_of(c, kGateAccept));
+  if (!c->devctl) launch_sum_partials<1>(c->stream, c->red_part, (int)nb, c->d_scal + SC_P2);
   PSBA_LAUNCH_TRY();
   return PSBA_OK;
 }
