@@ -373,7 +373,8 @@ def stage_rates(P, ctx, solve_on, sc, info, r, solve_s):
     with their roofline fractions.  Jacobian bytes per observation: the reads
     (camera, point, measurement, indices: 40 B) and the writes (residual 2,
     J_A 2C, J_B 6 and W 3C doubles).  Schur flops: per (point, camera pair
-    k >= l of its track) one C x 3 by 3 x C product, 6 C^2 flops.  Back
+    k >= l of its track) one C x 3 by 3 x C product, 6 C^2 flops (the launch
+    also runs the e_a chunks, whose few flops are not counted).  Back
     substitution bytes: W rows (3C doubles) and camera indices per
     observation, V^-1, g_b and the outputs per point."""
     out = {"lm_iterations_per_s": r["itno"] / solve_s, "lm_trials_per_s": len(r["trace"]) / solve_s}
@@ -403,8 +404,8 @@ def stage_rates(P, ctx, solve_on, sc, info, r, solve_s):
     ms, n = timed["schur"]
     avg = ms * 1e-3 / max(n, 1)
     flops = 6.0 * C * C * info["n_pair_items"]
-    traffic, src = profiled_traffic("k_schur_chunks")
-    out["schur"] = {"kernel": "k_schur_chunks", "launches": n, "avg_launch_us": avg * 1e6, "flops_per_launch": flops,
+    traffic, src = profiled_traffic("k_schur_ea_chunks")
+    out["schur"] = {"kernel": "k_schur_ea_chunks", "launches": n, "avg_launch_us": avg * 1e6, "flops_per_launch": flops,
                     "achieved_TFLOPs": flops / avg / 1e12, "frac_of_fp64_peak": flops / avg / 1e12 / FP64_PEAK_TFLOPS,
                     "traffic": traffic, "traffic_source": src}
     ms, n = timed["backsub"]

@@ -31,28 +31,29 @@ namespace psba {
 // item records at rec[s * kPairChunk ..] (observations, point, chunk; -1 past
 // the chunk's items), an address its index gives, so the W rows and V^-1 are
 // its second round of loads.
+// The workgroup's work as a routine of its launch index `wg` and its LDS
+// (sBuf: 2 kPairChunk 3 CNP values, sIdx: 3 kPairChunk, sMeta: 2), so that
+// k_schur_ea_chunks can run it beside the e_a chunks.
 template <int CNP, typename R, bool SLOTS>
-__global__ __launch_bounds__(256) void k_schur_chunks(const int2* __restrict__ items, const int* __restrict__ sb,
-                                                      const int* __restrict__ se, const int* __restrict__ obs_pt,
-                                                      const R* __restrict__ W, const double* __restrict__ Vinv,
-                                                      R* __restrict__ part, const int* __restrict__ ownr,
-                                                      const unsigned char* __restrict__ run, int nchunk,
-                                                      const int* __restrict__ order, const int4* __restrict__ rec) {
+__device__ __forceinline__ void schur_chunk(int wg, const int2* __restrict__ items, const int* __restrict__ sb,
+                                            const int* __restrict__ se, const int* __restrict__ obs_pt,
+                                            const R* __restrict__ W, const double* __restrict__ Vinv,
+                                            R* __restrict__ part, const int* __restrict__ ownr,
+                                            const unsigned char* __restrict__ run, int nchunk,
+                                            const int* __restrict__ order, const int4* __restrict__ rec, R* sBuf,
+                                            int* sIdx, int* sMeta) {
   constexpr int C2 = CNP * CNP, REC = 3 * CNP;
   constexpr int NBK = (CNP + 2) / 3;  // 3-wide row / column blocks
   constexpr int NB2 = NBK * NBK;
   constexpr int G = 256 / NB2;        // item groups
   static_assert(G * C2 <= 2 * kPairChunk * REC, "group partials must fit the staging buffer");
-  __shared__ R sBuf[2 * kPairChunk * REC];
-  __shared__ int sIdx[3 * kPairChunk];
-  __shared__ int sMeta[2];
   R* const sY = sBuf;
   R* const sW = sBuf + kPairChunk * REC;
 #if PSBA_SCHUR_XCD
-  int b = xcd_item(blockIdx.x, nchunk);  // one share of the launch order per XCD
+  int b = xcd_item(wg, nchunk);  // one share of the launch order per XCD
   if (b < 0) return;
 #else
-  int b = blockIdx.x;
+  int b = wg;
 #endif
   const int t = threadIdx.x;
   int cnt;
@@ -178,19 +179,31 @@ __global__ __launch_bounds__(256) void k_schur_chunks(const int2* __restrict__ i
   }
 }
 
+template <int CNP, typename R, bool SLOTS>
+__global__ __launch_bounds__(256) void k_schur_chunks(const int2* __restrict__ items, const int* __restrict__ sb,
+                                                      const int* __restrict__ se, const int* __restrict__ obs_pt,
+                                                      const R* __restrict__ W, const double* __restrict__ Vinv,
+                                                      R* __restrict__ part, const int* __restrict__ ownr,
+                                                      const unsigned char* __restrict__ run, int nchunk,
+                                                      const int* __restrict__ order, const int4* __restrict__ rec) {
+  __shared__ R sBuf[2 * kPairChunk * 3 * CNP];
+  __shared__ int sIdx[3 * kPairChunk];
+  __shared__ int sMeta[2];
+  schur_chunk<CNP, R, SLOTS>(blockIdx.x, items, sb, se, obs_pt, W, Vinv, part, ownr, run, nchunk, order, rec, sBuf,
+                             sIdx, sMeta);
+}
+
 // one block per lower block position (k >= l) of S, including empty ones
 // Blocks [0, nlb) write the lower block positions; the remaining blocks
 // rewrite the padding rows n..n_pad-1 as identity rows (a failed
 // factorisation may have left anything there).
 template <int CNP, typename R>
-__global__ __launch_bounds__(256) void k_schur_finish(const int* __restrict__ lblock_pair,
-                                                      const int* __restrict__ psp, const R* __restrict__ part,
-                                                      const double* __restrict__ U,
-                                                      const unsigned char* __restrict__ fix, double mu,
-                                                      const double* __restrict__ mu_p, int ld, int nlb, int n,
-                                                      double* __restrict__ S) {
+__device__ __forceinline__ void schur_finish_block(int b, const int* __restrict__ lblock_pair,
+                                                   const int* __restrict__ psp, const R* __restrict__ part,
+                                                   const double* __restrict__ U, const unsigned char* __restrict__ fix,
+                                                   double mu, const double* __restrict__ mu_p, int ld, int nlb, int n,
+                                                   double* __restrict__ S) {
   constexpr int C2 = CNP * CNP;
-  const int b = blockIdx.x;
   if (mu_p) mu = *mu_p;
   if (b >= nlb) {
     const int r = n + (b - nlb);
@@ -219,18 +232,30 @@ __global__ __launch_bounds__(256) void k_schur_finish(const int* __restrict__ lb
   }
 }
 
-// e_a chunks over the camera chunks: p_o = W_o z_i, column sums
 template <int CNP, typename R>
-__global__ __launch_bounds__(256) void k_ea_chunks(const int* __restrict__ cam_obs, const int* __restrict__ cb,
-                                                   const int* __restrict__ ce, const int* __restrict__ obs_pt,
-                                                   const R* __restrict__ W, const double* __restrict__ z,
-                                                   R* __restrict__ part, const int* __restrict__ ownr,
-                                                   const unsigned char* __restrict__ run,
-                                                   const int2* __restrict__ rec) {
-  constexpr int CH = (CNP > 11) ? 128 : kCamChunk;
+__global__ __launch_bounds__(256) void k_schur_finish(const int* __restrict__ lblock_pair,
+                                                      const int* __restrict__ psp, const R* __restrict__ part,
+                                                      const double* __restrict__ U,
+                                                      const unsigned char* __restrict__ fix, double mu,
+                                                      const double* __restrict__ mu_p, int ld, int nlb, int n,
+                                                      double* __restrict__ S) {
+  schur_finish_block<CNP, R>(blockIdx.x, lblock_pair, psp, part, U, fix, mu, mu_p, ld, nlb, n, S);
+}
+
+// e_a chunks over the camera chunks: p_o = W_o z_i, column sums
+template <int CNP>
+constexpr int ea_chunk_obs() { return (CNP > 11) ? 128 : kCamChunk; }
+
+// the workgroup of chunk b, with its LDS (sP: ea_chunk_obs() CNP values,
+// sO and sI: ea_chunk_obs() each) given
+template <int CNP, typename R>
+__device__ __forceinline__ void ea_chunk(int b, const int* __restrict__ cam_obs, const int* __restrict__ cb,
+                                         const int* __restrict__ ce, const int* __restrict__ obs_pt,
+                                         const R* __restrict__ W, const double* __restrict__ z, R* __restrict__ part,
+                                         const int* __restrict__ ownr, const unsigned char* __restrict__ run,
+                                         const int2* __restrict__ rec, R* sP, int* sO, int* sI) {
+  constexpr int CH = ea_chunk_obs<CNP>();
   static_assert(CH <= 256, "one observation per thread");
-  __shared__ R sP[CH * CNP];
-  const int b = blockIdx.x;
   // with rec (one problem) the observation and its point come from the
   // chunk's record at an address the workgroup index gives, beside the extent
   const int2 r_rec = (rec && (int)threadIdx.x < CH) ? rec[(size_t)b * CH + threadIdx.x] : make_int2(-1, -1);
@@ -238,7 +263,6 @@ __global__ __launch_bounds__(256) void k_ea_chunks(const int* __restrict__ cam_o
   if (gated_off(run, ownr, b)) return;
   // the chunk's observation and point indices first (the dependent loads once
   // per observation), then the products with independent, coalesced loads
-  __shared__ int sO[CH], sI[CH];
   for (int s = threadIdx.x; s < cnt; s += blockDim.x) {
     if (rec) {
       sO[s] = r_rec.x;
@@ -271,10 +295,51 @@ __global__ __launch_bounds__(256) void k_ea_chunks(const int* __restrict__ cam_o
 }
 
 template <int CNP, typename R>
-__global__ __launch_bounds__(64) void k_ea_finish(const int* __restrict__ ccp, const R* __restrict__ part,
-                                                  const double* __restrict__ ga, int nc, double* __restrict__ ea,
-                                                  const int* __restrict__ ownr, const unsigned char* __restrict__ run) {
-  int t = blockIdx.x * blockDim.x + threadIdx.x;
+__global__ __launch_bounds__(256) void k_ea_chunks(const int* __restrict__ cam_obs, const int* __restrict__ cb,
+                                                   const int* __restrict__ ce, const int* __restrict__ obs_pt,
+                                                   const R* __restrict__ W, const double* __restrict__ z,
+                                                   R* __restrict__ part, const int* __restrict__ ownr,
+                                                   const unsigned char* __restrict__ run,
+                                                   const int2* __restrict__ rec) {
+  constexpr int CH = ea_chunk_obs<CNP>();
+  __shared__ R sP[CH * CNP];
+  __shared__ int sO[CH], sI[CH];
+  ea_chunk<CNP, R>(blockIdx.x, cam_obs, cb, ce, obs_pt, W, z, part, ownr, run, rec, sP, sO, sI);
+}
+
+// One problem's Schur chunks and e_a chunks in one launch (they share no
+// data): workgroups [0, nb_schur) are k_schur_chunks' (SLOTS form, the same
+// launch indices, so the same XCD shares), the rest k_ea_chunks'; the e_a
+// chunks take their LDS from the Schur staging buffer.  Each chunk's
+// arithmetic is its own kernel's, so the partials are the same bits.
+template <int CNP, typename R>
+__global__ __launch_bounds__(256) void k_schur_ea_chunks(const R* __restrict__ W, const double* __restrict__ Vinv,
+                                                         R* __restrict__ spart, int nchunk,
+                                                         const int4* __restrict__ rec, int nb_schur,
+                                                         const int* __restrict__ cb, const int* __restrict__ ce,
+                                                         const double* __restrict__ z, R* __restrict__ epart,
+                                                         const int2* __restrict__ erec) {
+  constexpr int CH = ea_chunk_obs<CNP>();
+  static_assert(CH * CNP * sizeof(R) + 2 * CH * sizeof(int) <= 2 * kPairChunk * 3 * CNP * sizeof(R),
+                "the e_a chunk's LDS must fit the Schur staging buffer");
+  __shared__ R sBuf[2 * kPairChunk * 3 * CNP];
+  __shared__ int sIdx[3 * kPairChunk];
+  __shared__ int sMeta[2];
+  if ((int)blockIdx.x < nb_schur) {
+    schur_chunk<CNP, R, true>(blockIdx.x, nullptr, nullptr, nullptr, nullptr, W, Vinv, spart, nullptr, nullptr, nchunk,
+                              nullptr, rec, sBuf, sIdx, sMeta);
+  } else {
+    int* const sO = (int*)(sBuf + CH * CNP);
+    ea_chunk<CNP, R>((int)blockIdx.x - nb_schur, nullptr, cb, ce, nullptr, W, z, epart, nullptr, nullptr, erec, sBuf,
+                     sO, sO + CH);
+  }
+}
+
+// e_a entry t (camera t / CNP, row t % CNP)
+template <int CNP, typename R>
+__device__ __forceinline__ void ea_finish_one(int t, const int* __restrict__ ccp, const R* __restrict__ part,
+                                              const double* __restrict__ ga, int nc, double* __restrict__ ea,
+                                              const int* __restrict__ ownr, const unsigned char* __restrict__ run) {
   if (t >= nc * CNP) return;
   int j = t / CNP, r = t - j * CNP;
   if (run && !run[ownr[j]]) return;
@@ -283,25 +348,48 @@ __global__ __launch_bounds__(64) void k_ea_finish(const int* __restrict__ ccp, c
 }
 
 template <int CNP, typename R>
+__global__ __launch_bounds__(64) void k_ea_finish(const int* __restrict__ ccp, const R* __restrict__ part,
+                                                  const double* __restrict__ ga, int nc, double* __restrict__ ea,
+                                                  const int* __restrict__ ownr, const unsigned char* __restrict__ run) {
+  ea_finish_one<CNP, R>(blockIdx.x * blockDim.x + threadIdx.x, ccp, part, ga, nc, ea, ownr, run);
+}
+
+// k_schur_finish and k_ea_finish in one launch (one problem): workgroups
+// [0, nfin) are k_schur_finish's, the rest take blockDim e_a entries each
+// (the same per-entry sums as k_ea_finish)
+template <int CNP, typename R>
+__global__ __launch_bounds__(256) void k_schur_ea_finish(const int* __restrict__ lblock_pair,
+                                                         const int* __restrict__ psp, const R* __restrict__ spart,
+                                                         const double* __restrict__ U,
+                                                         const unsigned char* __restrict__ fix,
+                                                         const double* __restrict__ mu_p, int ld, int nlb, int n,
+                                                         double* __restrict__ S, int nfin, const int* __restrict__ ccp,
+                                                         const R* __restrict__ epart, const double* __restrict__ ga,
+                                                         int nc, double* __restrict__ ea, double mu) {
+  if ((int)blockIdx.x < nfin)
+    schur_finish_block<CNP, R>(blockIdx.x, lblock_pair, psp, spart, U, fix, mu, mu_p, ld, nlb, n, S);
+  else
+    ea_finish_one<CNP, R>(((int)blockIdx.x - nfin) * blockDim.x + threadIdx.x, ccp, epart, ga, nc, ea, nullptr,
+                          nullptr);
+}
+
+template <int CNP, typename R>
 static void run_schur_t(psba_ctx* c, double mu, const R* W, R* schpart, R* eapart) {
-  if (c->n_schunk > 0) {
+  // the Schur and e_a chunks in one launch (the e_a chunks after the Schur
+  // chunks, so they fill the CUs that the Schur launch's tail leaves idle),
+  // then both finishes in one launch
+  const int nb_schur = c->n_schunk > 0 ? (int)(PSBA_SCHUR_XCD ? xcd_grid(c->n_schunk) : c->n_schunk) : 0;
+  if (nb_schur + c->n_cchunk > 0) {
     probe_begin(c, PSBA_PROBE_SCHUR);
-    hipLaunchKernelGGL((k_schur_chunks<CNP, R, true>), dim3(PSBA_SCHUR_XCD ? xcd_grid(c->n_schunk) : c->n_schunk),
-                       dim3(256), 0, c->stream, c->pair_items, c->schunk_begin, c->schunk_end, c->obs_pt, W, c->Vinv,
-                       schpart, (const int*)nullptr, (const unsigned char*)nullptr, c->n_schunk,
-                       (const int*)c->schunk_order, (const int4*)c->slot_items);
+    hipLaunchKernelGGL((k_schur_ea_chunks<CNP, R>), dim3(nb_schur + c->n_cchunk), dim3(256), 0, c->stream, W, c->Vinv,
+                       schpart, c->n_schunk, (const int4*)c->slot_items, nb_schur, c->cchunk_begin, c->cchunk_end,
+                       c->z, eapart, (const int2*)c->cchunk_rec);
     probe_end(c, PSBA_PROBE_SCHUR);
   }
-  hipLaunchKernelGGL((k_schur_finish<CNP, R>), dim3(c->n_lblocks + (c->n_pad - c->n)),
-                     dim3(CNP * CNP > 128 ? 256 : 128), 0, c->stream, c->lblock_pair, c->pair_schunk_ptr, schpart, c->U,
-                     c->fix, mu, mu_of(c), c->n_pad, c->n_lblocks, c->n, c->S);
-  if (c->n_cchunk > 0)
-    hipLaunchKernelGGL((k_ea_chunks<CNP, R>), dim3(c->n_cchunk), dim3(256), 0, c->stream, c->cam_obs,
-                       c->cchunk_begin, c->cchunk_end, c->obs_pt, W, c->z, eapart, (const int*)nullptr,
-                       (const unsigned char*)nullptr, (const int2*)c->cchunk_rec);
-  hipLaunchKernelGGL((k_ea_finish<CNP, R>), dim3(ceil_div((long)c->nc * CNP, 64)), dim3(64), 0, c->stream,
-                     c->cam_cchunk_ptr, eapart, c->g, c->nc, c->ea, (const int*)nullptr,
-                     (const unsigned char*)nullptr);
+  const int nfin = c->n_lblocks + (c->n_pad - c->n), bs = CNP * CNP > 128 ? 256 : 128;
+  hipLaunchKernelGGL((k_schur_ea_finish<CNP, R>), dim3(nfin + ceil_div((long)c->nc * CNP, bs)), dim3(bs), 0,
+                     c->stream, c->lblock_pair, c->pair_schunk_ptr, schpart, c->U, c->fix, mu_of(c), c->n_pad,
+                     c->n_lblocks, c->n, c->S, nfin, c->cam_cchunk_ptr, eapart, c->g, c->nc, c->ea, mu);
 }
 
 template <int CNP>
