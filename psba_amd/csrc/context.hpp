@@ -146,7 +146,6 @@ struct psba_ctx {
   unsigned char* gate = nullptr;
   psba_trace_rec* lm_trace = nullptr;
   int lm_trace_cap = 0;
-  int* zero_owner = nullptr;
   psba::LmDev* h_lm = nullptr;  // pinned, two slots
   hipEvent_t ev_lm[2] = {nullptr, nullptr};
 
@@ -174,6 +173,7 @@ int launch_maxdiag(psba_ctx* c);                          // -> d_scal[SC_MAXDIA
 int launch_vinv(psba_ctx* c, double mu);                  // Vinv, z
 int launch_backsub(psba_ctx* c);                          // eb, dp[n..)
 int launch_step_stats(psba_ctx* c, double mu);            // SC_DP2, SC_RHODEN
+int launch_jacobian_normal_vinv(psba_ctx* c);  // device control, after the first trial: the stage + V^-1
 int launch_trial_eval(psba_ctx* c);  // device control: step statistics + trial cost partials, one launch
 int launch_commit(psba_ctx* c);                           // p += dp, SC_P2
 // out[SC_D0..] = dots of (J x1).(J x1), (J x1).(J x2), (J x2).(J x2); x2 may be null
@@ -211,7 +211,6 @@ int launch_lm_mu0(psba_ctx* c);
 int launch_lm_control(psba_ctx* c, LmDev* out);  // out: the state after the decision (pinned host slot)
 // gates of the launchers in device-control mode (nullptr otherwise)
 inline const unsigned char* gate_of(const psba_ctx* c, int which) { return c->devctl ? c->gate + which : nullptr; }
-inline const int* owner_of(const psba_ctx* c) { return c->devctl ? c->zero_owner : nullptr; }
 // Under device control a trial's cost partials (one per 256 observations) sit
 // in red_part after the step statistics' (two per 256 parameters), so that the
 // LM control kernel can sum both: red_part holds 8 per block of the larger.

@@ -287,7 +287,6 @@ int psba_destroy(psba_ctx* c) {
   if (c->lm) (void)hipFree(c->lm);
   if (c->gate) (void)hipFree(c->gate);
   if (c->lm_trace) (void)hipFree(c->lm_trace);
-  if (c->zero_owner) (void)hipFree(c->zero_owner);
   if (c->h_lm) (void)hipHostFree(c->h_lm);
   for (auto e : c->ev_lm)
     if (e) (void)hipEventDestroy(e);

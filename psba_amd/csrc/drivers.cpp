@@ -140,12 +140,6 @@ int lm_device_alloc(psba_ctx* c, int trace_cap) {
     PSBA_HIP_TRY(hipMalloc((void**)&c->gate, kGateCount));
     PSBA_HIP_TRY(hipHostMalloc((void**)&c->h_lm, sizeof(LmDev) * kLmDepth, hipHostMallocDefault));
     for (auto& e : c->ev_lm) PSBA_HIP_TRY(hipEventCreateWithFlags(&e, hipEventDisableTiming));
-    // the batch kernels' per-element problem index (observations, points,
-    // cameras, camera chunks, pair chunks): 0 everywhere for one problem
-    const long nown = std::max<long>(std::max<long>(c->no, c->np), std::max<long>(std::max(c->nc, c->n_cchunk),
-                                                                                   c->n_schunk)) + 1;
-    PSBA_HIP_TRY(hipMalloc((void**)&c->zero_owner, sizeof(int) * (size_t)nown));
-    PSBA_HIP_TRY(hipMemsetAsync(c->zero_owner, 0, sizeof(int) * (size_t)nown, c->stream));
   }
   if (trace_cap > c->lm_trace_cap) {
     if (c->lm_trace) (void)hipFree(c->lm_trace);
@@ -158,9 +152,13 @@ int lm_device_alloc(psba_ctx* c, int trace_cap) {
 }
 
 int lm_enqueue_trial(psba_ctx* c, int slot, bool first) {
-  PSBA_TRY(launch_jacobian_normal(c, 1.0, 1.0));
-  if (first) PSBA_TRY(launch_lm_mu0(c));
-  PSBA_TRY(launch_vinv(c, 0.0));
+  if (first) {
+    PSBA_TRY(launch_jacobian_normal(c, 1.0, 1.0));
+    PSBA_TRY(launch_lm_mu0(c));
+    PSBA_TRY(launch_vinv(c, 0.0));
+  } else {
+    PSBA_TRY(launch_jacobian_normal_vinv(c));
+  }
   PSBA_TRY(launch_schur(c, 0.0));
   PSBA_TRY(launch_factor_solve(c));
   PSBA_TRY(launch_backsub(c));

@@ -158,11 +158,13 @@ __global__ __launch_bounds__(256) void k_jacobian(const double* __restrict__ K, 
   // one element in each of 64 rows.  Rows of batched problems that do not
   // need a Jacobian (run) are left as they are.
   extern __shared__ double sJac[];
+  // one problem (no owner map): run[0] gates the whole launch
+  if (run && !ownr && !run[0]) return;
   R* const sb = (R*)sJac;
   unsigned char* const sAct = (unsigned char*)(sJac) + (size_t)blockDim.x * (3 * CNP + 1) * sizeof(R);
   const int o0 = blockIdx.x * blockDim.x, nrow = min((int)blockDim.x, no - o0);
   const int o = o0 + threadIdx.x;
-  const bool act = o < no && (!run || run[ownr[o]]);
+  const bool act = o < no && (!run || run[ownr ? ownr[o] : 0]);
   sAct[threadIdx.x] = act;
   auto put = [&](int L, const R* v) {
     if (act)
@@ -231,19 +233,20 @@ constexpr size_t jacobian_lds(int bs) {
 }
 
 // V_i (6 unique entries), g_b,i and the V part of UVdiag, one thread per point
-// over its observations in ascending camera order.
+// over its observations in ascending camera order (workgroup `blk`).
 template <typename R>
-__global__ __launch_bounds__(256) void k_point_normal(const int* __restrict__ pt_ptr, const R* __restrict__ JB,
-                                                      const R* __restrict__ e, int np, double coeff,
-                                                      double coeff_g, double* __restrict__ V,
-                                                      double* __restrict__ gb, double* __restrict__ uvd,
-                                                      const int* __restrict__ ownr,
-                                                      const unsigned char* __restrict__ run) {
+__device__ __forceinline__ void point_normal_block(int blk, const int* __restrict__ pt_ptr,
+                                                   const R* __restrict__ JB, const R* __restrict__ e, int np,
+                                                   double coeff, double coeff_g, double* __restrict__ V,
+                                                   double* __restrict__ gb, double* __restrict__ uvd,
+                                                   const int* __restrict__ ownr,
+                                                   const unsigned char* __restrict__ run) {
+  if (run && !ownr && !run[0]) return;  // one problem, gated off
   // a quad per point: lane p sums a quarter of the point's observations (a
   // long track is split four ways), the quad adds the quarters in a fixed order
-  const long gt = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long gt = (long)blk * blockDim.x + threadIdx.x;
   const int i = (int)(gt >> 2), p = (int)(gt & 3);
-  const bool ok = i < np && (!run || run[ownr[i]]);
+  const bool ok = i < np && (!run || run[ownr ? ownr[i] : 0]);
   R v[6] = {0, 0, 0, 0, 0, 0}, g[3] = {0, 0, 0};
   if (ok) {
     const int beg = pt_ptr[i], cnt = pt_ptr[i + 1] - beg;
@@ -278,21 +281,32 @@ __global__ __launch_bounds__(256) void k_point_normal(const int* __restrict__ pt
   uvd[(size_t)i * 3 + 2] = coeff * (double)v[5];
 }
 
+template <typename R>
+__global__ __launch_bounds__(256) void k_point_normal(const int* __restrict__ pt_ptr, const R* __restrict__ JB,
+                                                      const R* __restrict__ e, int np, double coeff,
+                                                      double coeff_g, double* __restrict__ V,
+                                                      double* __restrict__ gb, double* __restrict__ uvd,
+                                                      const int* __restrict__ ownr,
+                                                      const unsigned char* __restrict__ run) {
+  point_normal_block<R>(blockIdx.x, pt_ptr, JB, e, np, coeff, coeff_g, V, gb, uvd, ownr, run);
+}
+
 // Camera-segmented sums of the U_j upper triangle and g_a,j: stage 1.  Each
 // block takes one chunk (<= kCamChunk observations of one camera, ascending
 // point), stages JA and e in LDS, and thread t sums entry t over the chunk.
+template <int CNP>
+constexpr int cam_chunk_obs() { return (CNP > 11) ? 128 : kCamChunk; }
+
+// chunk b, staging in sJ (cam_chunk_obs() (2 CNP + 2) values of LDS)
 template <int CNP, typename R>
-__global__ __launch_bounds__(256) void k_cam_chunks(const int* __restrict__ cam_obs, const int* __restrict__ cb,
-                                                    const int* __restrict__ ce, const R* __restrict__ JA,
-                                                    const R* __restrict__ e, R* __restrict__ part,
-                                                    const int* __restrict__ ownr,
-                                                    const unsigned char* __restrict__ run,
-                                                    const int2* __restrict__ rec) {
+__device__ __forceinline__ void cam_chunk(int b, const int* __restrict__ cam_obs, const int* __restrict__ cb,
+                                          const int* __restrict__ ce, const R* __restrict__ JA,
+                                          const R* __restrict__ e, R* __restrict__ part,
+                                          const int* __restrict__ ownr, const unsigned char* __restrict__ run,
+                                          const int2* __restrict__ rec, R* sJ) {
   constexpr int NU = CNP * (CNP + 1) / 2, NE = NU + CNP, REC = 2 * CNP + 2;
-  constexpr int CH = (CNP > 11) ? 128 : kCamChunk;
+  constexpr int CH = cam_chunk_obs<CNP>();
   static_assert(CH <= 256, "one observation per thread");
-  __shared__ R sJ[CH * REC];
-  const int b = blockIdx.x;
   // with rec (one problem) the observation comes from the chunk's record at
   // an address the workgroup index gives, loaded beside the chunk's extent
   // and the gate
@@ -342,16 +356,45 @@ __global__ __launch_bounds__(256) void k_cam_chunks(const int* __restrict__ cam_
   }
 }
 
-// stage 2: per camera, sum its chunk partials in chunk order.
 template <int CNP, typename R>
-__global__ __launch_bounds__(256) void k_cam_finish(const int* __restrict__ ccp, const R* __restrict__ part,
-                                                    int nc, double coeff, double coeff_g, double* __restrict__ U,
-                                                    double* __restrict__ ga, double* __restrict__ uvd,
+__global__ __launch_bounds__(256) void k_cam_chunks(const int* __restrict__ cam_obs, const int* __restrict__ cb,
+                                                    const int* __restrict__ ce, const R* __restrict__ JA,
+                                                    const R* __restrict__ e, R* __restrict__ part,
                                                     const int* __restrict__ ownr,
-                                                    const unsigned char* __restrict__ run) {
+                                                    const unsigned char* __restrict__ run,
+                                                    const int2* __restrict__ rec) {
+  __shared__ R sJ[cam_chunk_obs<CNP>() * (2 * CNP + 2)];
+  cam_chunk<CNP, R>(blockIdx.x, cam_obs, cb, ce, JA, e, part, ownr, run, rec, sJ);
+}
+
+// One problem's point normal equations and camera chunks in one launch (both
+// read only the Jacobian stage's rows): workgroups [0, nb_pt) are
+// k_point_normal's, the rest k_cam_chunks' (the same per-workgroup work).
+template <int CNP, typename R>
+__global__ __launch_bounds__(256) void k_normal_chunks(const int* __restrict__ pt_ptr, const R* __restrict__ JB,
+                                                       const R* __restrict__ e, int np, double coeff, double coeff_g,
+                                                       double* __restrict__ V, double* __restrict__ gb,
+                                                       double* __restrict__ uvd_b, int nb_pt,
+                                                       const int* __restrict__ cb, const int* __restrict__ ce,
+                                                       const R* __restrict__ JA, R* __restrict__ part,
+                                                       const int2* __restrict__ rec,
+                                                       const unsigned char* __restrict__ run) {
+  __shared__ R sJ[cam_chunk_obs<CNP>() * (2 * CNP + 2)];
+  if ((int)blockIdx.x < nb_pt)
+    point_normal_block<R>(blockIdx.x, pt_ptr, JB, e, np, coeff, coeff_g, V, gb, uvd_b, nullptr, run);
+  else
+    cam_chunk<CNP, R>((int)blockIdx.x - nb_pt, nullptr, cb, ce, JA, e, part, nullptr, run, rec, sJ);
+}
+
+// stage 2: per camera, sum its chunk partials in chunk order (camera j).
+template <int CNP, typename R>
+__device__ __forceinline__ void cam_finish_block(int j, const int* __restrict__ ccp, const R* __restrict__ part,
+                                                 int nc, double coeff, double coeff_g, double* __restrict__ U,
+                                                 double* __restrict__ ga, double* __restrict__ uvd,
+                                                 const int* __restrict__ ownr,
+                                                 const unsigned char* __restrict__ run) {
   constexpr int NU = CNP * (CNP + 1) / 2, NE = NU + CNP;
-  const int j = blockIdx.x;
-  if (run && !run[ownr[j]]) return;
+  if (run && !run[ownr ? ownr[j] : 0]) return;
   for (int t = threadIdx.x; t < NE; t += blockDim.x) {
     const double acc = ordered_sum<16>(ccp[j], ccp[j + 1], [&](int q) { return (double)part[(size_t)q * NE + t]; });
     if (t < NU) {
@@ -372,58 +415,12 @@ __global__ __launch_bounds__(256) void k_cam_finish(const int* __restrict__ ccp,
 }
 
 template <int CNP, typename R>
-static void run_jac_normal_t(psba_ctx* c, double coeff, double coeff_g, R* e, R* JA, R* JB, R* W, R* ucpart) {
-  probe_begin(c, PSBA_PROBE_JACOBIAN);
-  hipLaunchKernelGGL((k_jacobian<CNP, R>), dim3(ceil_div(c->no, kJacBlock)), dim3(kJacBlock),
-                     (jacobian_lds<CNP, R>(kJacBlock)), c->stream, c->K, c->q0,
-                     c->cams, c->pts, c->meas, c->obs_cam, c->obs_pt, c->fix, c->no, coeff, e, JA, JB, W,
-                     owner_of(c), gate_of(c, kGateJac), c->loss, c->loss_scale);
-  probe_end(c, PSBA_PROBE_JACOBIAN);
-  hipLaunchKernelGGL(k_point_normal<R>, dim3(ceil_div(4L * c->np, 256)), dim3(256), 0, c->stream, c->pt_ptr, JB, e,
-                     c->np, coeff, coeff_g, c->V, c->g + c->n, c->uvdiag + c->n, owner_of(c),
-                     gate_of(c, kGateJac));
-  if (c->n_cchunk > 0)
-    hipLaunchKernelGGL((k_cam_chunks<CNP, R>), dim3(c->n_cchunk), dim3(256), 0, c->stream, c->cam_obs,
-                       c->cchunk_begin, c->cchunk_end, JA, e, ucpart, (const int*)nullptr, gate_of(c, kGateJac),
-                       (const int2*)c->cchunk_rec);
-  hipLaunchKernelGGL((k_cam_finish<CNP, R>), dim3(c->nc), dim3(256), 0, c->stream, c->cam_cchunk_ptr, ucpart, c->nc,
-                     coeff, coeff_g, c->U, c->g, c->uvdiag, owner_of(c), gate_of(c, kGateJac));
-}
-
-template <int CNP>
-static void run_jac_normal(psba_ctx* c, double coeff, double coeff_g) {
-  if (c->mixed())
-    run_jac_normal_t<CNP, float>(c, coeff, coeff_g, c->e32, c->JA32, c->JB32, c->W32, c->ucpart32);
-  else
-    run_jac_normal_t<CNP, double>(c, coeff, coeff_g, c->e, c->JA, c->JB, c->W, c->ucpart);
-}
-
-int launch_jacobian_normal(psba_ctx* c, double coeff, double coeff_g) {
-  PSBA_DISPATCH_CNP(c->cnp, run_jac_normal, c, coeff, coeff_g);
-  PSBA_LAUNCH_TRY();
-  return PSBA_OK;
-}
-
-// max over UVdiag (single block; once per LM entry)
-__global__ __launch_bounds__(1024) void k_maxdiag(const double* __restrict__ d, int n, double* out) {
-  __shared__ double sh[16];
-  double m = 0.0;
-  for (int q = threadIdx.x; q < n; q += blockDim.x) m = fmax(m, d[q]);
-#pragma unroll
-  for (int off = 32; off >= 1; off >>= 1) m = fmax(m, __shfl_xor(m, off, kWave));
-  if ((threadIdx.x & 63) == 0) sh[threadIdx.x / 64] = m;
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    double r = 0.0;
-    for (int w = 0; w < (int)(blockDim.x / 64); ++w) r = fmax(r, sh[w]);
-    *out = r;
-  }
-}
-
-int launch_maxdiag(psba_ctx* c) {
-  hipLaunchKernelGGL(k_maxdiag, dim3(1), dim3(1024), 0, c->stream, c->uvdiag, c->ntot, c->d_scal + SC_MAXDIAG);
-  PSBA_LAUNCH_TRY();
-  return PSBA_OK;
+__global__ __launch_bounds__(256) void k_cam_finish(const int* __restrict__ ccp, const R* __restrict__ part,
+                                                    int nc, double coeff, double coeff_g, double* __restrict__ U,
+                                                    double* __restrict__ ga, double* __restrict__ uvd,
+                                                    const int* __restrict__ ownr,
+                                                    const unsigned char* __restrict__ run) {
+  cam_finish_block<CNP, R>(blockIdx.x, ccp, part, nc, coeff, coeff_g, U, ga, uvd, ownr, run);
 }
 
 // ---------------------------------------------------------------------------
@@ -433,11 +430,10 @@ int launch_maxdiag(psba_ctx* c) {
 // |T| < 1e-16: the reference's pivoted-elimination divisor (compute_Vinv.cl:31-73).
 // Batched problems pass mu_p (damping per problem) with ownr (point -> problem)
 // and run (problem -> still iterating); the single-problem path passes mu.
-__global__ __launch_bounds__(256) void k_vinv(const double* __restrict__ V, const double* __restrict__ gb, int np,
-                                              double mu, double* __restrict__ Vinv, double* __restrict__ z,
-                                              const double* __restrict__ mu_p, const int* __restrict__ ownr,
-                                              const unsigned char* __restrict__ run) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
+__device__ __forceinline__ void vinv_one(int i, const double* __restrict__ V, const double* __restrict__ gb, int np,
+                                         double mu, double* __restrict__ Vinv, double* __restrict__ z,
+                                         const double* __restrict__ mu_p, const int* __restrict__ ownr,
+                                         const unsigned char* __restrict__ run) {
   if (i >= np) return;
   if (mu_p) {
     const int p = ownr ? ownr[i] : 0;
@@ -485,6 +481,97 @@ __global__ __launch_bounds__(256) void k_vinv(const double* __restrict__ V, cons
   z[3 * i + 0] = w[0] * g0 + w[1] * g1 + w[2] * g2;
   z[3 * i + 1] = w[1] * g0 + w[3] * g1 + w[4] * g2;
   z[3 * i + 2] = w[2] * g0 + w[4] * g1 + w[5] * g2;
+}
+
+__global__ __launch_bounds__(256) void k_vinv(const double* __restrict__ V, const double* __restrict__ gb, int np,
+                                              double mu, double* __restrict__ Vinv, double* __restrict__ z,
+                                              const double* __restrict__ mu_p, const int* __restrict__ ownr,
+                                              const unsigned char* __restrict__ run) {
+  vinv_one(blockIdx.x * blockDim.x + threadIdx.x, V, gb, np, mu, Vinv, z, mu_p, ownr, run);
+}
+
+// One problem under device control: k_cam_finish (gated like the Jacobian
+// stage) and k_vinv (every trial, the damping from the LM state) in one
+// launch, workgroups [0, nc) the cameras, the rest 256 points each.  (The
+// first trial of a solve keeps them apart: mu0 needs U before V^-1.)
+template <int CNP, typename R>
+__global__ __launch_bounds__(256) void k_finish_vinv(const int* __restrict__ ccp, const R* __restrict__ part, int nc,
+                                                     double coeff, double coeff_g, double* __restrict__ U,
+                                                     double* __restrict__ ga, double* __restrict__ uvd,
+                                                     const unsigned char* __restrict__ run,
+                                                     const double* __restrict__ V, const double* __restrict__ gb,
+                                                     int np, double* __restrict__ Vinv, double* __restrict__ z,
+                                                     const double* __restrict__ mu_p) {
+  if ((int)blockIdx.x < nc)
+    cam_finish_block<CNP, R>(blockIdx.x, ccp, part, nc, coeff, coeff_g, U, ga, uvd, nullptr, run);
+  else
+    vinv_one(((int)blockIdx.x - nc) * blockDim.x + threadIdx.x, V, gb, np, 0.0, Vinv, z, mu_p, nullptr, nullptr);
+}
+
+template <int CNP, typename R>
+static void run_jac_normal_t(psba_ctx* c, double coeff, double coeff_g, R* e, R* JA, R* JB, R* W, R* ucpart,
+                             bool with_vinv) {
+  const unsigned char* gate = gate_of(c, kGateJac);
+  probe_begin(c, PSBA_PROBE_JACOBIAN);
+  hipLaunchKernelGGL((k_jacobian<CNP, R>), dim3(ceil_div(c->no, kJacBlock)), dim3(kJacBlock),
+                     (jacobian_lds<CNP, R>(kJacBlock)), c->stream, c->K, c->q0,
+                     c->cams, c->pts, c->meas, c->obs_cam, c->obs_pt, c->fix, c->no, coeff, e, JA, JB, W,
+                     (const int*)nullptr, gate, c->loss, c->loss_scale);
+  probe_end(c, PSBA_PROBE_JACOBIAN);
+  const int nb_pt = (int)ceil_div(4L * c->np, 256);
+  hipLaunchKernelGGL((k_normal_chunks<CNP, R>), dim3(nb_pt + c->n_cchunk), dim3(256), 0, c->stream, c->pt_ptr, JB, e,
+                     c->np, coeff, coeff_g, c->V, c->g + c->n, c->uvdiag + c->n, nb_pt, c->cchunk_begin,
+                     c->cchunk_end, JA, ucpart, (const int2*)c->cchunk_rec, gate);
+  if (with_vinv)
+    hipLaunchKernelGGL((k_finish_vinv<CNP, R>), dim3(c->nc + ceil_div(c->np, 256)), dim3(256), 0, c->stream,
+                       c->cam_cchunk_ptr, ucpart, c->nc, coeff, coeff_g, c->U, c->g, c->uvdiag, gate, c->V,
+                       c->g + c->n, c->np, c->Vinv, c->z, mu_of(c));
+  else
+    hipLaunchKernelGGL((k_cam_finish<CNP, R>), dim3(c->nc), dim3(256), 0, c->stream, c->cam_cchunk_ptr, ucpart,
+                       c->nc, coeff, coeff_g, c->U, c->g, c->uvdiag, (const int*)nullptr, gate);
+}
+
+template <int CNP>
+static void run_jac_normal(psba_ctx* c, double coeff, double coeff_g, bool with_vinv) {
+  if (c->mixed())
+    run_jac_normal_t<CNP, float>(c, coeff, coeff_g, c->e32, c->JA32, c->JB32, c->W32, c->ucpart32, with_vinv);
+  else
+    run_jac_normal_t<CNP, double>(c, coeff, coeff_g, c->e, c->JA, c->JB, c->W, c->ucpart, with_vinv);
+}
+
+int launch_jacobian_normal(psba_ctx* c, double coeff, double coeff_g) {
+  PSBA_DISPATCH_CNP(c->cnp, run_jac_normal, c, coeff, coeff_g, false);
+  PSBA_LAUNCH_TRY();
+  return PSBA_OK;
+}
+
+int launch_jacobian_normal_vinv(psba_ctx* c) {
+  if (!c->devctl) return PSBA_ERR_ARG;
+  PSBA_DISPATCH_CNP(c->cnp, run_jac_normal, c, 1.0, 1.0, true);
+  PSBA_LAUNCH_TRY();
+  return PSBA_OK;
+}
+
+// max over UVdiag (single block; once per LM entry)
+__global__ __launch_bounds__(1024) void k_maxdiag(const double* __restrict__ d, int n, double* out) {
+  __shared__ double sh[16];
+  double m = 0.0;
+  for (int q = threadIdx.x; q < n; q += blockDim.x) m = fmax(m, d[q]);
+#pragma unroll
+  for (int off = 32; off >= 1; off >>= 1) m = fmax(m, __shfl_xor(m, off, kWave));
+  if ((threadIdx.x & 63) == 0) sh[threadIdx.x / 64] = m;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double r = 0.0;
+    for (int w = 0; w < (int)(blockDim.x / 64); ++w) r = fmax(r, sh[w]);
+    *out = r;
+  }
+}
+
+int launch_maxdiag(psba_ctx* c) {
+  hipLaunchKernelGGL(k_maxdiag, dim3(1), dim3(1024), 0, c->stream, c->uvdiag, c->ntot, c->d_scal + SC_MAXDIAG);
+  PSBA_LAUNCH_TRY();
+  return PSBA_OK;
 }
 
 int launch_vinv(psba_ctx* c, double mu) {
