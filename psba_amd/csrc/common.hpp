@@ -99,10 +99,11 @@ __device__ inline void grid_sum_partials(double (&v)[K], double* partials, doubl
 // t, t + blockDim, .., then block_sum) and writes out[0..K) from thread 0.
 // Eight of a thread's blocks are loaded at a time (clamped to the last block,
 // the additions past the end skipped), so a batch waits for one load latency;
-// the additions keep the order b = t, t + blockDim, ...
+// the additions keep the order b = t, t + blockDim, ...  With res, thread 0
+// also gets the totals there.
 template <int K>
 __device__ inline void sum_partials_block(const double* __restrict__ partials, int nb, double* __restrict__ out,
-                                          double* sh) {
+                                          double* sh, double* res = nullptr) {
   constexpr int B = 8;
   double acc[K];
 #pragma unroll
@@ -129,7 +130,10 @@ __device__ inline void sum_partials_block(const double* __restrict__ partials, i
   block_sum<K>(acc, sh);
   if (threadIdx.x == 0) {
 #pragma unroll
-    for (int k = 0; k < K; ++k) out[k] = acc[k];
+    for (int k = 0; k < K; ++k) {
+      out[k] = acc[k];
+      if (res) res[k] = acc[k];
+    }
   }
 }
 

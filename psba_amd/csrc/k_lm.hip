@@ -100,34 +100,59 @@ __global__ __launch_bounds__(1024) void k_lm_mu0(LmDev* s, const unsigned char* 
   }
 }
 
-// the decision after one trial (PSBA/levmar.cpp:124-245); the state then
-// goes to `out`, a slot of pinned host memory that the host reads once the
-// trial's event has completed (no separate copy)
-__device__ void lm_decide(LmDev* s, unsigned char* gate, const double* __restrict__ scal,
-                          psba_trace_rec* __restrict__ trace);
+// The trial's figures for the decision: the factorisation flag, |dp|^2, the
+// gain-ratio denominator, the cost at p + dp and (after an accept) |p|^2.
+struct TrialFig {
+  double factor_ok, dp2, rhoden, cost, p2;
+};
+
+// the decision after one trial (PSBA/levmar.cpp:124-245)
+__device__ void lm_decide(LmDev* s, unsigned char* gate, const TrialFig& f, psba_trace_rec* __restrict__ trace);
 
 // One block: first the trial's step statistics (|dp|^2 and the gain-ratio
 // denominator, from k_trial_eval's step partials) and cost (its cost
 // partials) are summed into scal, as k_sum_partials<2> and <1> would, and,
 // after an accepted trial (p_pending), |p|^2 from the commit's partials; then
 // thread 0 decides.
+// The state is read into LDS with the partials' loads (one word per thread),
+// decided on there, and written back to `s` and to `out`, a slot of pinned
+// host memory that the host reads once the trial's event has completed, one
+// word per thread again.
 __global__ __launch_bounds__(256) void k_lm_control(LmDev* s, unsigned char* gate, double* __restrict__ scal,
                                                     psba_trace_rec* __restrict__ trace, LmDev* out,
                                                     const double* __restrict__ step_part, int nb_step,
                                                     const double* __restrict__ cost_part, int nb_cost,
                                                     const double* __restrict__ p2_part, int nb_p2) {
+  constexpr int NW = sizeof(LmDev) / 8;
+  static_assert(sizeof(LmDev) % 8 == 0, "LmDev moves as 8-byte words");
   __shared__ double sh[8];
-  sum_partials_block<2>(step_part, nb_step, scal + SC_DP2, sh);
-  sum_partials_block<1>(cost_part, nb_cost, scal + SC_COST, sh);
-  if (gate[kGateActive] && s->p_pending) sum_partials_block<1>(p2_part, nb_p2, scal + SC_P2, sh);
-  if (threadIdx.x == 0) {
-    lm_decide(s, gate, scal, trace);
-    *out = *s;
+  __shared__ LmDev ls;
+  const int t = threadIdx.x;
+  if (t < NW) ((unsigned long long*)&ls)[t] = ((const unsigned long long*)s)[t];
+  TrialFig f{0.0, 0.0, 0.0, 0.0, 0.0};
+  if (t == 0) f.factor_ok = scal[SC_FACTOR_OK];
+  const bool active = gate[kGateActive];
+  double step[2], cost[1], p2[1] = {0.0};
+  sum_partials_block<2>(step_part, nb_step, scal + SC_DP2, sh, step);
+  sum_partials_block<1>(cost_part, nb_cost, scal + SC_COST, sh, cost);
+  // (ls is visible here: the sums above pass workgroup barriers)
+  if (active && ls.p_pending) sum_partials_block<1>(p2_part, nb_p2, scal + SC_P2, sh, p2);
+  if (t == 0) {
+    f.dp2 = step[0];
+    f.rhoden = step[1];
+    f.cost = cost[0];
+    f.p2 = p2[0];
+    lm_decide(&ls, gate, f, trace);
+  }
+  __syncthreads();
+  if (t < NW) {
+    const unsigned long long v = ((const unsigned long long*)&ls)[t];
+    ((unsigned long long*)s)[t] = v;
+    ((unsigned long long*)out)[t] = v;
   }
 }
 
-__device__ void lm_decide(LmDev* s, unsigned char* gate, const double* __restrict__ scal,
-                          psba_trace_rec* __restrict__ trace) {
+__device__ void lm_decide(LmDev* s, unsigned char* gate, const TrialFig& f, psba_trace_rec* __restrict__ trace) {
   gate[kGateAccept] = 0;
   if (!gate[kGateActive]) {
     gate[kGateJac] = 0;
@@ -136,22 +161,22 @@ __device__ void lm_decide(LmDev* s, unsigned char* gate, const double* __restric
   s->trials++;
   gate[kGateJac] = 0;
   if (s->p_pending) {
-    s->p_L2 = scal[SC_P2];
+    s->p_L2 = f.p2;
     s->p_pending = 0;
   }
   const double nan = __builtin_nan("");
   psba_trace_rec rec{s->itno, 0, 0, 0, nan, s->mu, nan, 0.0, nan};
   // a timed-out wait inside the dense solve is an execution error, not a
   // rejected trial
-  if (scal[SC_FACTOR_OK] < 0.0) {
+  if (f.factor_ok < 0.0) {
     s->flag = PSBA_ERR_HIP;
     gate[kGateActive] = 0;
     return;
   }
-  const bool ok = scal[SC_FACTOR_OK] != 0.0;
+  const bool ok = f.factor_ok != 0.0;
   rec.factor_ok = ok;
   if (ok) {
-    const double dp_L2 = scal[SC_DP2];
+    const double dp_L2 = f.dp2;
     rec.dp_sq = dp_L2;
     if (dp_L2 < s->p_L2 * kStopThresh * kStopThresh) {
       s->flag = PSBA_ITER_DP_NO_CHANGE;
@@ -165,8 +190,8 @@ __device__ void lm_decide(LmDev* s, unsigned char* gate, const double* __restric
       lm_end_iteration(s, gate);
       return;
     }
-    const double new_ex_L2 = scal[SC_COST];
-    const double rho = (s->ex_L2 - new_ex_L2) / scal[SC_RHODEN];
+    const double new_ex_L2 = f.cost;
+    const double rho = (s->ex_L2 - new_ex_L2) / f.rhoden;
     rec.cost = new_ex_L2;
     rec.rho = rho;
     if (rho > 0) {
