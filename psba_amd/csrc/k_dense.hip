@@ -1153,10 +1153,10 @@ __global__ __launch_bounds__(256) void k_chol_solve(double* __restrict__ A, int 
 
   // ---- back substitution L^T x = y: r_i = y_i - sum_{j > i} L_ji^T x_j with
   // the terms subtracted in descending j, then x_i = L_ii^-T r_i.  Workgroup 0
-  // solves the chain and applies x_{i+2} and x_{i+1} to r_i itself; the other
-  // workgroups ("helpers") apply x_j, j >= i+3, to the rows they own as soon
-  // as x_j is published, so r_i is complete two steps before workgroup 0
-  // needs it.  No grid barriers: workgroup 0 publishes x_k through a flag
+  // solves the chain and applies x_{i+1} to r_i itself; the other workgroups
+  // ("helpers") apply x_j, j >= i+2, to the rows they own as soon as x_j is
+  // published, so r_i lacks only that term one step before workgroup 0 needs
+  // it.  No grid barriers: workgroup 0 publishes x_k through a flag
   // (bar[1] = epoch + T - k) and each helper publishes a completed row i
   // through its own flag (bar[2 + i] = epoch + 1).  Workgroup 0 loads the next
   // step's factor tiles and completed r while it is still working; a helper
@@ -1211,16 +1211,15 @@ __global__ __launch_bounds__(256) void k_chol_solve(double* __restrict__ A, int 
     }
     __syncthreads();
     for (int k = T - 1; k >= 0; --k) {
-      // r_{k-1} (complete two steps ahead when the helpers have rows in it):
+      // r_{k-1} (done by the helpers but for x_k when they have terms in it):
       // its flag is read now and checked after this step's solve, so that
-      // the round trip overlaps the products and the solve
-      const bool need_flag = k >= 1 && k - 1 + 3 <= T - 1;
+      // the round trip overlaps the product and the solve
+      const bool need_flag = k >= 1 && k - 1 + 2 <= T - 1;
       unsigned long long fseen = 0;
       if (need_flag && tid == 0)
         fseen = __hip_atomic_load(rflag + (k - 1), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      // sm.w = x_{k+1}, sm.rd = x_{k+2}; sm.X = L_{k+1,k}, sm.Y = L_{k+2,k}
+      // sm.w = x_{k+1}; sm.X = L_{k+1,k}
       double br = r_next;
-      if (k + 2 < T) br -= tile_tmatvec(sm, sm.Y, TP, sm.rd);
       if (k + 1 < T) br -= tile_tmatvec(sm, sm.X, TP, sm.w);
       __syncthreads();
       tile_backward(LDS(sm.P), br, xs + (size_t)k * NB, LDS(sm.v), rows_of(k));
@@ -1228,22 +1227,19 @@ __global__ __launch_bounds__(256) void k_chol_solve(double* __restrict__ A, int 
       // once, while waves 1-3 load
       if (k >= 1 && tid == 0) flag_publish(xflag, epoch + (unsigned long long)(T - k));
       // Waves 1-3 return from tile_backward at once and load the next
-      // step's tiles while wave 0 solves: L_{k,k-1} and L_{k+1,k-1} straight
-      // into sm.X and sm.Y (the products above have read them), L_{k-1,k-1}
-      // into registers until wave 0 is done with sm.P.
+      // step's tiles while wave 0 solves: L_{k,k-1} straight into sm.X (the
+      // product above has read it), L_{k-1,k-1} into registers until wave 0
+      // is done with sm.P.
       double p0[22];
       const bool loader = k >= 1 && tid >= 64;
       if (loader) {
-        double p1[22], p2[22];
+        double p1[22];
         tile_regs_w123(p0, tile(Lf, k - 1, k - 1), ld);
         tile_regs_w123(p1, tile(Lf, k, k - 1), ld);
-        if (k + 1 < T) tile_regs_w123(p2, tile(Lf, k + 1, k - 1), ld);
         regs_to_lds_w123(sm.X, p1, false);
-        if (k + 1 < T) regs_to_lds_w123(sm.Y, p2, false);
       }
       __syncthreads();
       if (tid < NB) {
-        sm.rd[tid] = sm.w[tid];
         sm.w[tid] = sm.v[tid];
         const int q = k * NB + tid;
         if (q < n) dp[q] = sm.v[tid];
@@ -1255,28 +1251,28 @@ __global__ __launch_bounds__(256) void k_chol_solve(double* __restrict__ A, int 
       __syncthreads();
     }
   } else {
-    // helper: rows i0, i0 + stride, .. (those with i + 3 <= T - 1)
+    // helper: rows i0, i0 + stride, .. (those with i + 2 <= T - 1)
     const int i0 = (int)blockIdx.x - 1, stride = G - 1;
-    if (i0 + 3 <= T - 1) {
+    if (i0 + 2 <= T - 1) {
       double r_own = (tid < NB) ? yv[(size_t)i0 * NB + tid] : 0.0;
       load_full(sm.X, tile(Lf, T - 1, i0), ld);
       __syncthreads();
-      for (int j = T - 1; j >= i0 + 3; --j) {
+      for (int j = T - 1; j >= i0 + 2; --j) {
         flag_wait(xflag, epoch + (unsigned long long)(T - j), fail);
         if (tid < NB) sm.v[tid] = xs[(size_t)j * NB + tid];
         __syncthreads();
-        for (int i = i0; i + 3 <= j; i += stride) {
+        for (int i = i0; i + 2 <= j; i += stride) {
           if (i == i0) {
             const double t = tile_tmatvec(sm, sm.X, TP, sm.v);
             r_own -= t;
-            if (i + 3 == j && tid < NB) yv[(size_t)i * NB + tid] = r_own;
+            if (i + 2 == j && tid < NB) yv[(size_t)i * NB + tid] = r_own;
           } else {
             const double t = tile_tmatvec(sm, tile(Lf, j, i), ld, sm.v);
             if (tid < NB) yv[(size_t)i * NB + tid] -= t;
           }
-          if (i + 3 == j && tid == 0) flag_publish(rflag + i, epoch + 1);  // stores by wave 0
+          if (i + 2 == j && tid == 0) flag_publish(rflag + i, epoch + 1);  // stores by wave 0
         }
-        if (j - 1 >= i0 + 3) {
+        if (j - 1 >= i0 + 2) {
           load_full(sm.X, tile(Lf, j - 1, i0), ld);
           __syncthreads();
         }
