@@ -308,30 +308,48 @@ __global__ __launch_bounds__(256) void k_ea_chunks(const int* __restrict__ cam_o
 }
 
 // One problem's Schur chunks and e_a chunks in one launch (they share no
-// data): workgroups [0, nb_schur) are k_schur_chunks' (SLOTS form, the same
-// launch indices, so the same XCD shares), the rest k_ea_chunks'; the e_a
-// chunks take their LDS from the Schur staging buffer.  Each chunk's
-// arithmetic is its own kernel's, so the partials are the same bits.
+// data).  The launch is in groups of eight workgroups (one per XCD); of every
+// period + 1 groups, the first `period` are Schur groups and the last an e_a
+// group, until the e_a groups (neg) are placed, then Schur groups only.  A
+// Schur group gs gives workgroup index 8 gs + blockIdx % 8, the index
+// k_schur_chunks' workgroup on the same XCD has (SLOTS form, so the same XCD
+// shares of the launch order); e_a group eg takes camera chunk
+// 8 eg + blockIdx % 8.  The memory-streaming e_a chunks so run beside the
+// latency-bound Schur chunks.  The e_a chunks take their LDS from the Schur
+// staging buffer.  Each chunk's arithmetic is its own kernel's, so the
+// partials are the same bits.
 template <int CNP, typename R>
 __global__ __launch_bounds__(256) void k_schur_ea_chunks(const R* __restrict__ W, const double* __restrict__ Vinv,
                                                          R* __restrict__ spart, int nchunk,
-                                                         const int4* __restrict__ rec, int nb_schur,
-                                                         const int* __restrict__ cb, const int* __restrict__ ce,
-                                                         const double* __restrict__ z, R* __restrict__ epart,
-                                                         const int2* __restrict__ erec) {
+                                                         const int4* __restrict__ rec, int nb_schur, int neg,
+                                                         int period, int ncc, const int* __restrict__ cb,
+                                                         const int* __restrict__ ce, const double* __restrict__ z,
+                                                         R* __restrict__ epart, const int2* __restrict__ erec) {
   constexpr int CH = ea_chunk_obs<CNP>();
   static_assert(CH * CNP * sizeof(R) + 2 * CH * sizeof(int) <= 2 * kPairChunk * 3 * CNP * sizeof(R),
                 "the e_a chunk's LDS must fit the Schur staging buffer");
   __shared__ R sBuf[2 * kPairChunk * 3 * CNP];
   __shared__ int sIdx[3 * kPairChunk];
   __shared__ int sMeta[2];
-  if ((int)blockIdx.x < nb_schur) {
-    schur_chunk<CNP, R, true>(blockIdx.x, nullptr, nullptr, nullptr, nullptr, W, Vinv, spart, nullptr, nullptr, nchunk,
+  const int g = (int)blockIdx.x >> 3, l8 = (int)blockIdx.x & 7;
+  const int q = g / (period + 1), rm = g - q * (period + 1);
+  int gs = -1, eg = -1;
+  if (q < neg) {
+    if (rm == period) eg = q;
+    else gs = q * period + rm;
+  } else {
+    gs = neg * period + (g - neg * (period + 1));
+  }
+  if (gs >= 0) {
+    const int wg = 8 * gs + l8;
+    if (wg >= nb_schur) return;
+    schur_chunk<CNP, R, true>(wg, nullptr, nullptr, nullptr, nullptr, W, Vinv, spart, nullptr, nullptr, nchunk,
                               nullptr, rec, sBuf, sIdx, sMeta);
   } else {
+    const int b = 8 * eg + l8;
+    if (b >= ncc) return;
     int* const sO = (int*)(sBuf + CH * CNP);
-    ea_chunk<CNP, R>((int)blockIdx.x - nb_schur, nullptr, cb, ce, nullptr, W, z, epart, nullptr, nullptr, erec, sBuf,
-                     sO, sO + CH);
+    ea_chunk<CNP, R>(b, nullptr, cb, ce, nullptr, W, z, epart, nullptr, nullptr, erec, sBuf, sO, sO + CH);
   }
 }
 
@@ -379,11 +397,16 @@ static void run_schur_t(psba_ctx* c, double mu, const R* W, R* schpart, R* eapar
   // chunks, so they fill the CUs that the Schur launch's tail leaves idle),
   // then both finishes in one launch
   const int nb_schur = c->n_schunk > 0 ? (int)(PSBA_SCHUR_XCD ? xcd_grid(c->n_schunk) : c->n_schunk) : 0;
-  if (nb_schur + c->n_cchunk > 0) {
+  const int nsg = (nb_schur + 7) / 8, neg = (c->n_cchunk + 7) / 8;
+  if (nsg + neg > 0) {
+    // (with fewer Schur groups than e_a groups, period 1 and the surplus
+    // Schur slots of the interleaved region return at once)
+    const int period = neg > 0 ? std::max(1, nsg / neg) : 1;
+    const int ngroups = neg * (period + 1) + std::max(0, nsg - neg * period);
     probe_begin(c, PSBA_PROBE_SCHUR);
-    hipLaunchKernelGGL((k_schur_ea_chunks<CNP, R>), dim3(nb_schur + c->n_cchunk), dim3(256), 0, c->stream, W, c->Vinv,
-                       schpart, c->n_schunk, (const int4*)c->slot_items, nb_schur, c->cchunk_begin, c->cchunk_end,
-                       c->z, eapart, (const int2*)c->cchunk_rec);
+    hipLaunchKernelGGL((k_schur_ea_chunks<CNP, R>), dim3(8 * ngroups), dim3(256), 0, c->stream, W, c->Vinv,
+                       schpart, c->n_schunk, (const int4*)c->slot_items, nb_schur, neg, period, c->n_cchunk,
+                       c->cchunk_begin, c->cchunk_end, c->z, eapart, (const int2*)c->cchunk_rec);
     probe_end(c, PSBA_PROBE_SCHUR);
   }
   const int nfin = c->n_lblocks + (c->n_pad - c->n), bs = CNP * CNP > 128 ? 256 : 128;
