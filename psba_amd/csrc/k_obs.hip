@@ -142,6 +142,54 @@ int launch_residual(psba_ctx* c, bool trial) {
 // R is the storage (and accumulation) type of the normal-equation stages:
 // double on the plain path, float on the mixed path; the projection and its
 // derivatives are always evaluated in FP64.
+
+// One observation's rows in FP64: the residual e = m - f(p) and the
+// Jacobian blocks (fixed parameters' columns zeroed), reweighted by
+// sqrt(rho'(|e|^2)) under a robust loss.  k_jacobian and the camera chunks
+// that form U and g_a from it (without reading J_A back) both take the rows
+// from here, so they get the same values.
+template <int CNP>
+__device__ __forceinline__ void jac_row(int o, int j, int i, const double* __restrict__ K,
+                                        const double* __restrict__ q0, const double* __restrict__ cams,
+                                        const double* __restrict__ pts, const double* __restrict__ meas,
+                                        const unsigned char* __restrict__ fix, int loss, double loss_d,
+                                        double (&ja)[2 * CNP], double (&jb)[6], double& e0, double& e1) {
+  double cam[CNP], M[3], uv[2];
+#pragma unroll
+  for (int k = 0; k < CNP; ++k) cam[k] = cams[j * CNP + k];
+#pragma unroll
+  for (int k = 0; k < 3; ++k) M[k] = pts[3 * i + k];
+  project<double, CNP, true>(K + 5 * j, q0 + 4 * j, cam, M, uv, ja, jb);
+#pragma unroll
+  for (int k = 0; k < CNP; ++k)
+    if (fix[j * CNP + k]) ja[k] = ja[CNP + k] = 0.0;
+  e0 = meas[2 * o] - uv[0];
+  e1 = meas[2 * o + 1] - uv[1];
+  if (loss) {
+    // reweighted rows: residual and Jacobian scaled by sqrt(rho'(s))
+    double rho, drho;
+    loss_eval(loss, loss_d, e0 * e0 + e1 * e1, rho, drho);
+    const double w = sqrt(drho);
+#pragma unroll
+    for (int k = 0; k < 2 * CNP; ++k) ja[k] *= w;
+#pragma unroll
+    for (int k = 0; k < 6; ++k) jb[k] *= w;
+    e0 *= w;
+    e1 *= w;
+  }
+}
+
+// What jac_row reads, for kernels that recompute rows
+struct JacModel {
+  const double *K, *q0, *cams, *pts, *meas;
+  const int* obs_cam;
+  const unsigned char* fix;
+  int loss;
+  double loss_d;
+};
+
+// With write_ja false the J_A rows are not stored (the device-controlled LM
+// forms U and g_a from recomputed rows, k_normal_chunks with RC).
 template <int CNP, typename R>
 __global__ __launch_bounds__(256) void k_jacobian(const double* __restrict__ K, const double* __restrict__ q0,
                                                   const double* __restrict__ cams, const double* __restrict__ pts,
@@ -150,7 +198,7 @@ __global__ __launch_bounds__(256) void k_jacobian(const double* __restrict__ K, 
                                                   int no, double coeff, R* __restrict__ e, R* __restrict__ JA,
                                                   R* __restrict__ JB, R* __restrict__ W,
                                                   const int* __restrict__ ownr, const unsigned char* __restrict__ run,
-                                                  int loss, double loss_d) {
+                                                  int loss, double loss_d, int write_ja) {
   // The workgroup's rows of e, J_A, J_B and W go out through LDS: each thread
   // puts its row in the staging buffer (stride L + 1 against bank
   // conflicts), then the workgroup writes the block of rows as one contiguous
@@ -182,29 +230,8 @@ __global__ __launch_bounds__(256) void k_jacobian(const double* __restrict__ K, 
   // rows at once); only active threads compute and stage a row
   R jr[2 * CNP], br[6], er[2], wr[3 * CNP];
   if (act) {
-    int j = obs_cam[o], i = obs_pt[o];
-    double cam[CNP], M[3], uv[2], ja[2 * CNP], jb[6];
-#pragma unroll
-    for (int k = 0; k < CNP; ++k) cam[k] = cams[j * CNP + k];
-#pragma unroll
-    for (int k = 0; k < 3; ++k) M[k] = pts[3 * i + k];
-    project<double, CNP, true>(K + 5 * j, q0 + 4 * j, cam, M, uv, ja, jb);
-#pragma unroll
-    for (int k = 0; k < CNP; ++k)
-      if (fix[j * CNP + k]) ja[k] = ja[CNP + k] = 0.0;
-    double e0 = meas[2 * o] - uv[0], e1 = meas[2 * o + 1] - uv[1];
-    if (loss) {
-      // reweighted rows: residual and Jacobian scaled by sqrt(rho'(s))
-      double rho, drho;
-      loss_eval(loss, loss_d, e0 * e0 + e1 * e1, rho, drho);
-      const double w = sqrt(drho);
-#pragma unroll
-      for (int k = 0; k < 2 * CNP; ++k) ja[k] *= w;
-#pragma unroll
-      for (int k = 0; k < 6; ++k) jb[k] *= w;
-      e0 *= w;
-      e1 *= w;
-    }
+    double ja[2 * CNP], jb[6], e0, e1;
+    jac_row<CNP>(o, obs_cam[o], obs_pt[o], K, q0, cams, pts, meas, fix, loss, loss_d, ja, jb, e0, e1);
     er[0] = (R)e0;
     er[1] = (R)e1;
 #pragma unroll
@@ -218,8 +245,10 @@ __global__ __launch_bounds__(256) void k_jacobian(const double* __restrict__ K, 
   }
   put(2, er);
   flush(2, e);
-  put(2 * CNP, jr);
-  flush(2 * CNP, JA);
+  if (write_ja) {
+    put(2 * CNP, jr);
+    flush(2 * CNP, JA);
+  }
   put(6, br);
   flush(6, JB);
   put(3 * CNP, wr);
@@ -297,20 +326,24 @@ __global__ __launch_bounds__(256) void k_point_normal(const int* __restrict__ pt
 template <int CNP>
 constexpr int cam_chunk_obs() { return (CNP > 11) ? 128 : kCamChunk; }
 
-// chunk b, staging in sJ (cam_chunk_obs() (2 CNP + 2) values of LDS)
-template <int CNP, typename R>
+// chunk b, staging in sJ (cam_chunk_obs() (2 CNP + 2) values of LDS).
+// With RC (one problem, rec given) the rows are recomputed by jac_row from
+// the parameters in `mdl` rather than read from J_A and e: the same values
+// k_jacobian stores, rounded to R the same way.
+template <int CNP, typename R, bool RC = false>
 __device__ __forceinline__ void cam_chunk(int b, const int* __restrict__ cam_obs, const int* __restrict__ cb,
                                           const int* __restrict__ ce, const R* __restrict__ JA,
                                           const R* __restrict__ e, R* __restrict__ part,
                                           const int* __restrict__ ownr, const unsigned char* __restrict__ run,
-                                          const int2* __restrict__ rec, R* sJ) {
+                                          const int2* __restrict__ rec, R* sJ, const JacModel mdl = JacModel{}) {
   constexpr int NU = CNP * (CNP + 1) / 2, NE = NU + CNP, REC = 2 * CNP + 2;
   constexpr int CH = cam_chunk_obs<CNP>();
   static_assert(CH <= 256, "one observation per thread");
   // with rec (one problem) the observation comes from the chunk's record at
   // an address the workgroup index gives, loaded beside the chunk's extent
   // and the gate
-  const int o_rec = (rec && (int)threadIdx.x < CH) ? rec[(size_t)b * CH + threadIdx.x].x : -1;
+  const int2 r_rec = (rec && (int)threadIdx.x < CH) ? rec[(size_t)b * CH + threadIdx.x] : make_int2(-1, -1);
+  const int o_rec = r_rec.x;
   const int beg = cb[b], cnt = ce[b] - beg;
   if (gated_off(run, ownr, b)) return;
   // one thread per observation stages its Jacobian rows and residual (one
@@ -318,12 +351,22 @@ __device__ __forceinline__ void cam_chunk(int b, const int* __restrict__ cam_obs
   // flight together)
   for (int s = threadIdx.x; s < cnt; s += blockDim.x) {
     const int o = rec ? o_rec : cam_obs[beg + s];
-    const R* ja = JA + (size_t)o * 2 * CNP;
     R* dst = sJ + s * REC;
+    if constexpr (RC) {
+      double ja[2 * CNP], jb[6], e0, e1;
+      jac_row<CNP>(o, mdl.obs_cam[o], r_rec.y, mdl.K, mdl.q0, mdl.cams, mdl.pts, mdl.meas, mdl.fix, mdl.loss,
+                   mdl.loss_d, ja, jb, e0, e1);
 #pragma unroll
-    for (int el = 0; el < 2 * CNP; ++el) dst[el] = ja[el];
-    dst[2 * CNP] = e[2 * o];
-    dst[2 * CNP + 1] = e[2 * o + 1];
+      for (int el = 0; el < 2 * CNP; ++el) dst[el] = (R)ja[el];
+      dst[2 * CNP] = (R)e0;
+      dst[2 * CNP + 1] = (R)e1;
+    } else {
+      const R* ja = JA + (size_t)o * 2 * CNP;
+#pragma unroll
+      for (int el = 0; el < 2 * CNP; ++el) dst[el] = ja[el];
+      dst[2 * CNP] = e[2 * o];
+      dst[2 * CNP + 1] = e[2 * o + 1];
+    }
   }
   __syncthreads();
   // a quad per output: lane p of the quad sums observations p, p + 4, ..
@@ -369,8 +412,10 @@ __global__ __launch_bounds__(256) void k_cam_chunks(const int* __restrict__ cam_
 
 // One problem's point normal equations and camera chunks in one launch (both
 // read only the Jacobian stage's rows): workgroups [0, nb_pt) are
-// k_point_normal's, the rest k_cam_chunks' (the same per-workgroup work).
-template <int CNP, typename R>
+// k_point_normal's, the rest k_cam_chunks' (the same per-workgroup work;
+// with RC the camera chunks recompute their J_A rows, which k_jacobian then
+// does not store).
+template <int CNP, typename R, bool RC>
 __global__ __launch_bounds__(256) void k_normal_chunks(const int* __restrict__ pt_ptr, const R* __restrict__ JB,
                                                        const R* __restrict__ e, int np, double coeff, double coeff_g,
                                                        double* __restrict__ V, double* __restrict__ gb,
@@ -378,12 +423,12 @@ __global__ __launch_bounds__(256) void k_normal_chunks(const int* __restrict__ p
                                                        const int* __restrict__ cb, const int* __restrict__ ce,
                                                        const R* __restrict__ JA, R* __restrict__ part,
                                                        const int2* __restrict__ rec,
-                                                       const unsigned char* __restrict__ run) {
+                                                       const unsigned char* __restrict__ run, const JacModel mdl) {
   __shared__ R sJ[cam_chunk_obs<CNP>() * (2 * CNP + 2)];
   if ((int)blockIdx.x < nb_pt)
     point_normal_block<R>(blockIdx.x, pt_ptr, JB, e, np, coeff, coeff_g, V, gb, uvd_b, nullptr, run);
   else
-    cam_chunk<CNP, R>((int)blockIdx.x - nb_pt, nullptr, cb, ce, JA, e, part, nullptr, run, rec, sJ);
+    cam_chunk<CNP, R, RC>((int)blockIdx.x - nb_pt, nullptr, cb, ce, JA, e, part, nullptr, run, rec, sJ, mdl);
 }
 
 // stage 2: per camera, sum its chunk partials in chunk order (camera j).
@@ -512,16 +557,26 @@ template <int CNP, typename R>
 static void run_jac_normal_t(psba_ctx* c, double coeff, double coeff_g, R* e, R* JA, R* JB, R* W, R* ucpart,
                              bool with_vinv) {
   const unsigned char* gate = gate_of(c, kGateJac);
+  // under device control (the LM's trials) the camera chunks recompute the
+  // J_A rows, which are then not stored: 2 C values per observation neither
+  // written nor read back
+  const bool rc = c->devctl;
   probe_begin(c, PSBA_PROBE_JACOBIAN);
   hipLaunchKernelGGL((k_jacobian<CNP, R>), dim3(ceil_div(c->no, kJacBlock)), dim3(kJacBlock),
                      (jacobian_lds<CNP, R>(kJacBlock)), c->stream, c->K, c->q0,
                      c->cams, c->pts, c->meas, c->obs_cam, c->obs_pt, c->fix, c->no, coeff, e, JA, JB, W,
-                     (const int*)nullptr, gate, c->loss, c->loss_scale);
+                     (const int*)nullptr, gate, c->loss, c->loss_scale, rc ? 0 : 1);
   probe_end(c, PSBA_PROBE_JACOBIAN);
   const int nb_pt = (int)ceil_div(4L * c->np, 256);
-  hipLaunchKernelGGL((k_normal_chunks<CNP, R>), dim3(nb_pt + c->n_cchunk), dim3(256), 0, c->stream, c->pt_ptr, JB, e,
-                     c->np, coeff, coeff_g, c->V, c->g + c->n, c->uvdiag + c->n, nb_pt, c->cchunk_begin,
-                     c->cchunk_end, JA, ucpart, (const int2*)c->cchunk_rec, gate);
+  const JacModel mdl{c->K, c->q0, c->cams, c->pts, c->meas, c->obs_cam, c->fix, c->loss, c->loss_scale};
+  if (rc)
+    hipLaunchKernelGGL((k_normal_chunks<CNP, R, true>), dim3(nb_pt + c->n_cchunk), dim3(256), 0, c->stream, c->pt_ptr,
+                       JB, e, c->np, coeff, coeff_g, c->V, c->g + c->n, c->uvdiag + c->n, nb_pt, c->cchunk_begin,
+                       c->cchunk_end, JA, ucpart, (const int2*)c->cchunk_rec, gate, mdl);
+  else
+    hipLaunchKernelGGL((k_normal_chunks<CNP, R, false>), dim3(nb_pt + c->n_cchunk), dim3(256), 0, c->stream,
+                       c->pt_ptr, JB, e, c->np, coeff, coeff_g, c->V, c->g + c->n, c->uvdiag + c->n, nb_pt,
+                       c->cchunk_begin, c->cchunk_end, JA, ucpart, (const int2*)c->cchunk_rec, gate, mdl);
   if (with_vinv)
     hipLaunchKernelGGL((k_finish_vinv<CNP, R>), dim3(c->nc + ceil_div(c->np, 256)), dim3(256), 0, c->stream,
                        c->cam_cchunk_ptr, ucpart, c->nc, coeff, coeff_g, c->U, c->g, c->uvdiag, gate, c->V,
@@ -1148,7 +1203,7 @@ static void run_b_jac_t(psba_batch* b) {
   hipLaunchKernelGGL((k_jacobian<CNP, R>), dim3(ceil_div(b->NO, bs)), dim3(bs), (jacobian_lds<CNP, R>(bs)), b->stream,
                      b->K, b->q0, b->cams,
                      b->pts, b->meas, b->obs_cam, b->obs_pt, b->fix, (int)b->NO, 1.0, e, JA, JB, W, b->obs_prob,
-                     b->need_jac, PSBA_LOSS_NONE, 1.0);
+                     b->need_jac, PSBA_LOSS_NONE, 1.0, 1);
   hipLaunchKernelGGL(k_point_normal<R>, dim3(ceil_div(4L * b->NP, b->pt_block)), dim3(b->pt_block), 0, b->stream, b->pt_ptr, JB, e,
                      (int)b->NP, 1.0, 1.0, b->V, b->g + b->N, b->uvdiag + b->N, b->pt_prob, b->need_jac);
   if (b->n_cchunk > 0)
