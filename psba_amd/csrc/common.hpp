@@ -144,16 +144,6 @@ __global__ __launch_bounds__(256) void k_sum_partials(const double* __restrict__
   sum_partials_block<K>(partials, nb, out, sh);
 }
 
-// k_sum_partials that leaves `out` untouched unless *gate is set
-template <int K>
-__global__ __launch_bounds__(256) void k_sum_partials_gated(const double* __restrict__ partials, int nb,
-                                                            double* __restrict__ out,
-                                                            const unsigned char* __restrict__ gate) {
-  if (!*gate) return;
-  __shared__ double sh[K * 4];
-  sum_partials_block<K>(partials, nb, out, sh);
-}
-
 template <int K>
 inline void launch_sum_partials(hipStream_t st, const double* partials, int nb, double* out) {
   hipLaunchKernelGGL(k_sum_partials<K>, dim3(1), dim3(256), 0, st, partials, nb, out);

@@ -232,16 +232,6 @@ __device__ __forceinline__ void schur_finish_block(int b, const int* __restrict_
   }
 }
 
-template <int CNP, typename R>
-__global__ __launch_bounds__(256) void k_schur_finish(const int* __restrict__ lblock_pair,
-                                                      const int* __restrict__ psp, const R* __restrict__ part,
-                                                      const double* __restrict__ U,
-                                                      const unsigned char* __restrict__ fix, double mu,
-                                                      const double* __restrict__ mu_p, int ld, int nlb, int n,
-                                                      double* __restrict__ S) {
-  schur_finish_block<CNP, R>(blockIdx.x, lblock_pair, psp, part, U, fix, mu, mu_p, ld, nlb, n, S);
-}
-
 // e_a chunks over the camera chunks: p_o = W_o z_i, column sums
 template <int CNP>
 constexpr int ea_chunk_obs() { return (CNP > 11) ? 128 : kCamChunk; }
@@ -372,9 +362,9 @@ __global__ __launch_bounds__(64) void k_ea_finish(const int* __restrict__ ccp, c
   ea_finish_one<CNP, R>(blockIdx.x * blockDim.x + threadIdx.x, ccp, part, ga, nc, ea, ownr, run);
 }
 
-// k_schur_finish and k_ea_finish in one launch (one problem): workgroups
-// [0, nfin) are k_schur_finish's, the rest take blockDim e_a entries each
-// (the same per-entry sums as k_ea_finish)
+// The Schur finish and k_ea_finish in one launch (one problem): workgroups
+// [0, nfin) each write one lower block of S (schur_finish_block), the rest
+// take blockDim e_a entries each (the same per-entry sums as k_ea_finish)
 template <int CNP, typename R>
 __global__ __launch_bounds__(256) void k_schur_ea_finish(const int* __restrict__ lblock_pair,
                                                          const int* __restrict__ psp, const R* __restrict__ spart,
