@@ -125,23 +125,54 @@ __global__ __launch_bounds__(256) void k_lm_control(LmDev* s, unsigned char* gat
                                                     const double* __restrict__ p2_part, int nb_p2) {
   constexpr int NW = sizeof(LmDev) / 8;
   static_assert(sizeof(LmDev) % 8 == 0, "LmDev moves as 8-byte words");
-  __shared__ double sh[8];
+  constexpr int B = 8;
+  __shared__ double sh[16];
   __shared__ LmDev ls;
-  const int t = threadIdx.x;
+  const int t = threadIdx.x, T = blockDim.x;
   if (t < NW) ((unsigned long long*)&ls)[t] = ((const unsigned long long*)s)[t];
   TrialFig f{0.0, 0.0, 0.0, 0.0, 0.0};
   if (t == 0) f.factor_ok = scal[SC_FACTOR_OK];
   const bool active = gate[kGateActive];
-  double step[2], cost[1], p2[1] = {0.0};
-  sum_partials_block<2>(step_part, nb_step, scal + SC_DP2, sh, step);
-  sum_partials_block<1>(cost_part, nb_cost, scal + SC_COST, sh, cost);
-  // (ls is visible here: the sums above pass workgroup barriers)
-  if (active && ls.p_pending) sum_partials_block<1>(p2_part, nb_p2, scal + SC_P2, sh, p2);
+  const bool want_p2 = active && s->p_pending;
+  // The three sets of partials (|dp|^2 and the gain-ratio denominator, the
+  // cost, |p|^2) summed as sum_partials_block sums each set (thread t adds
+  // blocks t, t + T, .. in order, then the fixed-order block sum), with the
+  // loads of all three in flight together.
+  double acc[4] = {0.0, 0.0, 0.0, 0.0};
+  const int nmax = max(nb_step, max(nb_cost, want_p2 ? nb_p2 : 0));
+  for (int b0 = t; b0 < nmax; b0 += B * T) {
+    double vs[B][2], vc[B], vp[B];
+#pragma unroll
+    for (int u = 0; u < B; ++u) {
+      const int b = b0 + u * T;
+      const int bs = min(b, nb_step - 1), bc = min(b, nb_cost - 1);
+      vs[u][0] = step_part[(size_t)bs * 2];
+      vs[u][1] = step_part[(size_t)bs * 2 + 1];
+      vc[u] = cost_part[bc];
+      vp[u] = want_p2 ? p2_part[min(b, nb_p2 - 1)] : 0.0;
+    }
+#pragma unroll
+    for (int u = 0; u < B; ++u) {
+      const int b = b0 + u * T;
+      const bool ls_ = b < nb_step, lc = b < nb_cost, lp = want_p2 && b < nb_p2;
+      const double t0 = acc[0] + vs[u][0], t1 = acc[1] + vs[u][1], t2 = acc[2] + vc[u], t3 = acc[3] + vp[u];
+      acc[0] = ls_ ? t0 : acc[0];
+      acc[1] = ls_ ? t1 : acc[1];
+      acc[2] = lc ? t2 : acc[2];
+      acc[3] = lp ? t3 : acc[3];
+    }
+  }
+  block_sum<4>(acc, sh);
+  // (ls is visible here: the block sum passes workgroup barriers)
   if (t == 0) {
-    f.dp2 = step[0];
-    f.rhoden = step[1];
-    f.cost = cost[0];
-    f.p2 = p2[0];
+    scal[SC_DP2] = acc[0];
+    scal[SC_RHODEN] = acc[1];
+    scal[SC_COST] = acc[2];
+    if (want_p2) scal[SC_P2] = acc[3];
+    f.dp2 = acc[0];
+    f.rhoden = acc[1];
+    f.cost = acc[2];
+    f.p2 = acc[3];
     lm_decide(&ls, gate, f, trace);
   }
   __syncthreads();
