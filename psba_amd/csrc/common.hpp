@@ -177,11 +177,13 @@ __device__ __forceinline__ R quad_sum(R v) {
 }
 
 // Launch order of the Schur pair chunks: the chunks of camera pairs in the
-// same block of 8 x 8 cameras and the same range of 4096 points run together,
-// so the W rows they gather come back from L2 instead of HBM (with the XCD
-// mapping each XCD walks its own share of this order).  Sorting is stable, so
-// ties keep the chunk order.
-constexpr int kSchurTileCams = 8, kSchurPointRangeLog2 = 12;
+// same block of 24 x 24 cameras and the same range of 2048 points run
+// together, so the W rows they gather come back from L2 instead of HBM (with
+// the XCD mapping each XCD walks its own share of this order).  Sorting is
+// stable, so ties keep the chunk order.  (Measured on the 88-camera scene:
+// 8 x 8 cameras over 4096 points 254 us, 16 or 32 cameras over 1024-4096
+// points 249.5-251 us, 24 over 2048 248 us.)
+constexpr int kSchurTileCams = 24, kSchurPointRangeLog2 = 11;
 __host__ __device__ inline unsigned long long schunk_launch_key(int k, int l, int pt) {
   const unsigned long long K = (unsigned)k / kSchurTileCams, L = (unsigned)l / kSchurTileCams;
   return ((K * (K + 1) / 2 + L) << 32) | ((unsigned)pt >> kSchurPointRangeLog2);
