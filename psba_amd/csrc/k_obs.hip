@@ -15,7 +15,7 @@
 
 namespace psba {
 
-constexpr int kJacBlock = 128;  // threads per Jacobian workgroup (context launch)
+constexpr int kJacBlock = 64;  // threads per Jacobian workgroup (context launch; 64, 128 and 256 measured)
 
 // ---------------------------------------------------------------------------
 // residual and cost
