@@ -411,8 +411,8 @@ __global__ __launch_bounds__(256) void k_cam_chunks(const int* __restrict__ cam_
 }
 
 // One problem's point normal equations and camera chunks in one launch (both
-// read only the Jacobian stage's rows): workgroups [0, nb_pt) are
-// k_point_normal's, the rest k_cam_chunks' (the same per-workgroup work;
+// read only the Jacobian stage's rows): the last nb_pt workgroups are
+// k_point_normal's, the others k_cam_chunks' (the same per-workgroup work;
 // with RC the camera chunks recompute their J_A rows, which k_jacobian then
 // does not store).
 template <int CNP, typename R, bool RC>
@@ -425,10 +425,12 @@ __global__ __launch_bounds__(256) void k_normal_chunks(const int* __restrict__ p
                                                        const int2* __restrict__ rec,
                                                        const unsigned char* __restrict__ run, const JacModel mdl) {
   __shared__ R sJ[cam_chunk_obs<CNP>() * (2 * CNP + 2)];
-  if ((int)blockIdx.x < nb_pt)
-    point_normal_block<R>(blockIdx.x, pt_ptr, JB, e, np, coeff, coeff_g, V, gb, uvd_b, nullptr, run);
+  // the camera chunks first (the longer workgroups), then the points
+  const int ncc = (int)gridDim.x - nb_pt;
+  if ((int)blockIdx.x < ncc)
+    cam_chunk<CNP, R, RC>((int)blockIdx.x, nullptr, cb, ce, JA, e, part, nullptr, run, rec, sJ, mdl);
   else
-    cam_chunk<CNP, R, RC>((int)blockIdx.x - nb_pt, nullptr, cb, ce, JA, e, part, nullptr, run, rec, sJ, mdl);
+    point_normal_block<R>((int)blockIdx.x - ncc, pt_ptr, JB, e, np, coeff, coeff_g, V, gb, uvd_b, nullptr, run);
 }
 
 // stage 2: per camera, sum its chunk partials in chunk order (camera j).
