@@ -83,9 +83,9 @@ __device__ __forceinline__ void step_stats_block(const double* __restrict__ dp, 
 }
 
 // A device-controlled trial's step statistics and trial cost in one launch:
-// blocks [0, nb_step) are k_step_stats' blocks, the rest k_residual's (the
-// same per-block arithmetic, so the same partials), each set of partials
-// where the LM control kernel sums it.
+// the last nb_step blocks are k_step_stats' blocks, the others k_residual's
+// (the same per-block arithmetic, so the same partials), each set of
+// partials where the LM control kernel sums it.
 template <int CNP>
 __global__ __launch_bounds__(256) void k_trial_eval(const double* __restrict__ g, long ntot,
                                                     const double* __restrict__ mu_p, double* step_part, int nb_step,
@@ -96,12 +96,12 @@ __global__ __launch_bounds__(256) void k_trial_eval(const double* __restrict__ g
                                                     const int* __restrict__ obs_pt, int no, double* cost_part,
                                                     int loss, double loss_d) {
   __shared__ double sh[8];
-  const int b = blockIdx.x;
-  if (b < nb_step) {
-    step_stats_block(dp, g, ntot, 0.0, mu_p, step_part, b, sh);
+  // the residual blocks (the longer ones) first, then the step statistics
+  const int b = blockIdx.x, nb_cost = (int)gridDim.x - nb_step;
+  if (b < nb_cost) {
+    residual_block<CNP>(K, q0, cams, pts, dp, n, meas, obs_cam, obs_pt, no, nullptr, cost_part, loss, loss_d, b, sh);
   } else {
-    residual_block<CNP>(K, q0, cams, pts, dp, n, meas, obs_cam, obs_pt, no, nullptr, cost_part, loss, loss_d,
-                        b - nb_step, sh);
+    step_stats_block(dp, g, ntot, 0.0, mu_p, step_part, b - nb_cost, sh);
   }
 }
 
